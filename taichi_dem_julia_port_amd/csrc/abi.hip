@@ -1,0 +1,660 @@
+// abi.hip -- the extern "C" boundary of libdemhip (declared in include/demhip.h).
+//
+// dem_create performs the setup half of the reference's solve() (src/executor.jl:217-339):
+// per-grain derivations, sizing, the material/surface tables and the device allocation.
+// dem_initialize / dem_step replace initialize! / simulate! (src/executor.jl:155-172).
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "dem_internal.hpp"
+
+using namespace dem;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int32_t fail(dem_scene* s, int32_t code, const std::string& msg) {
+    if (s) s->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+int32_t hip_fail(dem_scene* s, const char* what, hipError_t e) {
+    return fail(s, DEM_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(s, call)                                      \
+    do {                                                     \
+        hipError_t e_ = (call);                              \
+        if (e_ != hipSuccess) return hip_fail(s, #call, e_); \
+    } while (0)
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e == hipSuccess) e = hipMemset(*p, 0, count * sizeof(T));
+    return e;
+}
+
+// executor.jl:223 r = cbrt(V / (4/3 pi))
+double grain_radius(double V) { return cbrt(V / (4.0 / 3.0 * M_PI)); }
+
+void free_scene(dem_scene* s) {
+    if (!s) return;
+    DevState& d = s->d;
+    void* ptrs[] = {d.pos[0], d.pos[1], d.vel[0], d.vel[1], d.omg[0], d.omg[1], d.quat[0], d.quat[1],
+                    d.acc, d.dw, d.ftot, d.ttot, d.r0, d.wact, d.wshear, d.hid, d.rank, d.bcnt,
+                    d.bstart, d.sidx, d.spos, d.spos32, d.tcnt, d.touch, d.sel, d.rcnt[0], d.rcnt[1],
+                    d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, d.scan.status, d.scan.counters,
+                    d.scratch_u32, d.scratch2_u32, d.total};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    if (s->h_ctr) hipHostFree(s->h_ctr);
+    for (int k = 0; k < 128; ++k)
+        if (s->ev[k]) hipEventDestroy(s->ev[k]);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete[] s->h_id; delete[] s->h_gid; delete[] s->h_mid; delete[] s->h_V; delete[] s->h_m;
+    delete[] s->h_r; delete[] s->h_r0; delete[] s->h_I;
+    delete s;
+}
+
+int32_t sync_counters(dem_scene* s) {
+    HIPCHK(s, hipMemcpyAsync(s->h_ctr, s->d.ctr, sizeof(unsigned long long) * CT_NCOUNTERS,
+                             hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    return DEM_OK;
+}
+
+int32_t collect_timing(dem_scene* s) {
+    if (!s->timing || s->nev == 0) return DEM_OK;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (int k = 0; k < s->nev; ++k) {
+        float ms = 0.f;
+        HIPCHK(s, hipEventElapsedTime(&ms, s->ev[2 * k], s->ev[2 * k + 1]));
+        s->kms[s->ev_kernel[k]] += ms;
+        s->klaunch[s->ev_kernel[k]] += 1;
+    }
+    s->nev = 0;
+    return DEM_OK;
+}
+
+// Reallocate empty contact rows with k slots per particle (before the first step only).
+int32_t resize_rows(dem_scene* s, int32_t k) {
+    DevState& d = s->d;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    void* old[] = {d.touch, d.pq[0], d.pq[1], d.st[0], d.st[1]};
+    for (void* p : old) if (p) HIPCHK(s, hipFree(p));
+    d.touch = nullptr; d.pq[0] = d.pq[1] = nullptr; d.st[0] = d.st[1] = nullptr;
+    const int64_t KN = (int64_t)k * s->n;
+    HIPCHK(s, dalloc(&d.touch, KN));
+    for (int b = 0; b < 2; ++b) {
+        HIPCHK(s, dalloc(&d.pq[b], KN));
+        HIPCHK(s, dalloc(&d.st[b], (int64_t)S_NCOMP * KN));
+        HIPCHK(s, hipMemset(d.rcnt[b], 0, s->n * sizeof(uint32_t)));
+    }
+    HIPCHK(s, hipMemset(d.sel, 0, s->n));
+    HIPCHK(s, hipMemset(d.tcnt, 0, s->n * sizeof(uint32_t)));
+    HIPCHK(s, hipMemset(d.ctr, 0, CT_NCOUNTERS * sizeof(unsigned long long)));
+    s->K = k;
+    s->P.K = k;
+    s->der.contact_slots = k;
+    return DEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t dem_abi_version(void) { return DEM_ABI_VERSION; }
+
+const char* dem_last_error(const dem_scene_t* s) {
+    return s ? s->err.c_str() : g_create_err.c_str();
+}
+
+const char* dem_kernel_name(int32_t k) {
+    static const char* names[DEM_NUM_KERNELS] = {"hash", "scan", "scatter", "broad_phase",
+                                                 "merge", "force_integrate", "init_bonds", "other"};
+    return (k >= 0 && k < DEM_NUM_KERNELS) ? names[k] : "";
+}
+
+int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwall,
+                   const dem_grain_in_t* g, int64_t n, const dem_options_t* opt,
+                   dem_scene_t** out) {
+    g_create_err.clear();
+    if (!out) return fail(nullptr, DEM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!cfg || !g) return fail(nullptr, DEM_ERR_INVALID, "cfg and grains must be non-NULL");
+    if (n <= 0 || n > (int64_t)IDX_MASK) return fail(nullptr, DEM_ERR_INVALID, "n out of range");
+    if (nwall < 0 || nwall > MAX_WALLS || (nwall > 0 && !walls))
+        return fail(nullptr, DEM_ERR_INVALID, "nwall must be in [0, 16] with a wall array");
+    if (!(cfg->dt > 0.0)) return fail(nullptr, DEM_ERR_INVALID, "dt must be positive");
+    dem_options_t o = {DEM_PRECISION_FP64, 0, 0, 0};
+    if (opt) o = *opt;
+    if (o.precision != DEM_PRECISION_FP64 && o.precision != DEM_PRECISION_MIXED)
+        return fail(nullptr, DEM_ERR_INVALID, "unknown precision");
+    // A row holds a particle's contacts in both roles (i and j), so the default is twice the
+    // reference's per-particle average budget (contacts = n * max_coordinate_number,
+    // executor.jl:277), at least 32.
+    int32_t K = o.contact_slots > 0 ? o.contact_slots
+                                    : std::min(128, std::max(32, 2 * cfg->max_coordinate_number));
+    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
+    for (int64_t i = 0; i < n; ++i)
+        if (!(g[i].V > 0.0) || !(g[i].m > 0.0))
+            return fail(nullptr, DEM_ERR_INVALID, "grain volume and mass must be positive");
+    for (int j = 0; j < nwall; ++j)
+        if (walls[j].mid < 1 || walls[j].mid > 2)
+            return fail(nullptr, DEM_ERR_INVALID, "wall material id must be 1 or 2");
+
+    dem_scene* s = new dem_scene();
+    s->n = n;
+    s->nwall = nwall;
+    s->precision = o.precision;
+    s->K = K;
+    s->device = o.device;
+    s->cfg = *cfg;
+    if (hipSetDevice(o.device) != hipSuccess) {
+        delete s;
+        return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
+    }
+
+    // ---- derivations (executor.jl:219-267) ------------------------------------------------
+    s->h_id = new int32_t[n]; s->h_gid = new int32_t[n]; s->h_mid = new int32_t[n];
+    s->h_V = new double[n]; s->h_m = new double[n]; s->h_r = new double[n];
+    s->h_r0 = new double[n]; s->h_I = new double[n];
+    double rmax_r = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double r = grain_radius(g[i].V);
+        s->h_id[i] = g[i].id; s->h_gid[i] = g[i].gid; s->h_mid[i] = 1;
+        s->h_V[i] = g[i].V; s->h_m[i] = g[i].m; s->h_r[i] = r;
+        s->h_r0[i] = r * cfg->particle_contact_radius_multiplier;
+        s->h_I[i] = 2.0 / 5.0 * g[i].m * (r * r);
+        if (i == 0 || r > rmax_r) rmax_r = r;
+    }
+    dem_derived_t& der = s->der;
+    der.r_max = rmax_r * cfg->particle_contact_radius_multiplier *
+                (1.0 + cfg->bond_tensile_strength / cfg->bond_elastic_modulus) *
+                cfg->neighboring_search_safety_factor;
+    der.cell_size = 4.0 * der.r_max;
+    der.material_density = g[n - 1].m / g[n - 1].V;
+    uint32_t prod = 1u;
+    for (int a = 0; a < 3; ++a)
+        prod *= (uint32_t)(int32_t)ceil((cfg->domain_max[a] - cfg->domain_min[a]) / der.cell_size);
+    der.grid_product_i32 = (int32_t)prod;
+    int64_t H = der.grid_product_i32;
+    H = std::min<int64_t>(std::max<int64_t>(H, 1 << 20), 1 << 22);
+    der.hash_table_size = H;
+    der.contact_slots = K;
+    der.total_steps = (int64_t)trunc(cfg->target_time / cfg->dt);
+    der.save_per_steps = (int64_t)trunc(cfg->saving_interval_time / cfg->dt);
+    if (!(der.cell_size > 0.0) || !std::isfinite(der.cell_size)) {
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_INVALID, "derived cell size is not positive and finite");
+    }
+
+    // ---- kernel parameters (executor.jl:283-315) --------------------------------------------
+    Params& P = s->P;
+    for (int a = 0; a < 3; ++a) P.grid.dmin[a] = cfg->domain_min[a];
+    P.grid.cs = der.cell_size;
+    P.grid.H = (uint64_t)H;
+    P.grid.Hmask = ((uint64_t)H & ((uint64_t)H - 1)) == 0 ? (uint64_t)H - 1 : 0;
+    P.n = n;
+    P.K = K;
+    P.nwall = nwall;
+    P.dt = cfg->dt;
+    P.tol = cfg->tolerance;
+    P.damping = cfg->global_damping;
+    for (int a = 0; a < 3; ++a) P.g[a] = cfg->gravity[a];
+    P.matE[0] = cfg->particle_elastic_modulus; P.matNu[0] = cfg->particle_poisson_ratio;
+    P.matE[1] = cfg->wall_elastic_modulus;     P.matNu[1] = cfg->wall_poisson_ratio;
+    auto beta_of = [](double e) { const double le = log(e); return le / sqrt(le * le + M_PI * M_PI); };
+    SurfaceP pp{cfg->pp_coefficient_friction, cfg->pp_coefficient_restitution,
+                beta_of(cfg->pp_coefficient_restitution), cfg->bond_radius_ratio,
+                cfg->bond_elastic_modulus, cfg->bond_poisson_ratio, cfg->bond_compressive_strength,
+                cfg->bond_tensile_strength, cfg->bond_shear_strength};
+    SurfaceP pw{cfg->pw_coefficient_friction, cfg->pw_coefficient_restitution,
+                beta_of(cfg->pw_coefficient_restitution), 0, 0, 0, 0, 0, 0};
+    P.surf[0][0] = pp; P.surf[0][1] = pw; P.surf[1][0] = pw; P.surf[1][1] = pw;
+    for (int j = 0; j < nwall; ++j) {
+        for (int a = 0; a < 3; ++a) P.walls[j].n[a] = walls[j].n[a];
+        P.walls[j].d = walls[j].d;
+        P.walls[j].mid = walls[j].mid;
+    }
+    P.margin_rel = 0.f;
+
+    // ---- device allocation -------------------------------------------------------------------
+    DevState& d = s->d;
+    const int64_t KN = (int64_t)K * n;
+    hipError_t e = hipSuccess;
+#define ALLOC(ptr, cnt)                                                                   \
+    if (e == hipSuccess) {                                                                \
+        e = dalloc(&(ptr), (size_t)(cnt));                                                \
+        if (e != hipSuccess) {                                                            \
+            std::string m_ = std::string("device allocation of " #ptr " failed: ") +     \
+                             hipGetErrorString(e);                                        \
+            free_scene(s);                                                                \
+            return fail(nullptr, DEM_ERR_HIP, m_);                                        \
+        }                                                                                 \
+    }
+    for (int b = 0; b < 2; ++b) {
+        ALLOC(d.pos[b], n); ALLOC(d.vel[b], n); ALLOC(d.omg[b], n); ALLOC(d.quat[b], n);
+        ALLOC(d.rcnt[b], n); ALLOC(d.pq[b], KN); ALLOC(d.st[b], (int64_t)S_NCOMP * KN);
+    }
+    ALLOC(d.acc, n); ALLOC(d.dw, n); ALLOC(d.ftot, n); ALLOC(d.ttot, n); ALLOC(d.r0, n);
+    ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall > 0 ? nwall : 1) * n);
+    ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.bcnt, H + 1); ALLOC(d.bstart, H + 1);
+    ALLOC(d.sidx, n); ALLOC(d.spos, n);
+    if (o.precision == DEM_PRECISION_MIXED) { ALLOC(d.spos32, n); }
+    ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
+    ALLOC(d.ctr, CT_NCOUNTERS);
+    const int64_t max_tiles = scan_tiles(std::max<int64_t>(H + 1, n + 1));
+    ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2);
+    d.scan.max_tiles = max_tiles;
+    ALLOC(d.scratch_u32, n + 1); ALLOC(d.scratch2_u32, n + 1); ALLOC(d.total, 1);
+#undef ALLOC
+    if (hipHostMalloc((void**)&s->h_ctr, sizeof(unsigned long long) * CT_NCOUNTERS) != hipSuccess) {
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_HIP, "hipHostMalloc failed");
+    }
+    memset(s->h_ctr, 0, sizeof(unsigned long long) * CT_NCOUNTERS);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_HIP, "hipStreamCreate failed");
+    }
+
+    // ---- initial state (executor.jl:229-242): q = 1, w = a = dw = 0 ---------------------------
+    std::vector<double4> hp(n), hv(n), hw(n), hq(n);
+    for (int64_t i = 0; i < n; ++i) {
+        hp[i] = make_double4(g[i].k[0], g[i].k[1], g[i].k[2], s->h_r[i]);
+        hv[i] = make_double4(g[i].v[0], g[i].v[1], g[i].v[2], g[i].m);
+        hw[i] = make_double4(0.0, 0.0, 0.0, s->h_I[i]);
+        hq[i] = make_double4(1.0, 0.0, 0.0, 0.0);
+    }
+    e = hipMemcpy(d.pos[0], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.vel[0], hv.data(), n * sizeof(double4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.omg[0], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.quat[0], hq.data(), n * sizeof(double4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.r0, s->h_r0, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        std::string m = std::string("initial upload failed: ") + hipGetErrorString(e);
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_HIP, m);
+    }
+    *out = s;
+    return DEM_OK;
+}
+
+void dem_destroy(dem_scene_t* s) {
+    if (!s) return;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    free_scene(s);
+}
+
+int32_t dem_get_derived(const dem_scene_t* s, dem_derived_t* out) {
+    if (!s || !out) return DEM_ERR_INVALID;
+    *out = s->der;
+    return DEM_OK;
+}
+
+int32_t dem_initialize(dem_scene_t* s) {
+    if (!s) return DEM_ERR_INVALID;
+    if (s->initialized || s->steps > 0)
+        return fail(s, DEM_ERR_STATE, "dem_initialize must be called once, before the first step");
+    HIPCHK(s, hipSetDevice(s->device));
+    while (true) {
+        HIPCHK(s, run_contact_detection(s, true));
+        int32_t rc = sync_counters(s);
+        if (rc) return rc;
+        if ((rc = collect_timing(s))) return rc;
+        if (!s->h_ctr[CT_OVERFLOW]) break;
+        // the rows are still empty before the first step: widen them and redo the bond pass
+        if (s->K >= 128)
+            return fail(s, DEM_ERR_CAPACITY, "more than 128 bonds on one particle during bond init");
+        if ((rc = resize_rows(s, std::min(128, 2 * s->K)))) return rc;
+    }
+    s->initialized = true;
+    return DEM_OK;
+}
+
+int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
+    if (!s || nsteps < 0) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    for (int64_t t = 0; t < nsteps; ++t) {
+        HIPCHK(s, run_contact_detection(s, false));
+        HIPCHK(s, run_force_integrate(s, t == nsteps - 1));
+        s->steps++;
+        if (s->timing) {
+            int32_t rc = collect_timing(s);
+            if (rc) return rc;
+        }
+    }
+    int32_t rc = sync_counters(s);
+    if (rc) return rc;
+    if (s->h_ctr[CT_OVERFLOW])
+        return fail(s, DEM_ERR_CAPACITY,
+                    "contact slots per particle exceeded (raise contact_slots / max_coordinate_number)");
+    return DEM_OK;
+}
+
+int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
+    if (!s || !out) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    int32_t rc = sync_counters(s);
+    if (rc) return rc;
+    const unsigned long long* c = s->h_ctr;
+    memset(out, 0, sizeof(*out));
+    out->step = s->steps;
+    out->n = s->n;
+    out->hash_table_size = s->der.hash_table_size;
+    out->candidate_pairs = (int64_t)c[CT_CAND];
+    if (s->steps == 0) {  // after initialize only: every created contact is a bond
+        out->contacts = (int64_t)c[CT_CREATED];
+        out->bonded = (int64_t)c[CT_CREATED];
+    } else {
+        out->contacts = (int64_t)c[CT_ACTIVE];
+        out->bonded = (int64_t)c[CT_ACTIVE_BOND];
+    }
+    out->hertz = out->contacts - out->bonded;
+    out->wall_contacts = (int64_t)c[CT_WALL];
+    out->created = (int64_t)c[CT_CREATED];
+    out->deactivated = (int64_t)c[CT_DEACT];
+    out->bonds_broken = (int64_t)c[CT_BROKEN];
+    out->bonds_broken_total = (int64_t)c[CT_BROKEN_TOTAL];
+    out->fatal_gap = (int64_t)c[CT_FATAL];
+    out->overflow = (int64_t)c[CT_OVERFLOW];
+    return DEM_OK;
+}
+
+int32_t dem_read_grains(dem_scene_t* s, dem_grain_t* out, int64_t n) {
+    if (!s || !out || n != s->n) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    std::vector<double4> p(n), v(n), w(n), q(n), a(n), dw(n);
+    const DevState& d = s->d;
+    const int c = s->cur;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(p.data(), d.pos[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(v.data(), d.vel[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(w.data(), d.omg[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(q.data(), d.quat[c], n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(a.data(), d.acc, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(dw.data(), d.dw, n * sizeof(double4), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) {
+        dem_grain_t& o = out[i];
+        memset(&o, 0, sizeof(o));
+        o.id = s->h_id[i]; o.gid = s->h_gid[i]; o.mid = s->h_mid[i];
+        o.V = s->h_V[i]; o.m = s->h_m[i]; o.r = s->h_r[i]; o.r0 = s->h_r0[i];
+        o.k[0] = p[i].x; o.k[1] = p[i].y; o.k[2] = p[i].z;
+        o.v[0] = v[i].x; o.v[1] = v[i].y; o.v[2] = v[i].z;
+        o.a[0] = a[i].x; o.a[1] = a[i].y; o.a[2] = a[i].z;
+        o.q[0] = q[i].x; o.q[1] = q[i].y; o.q[2] = q[i].z; o.q[3] = q[i].w;
+        o.w[0] = w[i].x; o.w[1] = w[i].y; o.w[2] = w[i].z;
+        o.dw[0] = dw[i].x; o.dw[1] = dw[i].y; o.dw[2] = dw[i].z;
+        o.I[0] = s->h_I[i]; o.I[4] = s->h_I[i]; o.I[8] = s->h_I[i];
+    }
+    return DEM_OK;
+}
+
+int32_t dem_write_grains(dem_scene_t* s, const dem_grain_t* in, int64_t n) {
+    if (!s || !in || n != s->n) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    std::vector<double4> p(n), v(n), w(n), q(n), a(n), dw(n);
+    for (int64_t i = 0; i < n; ++i) {
+        const dem_grain_t& g = in[i];
+        p[i] = make_double4(g.k[0], g.k[1], g.k[2], s->h_r[i]);
+        v[i] = make_double4(g.v[0], g.v[1], g.v[2], s->h_m[i]);
+        w[i] = make_double4(g.w[0], g.w[1], g.w[2], s->h_I[i]);
+        q[i] = make_double4(g.q[0], g.q[1], g.q[2], g.q[3]);
+        a[i] = make_double4(g.a[0], g.a[1], g.a[2], 0.0);
+        dw[i] = make_double4(g.dw[0], g.dw[1], g.dw[2], 0.0);
+    }
+    DevState& d = s->d;
+    const int c = s->cur;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(d.pos[c], p.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.vel[c], v.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.omg[c], w.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.quat[c], q.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.acc, a.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(d.dw, dw.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
+int32_t dem_read_contacts(dem_scene_t* s, dem_contact_t* out, uint8_t* bonded, int64_t cap,
+                          int64_t* count) {
+    if (!s || !count || cap < 0 || (cap > 0 && !out)) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    dem_contact_t* dout = nullptr;
+    uint8_t* dbd = nullptr;
+    int64_t total = 0;
+    HIPCHK(s, run_read_contacts(s, &dout, &dbd, &total));
+    const int64_t w = std::min(cap, total);
+    hipError_t e = hipSuccess;
+    if (w > 0) e = hipMemcpy(out, dout, w * sizeof(dem_contact_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && w > 0 && bonded) e = hipMemcpy(bonded, dbd, w, hipMemcpyDeviceToHost);
+    hipFree(dout);
+    hipFree(dbd);
+    if (e != hipSuccess) return hip_fail(s, "hipMemcpy(contacts)", e);
+    *count = total;
+    return DEM_OK;
+}
+
+int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_t* bonded,
+                           int64_t m) {
+    if (!s || m < 0 || (m > 0 && !in)) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    const int64_t n = s->n;
+    const int K = s->K;
+    struct Ent { uint32_t row, partner; int64_t src; };
+    std::vector<Ent> ents;
+    ents.reserve(2 * m);
+    for (int64_t c = 0; c < m; ++c) {
+        const int64_t i = in[c].i, j = in[c].j;
+        if (i < 1 || j < 1 || i > n || j > n || i >= j)
+            return fail(s, DEM_ERR_INVALID, "contacts need 1 <= i < j <= n");
+        ents.push_back({(uint32_t)(i - 1), (uint32_t)(j - 1), c});
+        ents.push_back({(uint32_t)(j - 1), (uint32_t)(i - 1), c});
+    }
+    std::sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
+        return a.row != b.row ? a.row < b.row : a.partner < b.partner;
+    });
+    std::vector<uint32_t> rc(n, 0);
+    std::vector<uint64_t> dst(ents.size());
+    std::vector<uint32_t> word(ents.size());
+    std::vector<double> state(ents.size() * S_NCOMP);
+    for (size_t k = 0; k < ents.size(); ++k) {
+        const Ent& en = ents[k];
+        if (k > 0 && ents[k - 1].row == en.row && ents[k - 1].partner == en.partner)
+            return fail(s, DEM_ERR_DUPLICATE, "duplicate contact pair");
+        const uint32_t slot = rc[en.row]++;
+        if ((int)slot >= K) return fail(s, DEM_ERR_CAPACITY, "contact slots per particle exceeded");
+        dst[k] = (uint64_t)slot * n + en.row;
+        const bool bd = bonded && bonded[en.src];
+        word[k] = en.partner | (bd ? BONDED : 0u);
+        const dem_contact_t& c = in[en.src];
+        double* sv = &state[k * S_NCOMP];
+        for (int a = 0; a < 3; ++a) {
+            sv[S_KX + a] = c.k[a]; sv[S_FX + a] = c.F[a]; sv[S_TIX + a] = c.ti[a];
+            sv[S_TJX + a] = c.tj[a]; sv[S_SX + a] = c.s[a];
+        }
+    }
+    DevState& d = s->d;
+    uint64_t* ddst = nullptr;
+    uint32_t* dword = nullptr;
+    double* dstate = nullptr;
+    const size_t me = std::max<size_t>(ents.size(), 1);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMalloc(&ddst, me * sizeof(uint64_t)));
+    HIPCHK(s, hipMalloc(&dword, me * sizeof(uint32_t)));
+    HIPCHK(s, hipMalloc(&dstate, me * S_NCOMP * sizeof(double)));
+    hipError_t e = hipMemcpy(ddst, dst.data(), ents.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dword, word.data(), ents.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dstate, state.data(), ents.size() * S_NCOMP * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.rcnt[0], rc.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(d.rcnt[1], 0, n * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(d.sel, 0, n);
+    if (e == hipSuccess) e = run_write_contacts(s, ddst, dword, dstate, (int64_t)ents.size());
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    hipFree(ddst); hipFree(dword); hipFree(dstate);
+    if (e != hipSuccess) return hip_fail(s, "dem_write_contacts", e);
+    return DEM_OK;
+}
+
+int32_t dem_read_wall_shear(dem_scene_t* s, double* out, int64_t n) {
+    if (!s || !out || n != s->n) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    const int nw = s->nwall;
+    if (nw == 0) return DEM_OK;
+    std::vector<double4> ws((size_t)nw * n);
+    std::vector<uint32_t> act(n);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(ws.data(), s->d.wshear, ws.size() * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(act.data(), s->d.wact, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i)
+        for (int j = 0; j < nw; ++j) {
+            const bool on = (act[i] >> j) & 1u;
+            const double4 v = ws[(size_t)j * n + i];
+            double* o = &out[(i * nw + j) * 3];
+            o[0] = on ? v.x : 0.0; o[1] = on ? v.y : 0.0; o[2] = on ? v.z : 0.0;
+        }
+    return DEM_OK;
+}
+
+int32_t dem_write_wall_shear(dem_scene_t* s, const double* in, int64_t n) {
+    if (!s || !in || n != s->n) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    const int nw = s->nwall;
+    if (nw == 0) return DEM_OK;
+    std::vector<double4> ws((size_t)nw * n);
+    std::vector<uint32_t> act(n, 0u);
+    for (int64_t i = 0; i < n; ++i)
+        for (int j = 0; j < nw; ++j) {
+            const double* v = &in[(i * nw + j) * 3];
+            ws[(size_t)j * n + i] = make_double4(v[0], v[1], v[2], 0.0);
+            if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) act[i] |= 1u << j;
+        }
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(s->d.wshear, ws.data(), ws.size() * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, hipMemcpy(s->d.wact, act.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
+int32_t dem_read_forces(dem_scene_t* s, double* F, double* T, int64_t n) {
+    if (!s || n != s->n) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    std::vector<double4> f(n), t(n);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(f.data(), s->d.ftot, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(t.data(), s->d.ttot, n * sizeof(double4), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < n; ++i) {
+        if (F) { F[3 * i] = f[i].x; F[3 * i + 1] = f[i].y; F[3 * i + 2] = f[i].z; }
+        if (T) { T[3 * i] = t[i].x; T[3 * i + 1] = t[i].y; T[3 * i + 2] = t[i].z; }
+    }
+    return DEM_OK;
+}
+
+int32_t dem_debug_cells(dem_scene_t* s, int32_t* kid, uint32_t* hid, double* kcenter,
+                        uint32_t* nbr) {
+    if (!s) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    const int64_t n = s->n;
+    int32_t* dk = nullptr; uint32_t* dh = nullptr; double* dc = nullptr; uint32_t* dn = nullptr;
+    HIPCHK(s, hipMalloc(&dk, 3 * n * sizeof(int32_t)));
+    HIPCHK(s, hipMalloc(&dh, n * sizeof(uint32_t)));
+    HIPCHK(s, hipMalloc(&dc, 3 * n * sizeof(double)));
+    HIPCHK(s, hipMalloc(&dn, 8 * n * sizeof(uint32_t)));
+    hipError_t e = run_debug_cells(s, dk, dh, dc, dn);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess && kid) e = hipMemcpy(kid, dk, 3 * n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && hid) e = hipMemcpy(hid, dh, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && kcenter) e = hipMemcpy(kcenter, dc, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && nbr) e = hipMemcpy(nbr, dn, 8 * n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    hipFree(dk); hipFree(dh); hipFree(dc); hipFree(dn);
+    if (e != hipSuccess) return hip_fail(s, "dem_debug_cells", e);
+    return DEM_OK;
+}
+
+int32_t dem_debug_hash_table(dem_scene_t* s, uint32_t* counts, uint32_t* start, uint32_t* pid) {
+    if (!s) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    HIPCHK(s, run_debug_sort(s));
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    const int64_t H = s->der.hash_table_size, n = s->n;
+    std::vector<uint32_t> bs(H + 1), si(n);
+    HIPCHK(s, hipMemcpy(bs.data(), s->d.bstart, (H + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(s, hipMemcpy(si.data(), s->d.sidx, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (int64_t b = 0; b < H; ++b) {
+        if (counts) counts[b] = bs[b + 1] - bs[b];
+        if (start) start[b] = bs[b];
+    }
+    if (pid)
+        for (int64_t k = 0; k < n; ++k) pid[k] = si[k] + 1u;
+    return DEM_OK;
+}
+
+int32_t dem_debug_cp_list(dem_scene_t* s, int32_t* pairs, int64_t cap, int64_t* count) {
+    if (!s || !count || cap < 0 || (cap > 0 && !pairs)) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    int32_t* dp = nullptr;
+    int64_t total = 0;
+    HIPCHK(s, run_debug_cp_list(s, &dp, &total));
+    const int64_t w = std::min(cap, total);
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (e == hipSuccess && w > 0) e = hipMemcpy(pairs, dp, 2 * w * sizeof(int32_t), hipMemcpyDeviceToHost);
+    hipFree(dp);
+    if (e != hipSuccess) return hip_fail(s, "dem_debug_cp_list", e);
+    *count = total;
+    return DEM_OK;
+}
+
+int32_t dem_debug_scan(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* total) {
+    if (n < 0 || (n > 0 && (!in || !out))) return DEM_ERR_INVALID;
+    if (n == 0) { if (total) *total = 0; return DEM_OK; }
+    ScanWorkspace ws;
+    ws.max_tiles = scan_tiles(n);
+    uint32_t *din = nullptr, *dout = nullptr;
+    uint64_t* dtot = nullptr;
+    hipError_t e = dalloc(&ws.status, ws.max_tiles);
+    if (e == hipSuccess) e = dalloc(&ws.counters, 2);
+    if (e == hipSuccess) e = dalloc(&din, n);
+    if (e == hipSuccess) e = dalloc(&dout, n);
+    if (e == hipSuccess) e = dalloc(&dtot, 1);
+    if (e == hipSuccess) e = hipMemcpy(din, in, n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_scan(ws, din, dout, n, false, dtot, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dout, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    uint64_t t = 0;
+    if (e == hipSuccess) e = hipMemcpy(&t, dtot, sizeof(uint64_t), hipMemcpyDeviceToHost);
+    hipFree(ws.status); hipFree(ws.counters); hipFree(din); hipFree(dout); hipFree(dtot);
+    if (e != hipSuccess) { g_create_err = std::string("dem_debug_scan: ") + hipGetErrorString(e); return DEM_ERR_HIP; }
+    if (total) *total = t;
+    return DEM_OK;
+}
+
+int32_t dem_set_timing(dem_scene_t* s, int32_t on) {
+    if (!s) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    if (on && !s->ev[0])
+        for (int k = 0; k < 128; ++k) HIPCHK(s, hipEventCreate(&s->ev[k]));
+    s->timing = on != 0;
+    s->nev = 0;
+    for (int k = 0; k < DEM_NUM_KERNELS; ++k) { s->kms[k] = 0.0; s->klaunch[k] = 0; }
+    return DEM_OK;
+}
+
+int32_t dem_get_kernel_times(dem_scene_t* s, dem_kernel_times_t* out) {
+    if (!s || !out) return DEM_ERR_INVALID;
+    int32_t rc = collect_timing(s);
+    if (rc) return rc;
+    for (int k = 0; k < DEM_NUM_KERNELS; ++k) { out->ms[k] = s->kms[k]; out->launches[k] = s->klaunch[k]; }
+    return DEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,161 @@
+// dem_internal.hpp -- scene state and kernel launchers shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/demhip.h"
+#include "dem_device.hpp"
+
+namespace dem {
+
+constexpr int MAX_WALLS = 16;
+constexpr uint32_t IDX_MASK = 0x3fffffffu;  // partner index bits of a row entry
+constexpr uint32_t BONDED = 1u << 30;       // entry is an EBPM bond
+constexpr uint32_t BROKEN = 1u << 31;       // bond broke this step; dropped by the next merge
+
+// Row-entry state components (each an array of contact_slots * n doubles per buffer).
+enum StateComp { S_KX = 0, S_KY, S_KZ, S_FX, S_FY, S_FZ, S_TIX, S_TIY, S_TIZ,
+                 S_TJX, S_TJY, S_TJZ, S_SX, S_SY, S_SZ, S_NCOMP };
+
+enum CounterIdx {
+    CT_CAND = 0,     // candidate pairs i<j over the 8-bucket windows (cp_list length)
+    CT_TOUCH,        // touching pairs found by the broad phase
+    CT_ENTRIES,      // row entries after merge (both roles)
+    CT_CREATED,      // contacts created
+    CT_DEACT,        // Hertz contacts deactivated
+    CT_BROKEN,       // bonds broken
+    CT_WALL,         // wall contacts
+    CT_FATAL,        // gap > 0 or negative damping coefficient evaluations
+    CT_ACTIVE,       // active contacts after the step (counted once per pair)
+    CT_ACTIVE_BOND,  // active bonds after the step
+    CT_NSTEP,        // counters below are reset at the start of every step
+    CT_BROKEN_TOTAL = 12,  // cumulative
+    CT_OVERFLOW = 13,      // sticky: a capacity overflow invalidates the scene
+    CT_NCOUNTERS = 16
+};
+
+struct SurfaceP {
+    double mu, e, beta;          // friction, restitution, log(e)/sqrt(log(e)^2 + pi^2)
+    double rho, E, nu;           // bond radius ratio, modulus, Poisson ratio
+    double sc, st, ss;           // bond strengths
+};
+
+struct WallP { double n[3]; double d; int32_t mid; int32_t pad; };
+
+// Everything a kernel needs by value.
+struct Params {
+    HashGrid grid;
+    int64_t n;
+    int32_t K;       // contact slots per particle
+    int32_t nwall;
+    double dt, tol, damping;
+    double g[3];
+    double matE[2], matNu[2];
+    SurfaceP surf[2][2];
+    WallP walls[MAX_WALLS];
+    float margin_rel;  // FP32 candidate filter slack (mixed precision)
+};
+
+struct ScanWorkspace {
+    uint64_t* status = nullptr;
+    uint32_t* counters = nullptr;
+    int64_t max_tiles = 0;
+};
+
+int64_t scan_tiles(int64_t n);
+hipError_t launch_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n, bool zero_input,
+                       uint64_t* total, hipStream_t stream);
+
+// Device arrays of one scene.
+struct DevState {
+    // particle state, double-buffered (index cur / 1-cur)
+    double4* pos[2] = {nullptr, nullptr};   // x, y, z, r
+    double4* vel[2] = {nullptr, nullptr};   // vx, vy, vz, m
+    double4* omg[2] = {nullptr, nullptr};   // wx, wy, wz, I (isotropic inertia)
+    double4* quat[2] = {nullptr, nullptr};  // w, x, y, z
+    double4* acc = nullptr;                 // a (output of update!)
+    double4* dw = nullptr;                  // dω (output of update!)
+    double4* ftot = nullptr;                // force of the last step
+    double4* ttot = nullptr;                // moment of the last step
+    double* r0 = nullptr;
+    uint32_t* wact = nullptr;               // bit w: particle touched wall w last step
+    double4* wshear = nullptr;              // [nwall][n] shear displacement
+    // cell hashing / counting sort
+    uint32_t* hid = nullptr;     // 0-based bucket
+    uint32_t* rank = nullptr;    // slot within the bucket
+    uint32_t* bcnt = nullptr;    // [H + 1] bucket counts (kept zero between steps)
+    uint32_t* bstart = nullptr;  // [H + 1] exclusive bucket starts
+    uint32_t* sidx = nullptr;    // sorted slot -> particle
+    double4* spos = nullptr;     // sorted copy of pos
+    float4* spos32 = nullptr;    // sorted FP32 copy (mixed precision)
+    // broad phase output
+    uint32_t* tcnt = nullptr;    // [n]
+    uint32_t* touch = nullptr;   // [K][n]
+    // contact rows, double-buffered per particle (sel[p])
+    uint8_t* sel = nullptr;
+    uint32_t* rcnt[2] = {nullptr, nullptr};
+    uint32_t* pq[2] = {nullptr, nullptr};   // [K][n] partner | flags
+    double* st[2] = {nullptr, nullptr};     // [S_NCOMP][K][n]
+    // counters
+    unsigned long long* ctr = nullptr;
+    // scan workspace and scratch
+    ScanWorkspace scan;
+    uint32_t* scratch_u32 = nullptr;  // [n + 1]
+    uint32_t* scratch2_u32 = nullptr; // [n + 1]
+    uint64_t* total = nullptr;
+};
+
+}  // namespace dem
+
+struct dem_scene {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0;
+    int32_t nwall = 0;
+    int32_t precision = 0;
+    int32_t K = 0;
+    int cur = 0;
+    bool initialized = false;
+    int64_t steps = 0;
+    uint64_t broken_total_host = 0;
+    dem_config_t cfg{};
+    dem_derived_t der{};
+    dem::Params P{};
+    dem::DevState d{};
+    // host copies of static grain fields
+    int32_t* h_id = nullptr;
+    int32_t* h_gid = nullptr;
+    int32_t* h_mid = nullptr;
+    double* h_V = nullptr;
+    double* h_m = nullptr;
+    double* h_r = nullptr;
+    double* h_r0 = nullptr;
+    double* h_I = nullptr;
+    unsigned long long* h_ctr = nullptr;  // pinned
+    std::string err;
+    // timing
+    bool timing = false;
+    hipEvent_t ev[2 * 64] = {};
+    int ev_kernel[64] = {};
+    int nev = 0;
+    double kms[DEM_NUM_KERNELS] = {};
+    int64_t klaunch[DEM_NUM_KERNELS] = {};
+};
+
+namespace dem {
+// kernel launchers (kernels.hip)
+enum KernelId { K_HASH = 0, K_SCAN, K_SCATTER, K_BROAD, K_MERGE, K_FORCE, K_INIT, K_OTHER };
+hipError_t run_contact_detection(dem_scene* s, bool init);
+hipError_t run_force_integrate(dem_scene* s, bool write_forces);
+hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc, uint32_t* nbr);
+hipError_t run_debug_sort(dem_scene* s);
+hipError_t run_debug_cp_list(dem_scene* s, int32_t** pairs_dev, int64_t* count);
+hipError_t run_read_contacts(dem_scene* s, dem_contact_t** out_dev, uint8_t** bonded_dev,
+                             int64_t* count);
+hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint32_t* word_dev,
+                              const double* state_dev, int64_t m);
+void timing_begin(dem_scene* s, int kernel);
+void timing_end(dem_scene* s);
+}  // namespace dem

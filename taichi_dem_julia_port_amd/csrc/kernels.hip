@@ -1,0 +1,824 @@
+// kernels.hip -- the DEM step as gfx950 kernels.
+//
+// One step of the reference (simulate!, src/executor.jl:162-172) becomes six launches with no
+// host synchronisation:
+//   k_hash     cell + Morton bucket per particle, bucket histogram (kernels.jl:7-13, utils.jl:1-2)
+//   scan       exclusive bucket starts (accumulate!, executor.jl:50)
+//   k_scatter  counting-sort scatter into a sorted mirror of positions (kernels.jl:15-22)
+//   k_broad    per-particle walk over the 8 window buckets (kernels.jl:24-58) that keeps only
+//              touching pairs i<j (the test of kernels.jl:103 / :72) and appends each to the
+//              rows of both particles
+//   k_merge    per-particle merge of the new touching set with the particle's persistent
+//              contact row -- the BitTable state machine of update_contacts! (kernels.jl:87-120)
+//   k_force    per-particle loop over its row: Hertz-Mindlin / EBPM (kernels.jl:122-315), walls
+//              (kernels.jl:317-426), body force (kernels.jl:428-443) and update! (445-485)
+//
+// Contact state lives in per-particle rows (column-major [slot][particle] arrays, sorted by
+// partner index).  Every contact appears in the rows of both its particles and both copies are
+// advanced with bit-identical arithmetic in the canonical (i<j) orientation, so each particle
+// sums its own forces without atomics and the sum order is deterministic.
+#include "dem_internal.hpp"
+
+#include <float.h>
+
+namespace dem {
+
+namespace {
+
+constexpr int TPB = 256;
+constexpr int MERGE_TPB = 128;
+
+__device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
+
+// ------------------------------------------------------------------------------------------
+// hashing and counting sort
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB)
+k_hash(const Params P, const double4* __restrict__ pos, uint32_t* __restrict__ hid,
+       uint32_t* __restrict__ rank, uint32_t* __restrict__ bcnt, unsigned long long* ctr,
+       int reset_counters) {
+    if (reset_counters && blockIdx.x == 0 && threadIdx.x < CT_NSTEP) ctr[threadIdx.x] = 0ull;
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= P.n) return;
+    const double4 x = pos[p];
+    int32_t c[3];
+    cell_of(P.grid, mk3(x.x, x.y, x.z), c);
+    const uint32_t b = bucket_of(P.grid, c[0], c[1], c[2]);
+    hid[p] = b;
+    rank[p] = atomicAdd(&bcnt[b], 1u);
+}
+
+__global__ void __launch_bounds__(TPB)
+k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ hid,
+          const uint32_t* __restrict__ rank, const uint32_t* __restrict__ bstart,
+          uint32_t* __restrict__ sidx, double4* __restrict__ spos, float4* __restrict__ spos32) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= P.n) return;
+    const uint32_t slot = bstart[hid[p]] + rank[p];
+    const double4 x = pos[p];
+    sidx[slot] = (uint32_t)p;
+    spos[slot] = x;
+    if (spos32) spos32[slot] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
+}
+
+// ------------------------------------------------------------------------------------------
+// broad phase: touching pairs of the 8-bucket window
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ void row_append(uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
+                                           int64_t n, int K, uint32_t row, uint32_t partner,
+                                           uint32_t& overflow) {
+    const uint32_t m = atomicAdd(&tcnt[row], 1u);
+    if ((int)m < K) touch[(int64_t)m * n + row] = partner;
+    else overflow = 1;
+}
+
+template <bool INIT, bool MIXED>
+__global__ void __launch_bounds__(TPB)
+k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
+        const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
+        const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
+        unsigned long long* ctr) {
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint64_t cand = 0, found = 0;
+    uint32_t overflow = 0;
+    if (t < P.n) {
+        const uint32_t p = sidx[t];
+        const double4 me = spos[t];
+        const d3 xp = mk3(me.x, me.y, me.z);
+        const double rp = INIT ? r0[p] : me.w;
+        float4 me32 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float slack = 0.f;
+        if (MIXED) {
+            me32 = spos32[t];
+            slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
+        }
+        uint32_t w[8];
+        window_of(P.grid, xp, w);
+#pragma unroll 1
+        for (int wi = 0; wi < 8; ++wi) {
+            const uint32_t b = w[wi];
+            bool dup = false;
+            for (int wj = 0; wj < wi; ++wj) dup |= (w[wj] == b);
+            const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
+            for (uint32_t k = s0; k < s1; ++k) {
+                const uint32_t q = sidx[k];
+                if (q <= p) continue;
+                ++cand;
+                if (dup) continue;  // an aliased window bucket repeats the same candidates
+                if (MIXED) {
+                    const float4 o = spos32[k];
+                    const float dx = me32.x - o.x, dy = me32.y - o.y, dz = me32.z - o.z;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    const float rs = me32.w + o.w;
+                    const float thr = rs * (1.0f + 64.0f * FLT_EPSILON) + slack;
+                    if (d2 > thr * thr) continue;
+                }
+                const double4 o = spos[k];
+                const double rq = INIT ? r0[q] : o.w;
+                // kernels.jl:103 (norm(k_i - k_j) < r_i + r_j), kernels.jl:72 with r0 at t = 0
+                if (norm(xp - mk3(o.x, o.y, o.z)) < rp + rq) {
+                    ++found;
+                    row_append(tcnt, touch, P.n, P.K, p, q, overflow);
+                    row_append(tcnt, touch, P.n, P.K, q, p, overflow);
+                }
+            }
+        }
+    }
+    wave_add(&ctr[CT_CAND], cand);
+    wave_add(&ctr[CT_TOUCH], found);
+    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+}
+
+// ------------------------------------------------------------------------------------------
+// merge: persistent contact rows vs the new touching set (update_contacts! / init_bonds!)
+// ------------------------------------------------------------------------------------------
+
+// Is hi's bucket one of lo's 8 window buckets (the pair is in the reference's cp_list)?
+__device__ __forceinline__ bool in_window(const Params& P, const double4* __restrict__ pos,
+                                          const uint32_t* __restrict__ hid, uint32_t lo, uint32_t hi) {
+    const double4 x = pos[lo];
+    uint32_t w[8];
+    window_of(P.grid, mk3(x.x, x.y, x.z), w);
+    const uint32_t b = hid[hi];
+    bool in = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) in |= (w[k] == b);
+    return in;
+}
+
+template <bool INIT>
+__global__ void __launch_bounds__(MERGE_TPB)
+k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ hid,
+        uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ touch, uint8_t* __restrict__ sel,
+        uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1, uint32_t* __restrict__ pq0,
+        uint32_t* __restrict__ pq1, double* __restrict__ st0, double* __restrict__ st1,
+        unsigned long long* ctr) {
+    extern __shared__ uint32_t lds[];
+    const int64_t p64 = (int64_t)blockIdx.x * MERGE_TPB + threadIdx.x;
+    const int64_t n = P.n;
+    const int K = P.K;
+    uint64_t created = 0, deact = 0, entries = 0;
+    uint32_t overflow = 0;
+#define LT(m) lds[(m) * MERGE_TPB + threadIdx.x]
+    if (p64 < n) {
+        const uint32_t p = (uint32_t)p64;
+        uint32_t nt = tcnt[p];
+        if (nt) tcnt[p] = 0u;
+        if ((int)nt > K) { overflow = 1; nt = K; }
+        for (uint32_t m = 0; m < nt; ++m) LT(m) = touch[(int64_t)m * n + p];
+        // insertion sort + de-duplication of the new touching partners
+        for (uint32_t a = 1; a < nt; ++a) {
+            const uint32_t v = LT(a);
+            int b = (int)a - 1;
+            while (b >= 0 && LT(b) > v) { LT(b + 1) = LT(b); --b; }
+            LT(b + 1) = v;
+        }
+        uint32_t nu = 0;
+        for (uint32_t a = 0; a < nt; ++a)
+            if (nu == 0 || LT(a) != LT(nu - 1)) { LT(nu) = LT(a); ++nu; }
+
+        const int b = sel[p];
+        const uint32_t* pqo = b ? pq1 : pq0;
+        const uint32_t nold = (b ? rcnt1 : rcnt0)[p];
+        const uint32_t INF = 0xffffffffu;
+
+        // pass 1: decide the new row
+        uint32_t io = 0, in = 0, nout = 0;
+        bool changed = false;
+        while (io < nold || in < nu) {
+            const uint32_t eo = io < nold ? pqo[(int64_t)io * n + p] : INF;
+            const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
+            const uint32_t qn = in < nu ? LT(in) : INF;
+            if (qo < qn) {  // only in the old row
+                ++io;
+                if (eo & BROKEN) { changed = true; continue; }
+                if ((eo & BONDED) || INIT) { ++nout; continue; }
+                const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
+                if (in_window(P, pos, hid, lo, hi)) {  // in cp_list and not touching: deactivate
+                    changed = true;
+                    if (p < qo) ++deact;
+                } else {
+                    ++nout;  // not a candidate this step: untouched (kernels.jl:97-119)
+                }
+            } else if (qn < qo) {  // only in the new set: append a zeroed contact
+                ++in; ++nout; changed = true;
+                if (p < qn) ++created;
+            } else {  // in both
+                ++io; ++in; ++nout;
+                if (eo & BROKEN) {  // bond removed at the end of last step, re-created as Hertz
+                    changed = true;
+                    if (p < qn) ++created;
+                } else if (INIT && !(eo & BONDED)) {
+                    changed = true;
+                }
+            }
+        }
+        if ((int)nout > K) { overflow = 1; }
+        entries = nout;
+        if (changed && (int)nout <= K) {
+            const int nb = 1 - b;
+            uint32_t* pqn = nb ? pq1 : pq0;
+            const double* so = b ? st1 : st0;
+            double* sn = nb ? st1 : st0;
+            const int64_t cs = (int64_t)K * n;  // component stride
+            io = 0; in = 0;
+            uint32_t o = 0;
+            while (io < nold || in < nu) {
+                const uint32_t eo = io < nold ? pqo[(int64_t)io * n + p] : INF;
+                const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
+                const uint32_t qn = in < nu ? LT(in) : INF;
+                const int64_t dst = (int64_t)o * n + p;
+                if (qo < qn) {
+                    const int64_t src = (int64_t)io * n + p;
+                    ++io;
+                    if (eo & BROKEN) continue;
+                    if (!(eo & BONDED) && !INIT) {
+                        const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
+                        if (in_window(P, pos, hid, lo, hi)) continue;
+                    }
+                    pqn[dst] = eo;
+#pragma unroll
+                    for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
+                    ++o;
+                } else if (qn < qo) {
+                    ++in;
+                    pqn[dst] = qn | (INIT ? BONDED : 0u);
+#pragma unroll
+                    for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+                    ++o;
+                } else {
+                    const int64_t src = (int64_t)io * n + p;
+                    ++io; ++in;
+                    if (eo & BROKEN) {
+                        pqn[dst] = qn;
+#pragma unroll
+                        for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+                    } else if (INIT && !(eo & BONDED)) {
+                        pqn[dst] = eo | BONDED;
+#pragma unroll
+                        for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+                    } else {
+                        pqn[dst] = eo;
+#pragma unroll
+                        for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
+                    }
+                    ++o;
+                }
+            }
+            (nb ? rcnt1 : rcnt0)[p] = o;
+            sel[p] = (uint8_t)nb;
+        }
+    }
+#undef LT
+    wave_add(&ctr[CT_CREATED], created);
+    wave_add(&ctr[CT_DEACT], deact);
+    wave_add(&ctr[CT_ENTRIES], entries);
+    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+}
+
+// ------------------------------------------------------------------------------------------
+// forces + integration
+// ------------------------------------------------------------------------------------------
+struct HMOut { d3 F; d3 s; bool fatal; };
+
+// Hertz-Mindlin law (kernels.jl:261-303, 369-406).
+__device__ __forceinline__ HMOut hertz_mindlin(double gap, double dn, d3 vc, double nui, double Ei,
+                                               double nuj, double Ej, double Rs, double ms,
+                                               const SurfaceP& sf, double dt, d3 s) {
+    HMOut o;
+    const double Ys = 1.0 / ((1.0 - nui * nui) / Ei + (1.0 - nuj * nuj) / Ej);
+    const double Gs = 1.0 / (2.0 * (2.0 - nui) * (1.0 + nui) / Ei + 2.0 * (2.0 - nuj) * (1.0 + nuj) / Ej);
+    const double beta = sf.beta;
+    const double srd = sqrt(Rs * dn);
+    const double Sn = 2.0 * Ys * srd;
+    const double St = 8.0 * Gs * srd;
+    const double kn = 4.0 / 3.0 * Ys * srd;
+    const double gn = -2.0 * beta * sqrt(5.0 / 6.0 * Sn * ms);
+    const double kt = 8.0 * Gs * srd;
+    const double gt = -2.0 * beta * sqrt(5.0 / 6.0 * St * ms);
+    o.fatal = (gn < 0) || (gt < 0);
+    const d3 sv = s + mk3(vc.x * 0.0, vc.y * dt, vc.z * dt);
+    const double F1 = -kn * gap - gn * vc.x;
+    const d3 Ft = (-kt) * sv;
+    const double nFt = norm(Ft);
+    double F2, F3;
+    if (nFt >= sf.mu * F1) {
+        const double ratio = sf.mu * F1 / nFt;
+        F2 = ratio * Ft.y;
+        F3 = ratio * Ft.z;
+        o.s = mk3(sv.x, F2 / kt, F3 / kt);
+    } else {
+        F2 = Ft.y - gt * vc.y;
+        F3 = Ft.z - gt * vc.z;
+        o.s = sv;
+    }
+    o.F = mk3(F1, F2, F3);
+    return o;
+}
+
+// Neumaier-compensated FP32 accumulator (mixed precision force sums).
+struct CSum3 {
+    float s[3] = {0.f, 0.f, 0.f};
+    float c[3] = {0.f, 0.f, 0.f};
+    __device__ __forceinline__ void add(int k, float v) {
+        const float t = s[k] + v;
+        if (fabsf(s[k]) >= fabsf(v)) c[k] += (s[k] - t) + v;
+        else c[k] += (v - t) + s[k];
+        s[k] = t;
+    }
+    __device__ __forceinline__ void add(d3 v) { add(0, (float)v.x); add(1, (float)v.y); add(2, (float)v.z); }
+    __device__ __forceinline__ d3 value() const {
+        return mk3((double)s[0] + (double)c[0], (double)s[1] + (double)c[1], (double)s[2] + (double)c[2]);
+    }
+};
+
+template <bool MIXED>
+__global__ void __launch_bounds__(TPB)
+k_force(const Params P, const double4* __restrict__ pos, const double4* __restrict__ vel,
+        const double4* __restrict__ omg, const double4* __restrict__ quat,
+        double4* __restrict__ pos_n, double4* __restrict__ vel_n, double4* __restrict__ omg_n,
+        double4* __restrict__ quat_n, double4* __restrict__ acc, double4* __restrict__ dwo,
+        double4* __restrict__ ftot, double4* __restrict__ ttot, const uint8_t* __restrict__ sel,
+        const uint32_t* __restrict__ rcnt0, const uint32_t* __restrict__ rcnt1,
+        uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
+        double* __restrict__ st1, uint32_t* __restrict__ wact, double4* __restrict__ wshear,
+        unsigned long long* ctr, int write_forces) {
+    const int64_t p64 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t n = P.n;
+    uint64_t broken = 0, nwallc = 0, fatal = 0, active = 0, active_b = 0;
+    if (p64 < n) {
+        const uint32_t p = (uint32_t)p64;
+        const double dt = P.dt;
+        const double4 X = pos[p], Vv = vel[p], W = omg[p], Q = quat[p];
+        const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
+        d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
+        CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
+        const int b = sel[p];
+        uint32_t* pq = b ? pq1 : pq0;
+        double* st = b ? st1 : st0;
+        const uint32_t cnt = (b ? rcnt1 : rcnt0)[p];
+        const int64_t cs = (int64_t)P.K * n;
+        for (uint32_t m = 0; m < cnt; ++m) {
+            const int64_t e = (int64_t)m * n + p;
+            const uint32_t word = pq[e];
+            const uint32_t q = word & IDX_MASK;
+            const bool plo = p < q;
+            const double4 Xq = pos[q], Vq = vel[q], Wq = omg[q];
+            // canonical orientation: i = min(p, q), j = max(p, q)
+            const d3 ki = plo ? kp : mk3(Xq.x, Xq.y, Xq.z);
+            const d3 kj = plo ? mk3(Xq.x, Xq.y, Xq.z) : kp;
+            const d3 vi = plo ? vp : mk3(Vq.x, Vq.y, Vq.z);
+            const d3 vj = plo ? mk3(Vq.x, Vq.y, Vq.z) : vp;
+            const d3 wi = plo ? wp : mk3(Wq.x, Wq.y, Wq.z);
+            const d3 wj = plo ? mk3(Wq.x, Wq.y, Wq.z) : wp;
+            const double ri = plo ? X.w : Xq.w, rj = plo ? Xq.w : X.w;
+            const double mi = plo ? Vv.w : Vq.w, mj = plo ? Vq.w : Vv.w;
+            const d3 a = normalize(kj - ki);
+            const m3 R = contact_frame(a, P.tol);
+            const double Li = norm(kj - ki);
+            if (word & BONDED) {
+                // EBPM bond (kernels.jl:170-239)
+                const d3 kc = (ki + kj) * 0.5;
+                const d3 di = mul(R, vi * dt), dj = mul(R, vj * dt);
+                const d3 thi = mul(R, wi * dt), thj = mul(R, wj * dt);
+                const SurfaceP& sf = P.surf[0][0];
+                const double rmin = ri < rj ? ri : rj;
+                const double rb = sf.rho * rmin;
+                const double L = Li;
+                const double E = sf.E, nu = sf.nu;
+                const double rb2 = rb * rb;
+                const double Ib = rb2 * rb2 * M_PI / 4.0;
+                const double phi = 20.0 / 3.0 * rb2 / (L * L) * (1.0 + nu);
+                const double Ab = rb2 * M_PI;
+                const double k1 = E * Ab / L;
+                const double k2 = 12.0 * E * Ib / (L * L * L) / (1.0 + phi);
+                const double k3 = 6.0 * E * Ib / (L * L) / (1.0 + phi);
+                const double k4 = E * Ib / L / (1.0 + nu);
+                const double k5 = E * Ib * (4.0 + phi) / L / (1.0 + phi);
+                const double k6 = E * Ib * (2.0 - phi) / L / (1.0 + phi);
+                const d3 dF = mk3(k1 * (di.x - dj.x), k2 * (di.y - dj.y) + k3 * (thi.z + thj.z),
+                                  k2 * (di.z - dj.z) - k3 * (thi.y + thj.y));
+                const d3 dTi = mk3(k4 * (thi.x - thj.x), k3 * (dj.z - di.z) + k5 * thi.y + k6 * thj.y,
+                                   k3 * (di.y - dj.y) + k5 * thi.z + k6 * thj.z);
+                const d3 dTj = mk3(k4 * (thj.x - thi.x), k3 * (dj.z - di.z) + k6 * thi.y + k5 * thj.y,
+                                   k3 * (di.y - dj.y) + k6 * thi.z + k5 * thj.z);
+                const d3 Fi = mk3(st[S_FX * cs + e], st[S_FY * cs + e], st[S_FZ * cs + e]) + dF;
+                const d3 Fj = -Fi;
+                const d3 Ti = mk3(st[S_TIX * cs + e], st[S_TIY * cs + e], st[S_TIZ * cs + e]) + dTi;
+                const d3 Tj = mk3(st[S_TJX * cs + e], st[S_TJY * cs + e], st[S_TJZ * cs + e]) + dTj;
+                const double sci = Fj.x / Ab - rb / Ib * sqrt(Ti.y * Ti.y + Ti.z * Ti.z);
+                const double scj = Fj.x / Ab - rb / Ib * sqrt(Tj.y * Tj.y + Tj.z * Tj.z);
+                const double sc = -(sci < scj ? sci : scj);
+                const double stt = sci > scj ? sci : scj;
+                const double ss = fabs(Ti.x) * rb / 2.0 / Ib + 4.0 / 3.0 / Ab * sqrt(Fi.y * Fi.y + Fi.z * Fi.z);
+                if (sc >= sf.sc || stt >= sf.st || ss >= sf.ss) {
+                    pq[e] = word | BROKEN;  // both copies take the same decision
+                    if (plo) ++broken;
+                    continue;
+                }
+                const m3 Ri = inverse(R);
+                if (plo) {
+                    F = F + mul(Ri, -Fi);
+                    T = T + mul(Ri, -Ti);
+                } else {
+                    F = F + mul(Ri, -Fj);
+                    T = T + mul(Ri, -Tj);
+                }
+                st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
+                st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
+                st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
+                st[S_SX * cs + e] = 0.0; st[S_SY * cs + e] = 0.0; st[S_SZ * cs + e] = 0.0;
+                if (plo) { ++active; ++active_b; }
+            } else {
+                // Hertz-Mindlin (kernels.jl:240-313)
+                const double gap = Li - ri - rj;
+                const double dn = fabs(gap);
+                const d3 kc = ki + normalize(kj - ki) * (ri - dn);
+                const d3 kip = kc - ki, kjp = kc - kj;
+                const d3 vci = cross(wi, kip) + vi;
+                const d3 vcj = cross(wj, kjp) + vj;
+                const d3 vc = mul(R, vcj - vci);
+                const SurfaceP& sf = P.surf[0][0];
+                const double Rs = 1.0 / (1.0 / ri + 1.0 / rj);
+                const double ms = 1.0 / (1.0 / mi + 1.0 / mj);
+                const d3 s0 = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
+                const HMOut h = hertz_mindlin(gap, dn, vc, P.matNu[0], P.matE[0], P.matNu[0],
+                                              P.matE[0], Rs, ms, sf, dt, s0);
+                const m3 Ri = inverse(R);
+                const d3 Fg = mul(Ri, -h.F);
+                if (MIXED) {
+                    if (plo) { FH.add(Fg); TH.add(cross(kip, Fg)); }
+                    else { FH.add(-Fg); TH.add(cross(kjp, -Fg)); }
+                } else {
+                    if (plo) { F = F + Fg; T = T + cross(kip, Fg); }
+                    else { F = F + (-Fg); T = T + cross(kjp, -Fg); }
+                }
+                st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                st[S_FX * cs + e] = h.F.x; st[S_FY * cs + e] = h.F.y; st[S_FZ * cs + e] = h.F.z;
+                st[S_SX * cs + e] = h.s.x; st[S_SY * cs + e] = h.s.y; st[S_SZ * cs + e] = h.s.z;
+                if (plo) {
+                    ++active;
+                    if (gap > 0 || h.fatal) ++fatal;
+                }
+            }
+        }
+        if (MIXED) {
+            F = F + FH.value();
+            T = T + TH.value();
+        }
+        // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
+        uint32_t wa = P.nwall ? wact[p] : 0u, wa_new = 0u;
+        for (int j = 0; j < P.nwall; ++j) {
+            const WallP& wl = P.walls[j];
+            const d3 nrm = mk3(wl.n[0], wl.n[1], wl.n[2]);
+            if (fabs(dot(kp, nrm) - wl.d) < X.w) {
+                ++nwallc;
+                const m3 R = contact_frame(nrm, P.tol);
+                const double L = dot(kp, nrm) - wl.d;
+                const double gap = fabs(L) - X.w;
+                const double dn = fabs(gap);
+                const d3 kip = (((-L) * nrm) / fabs(L)) * (fabs(L) + dn / 2.0);
+                const d3 vci = cross(wp, kip) + vp;
+                const d3 vc = mul(R, -vci);
+                const int mj = wl.mid - 1;
+                const SurfaceP& sf = P.surf[0][mj];
+                d3 s0 = mk3(0.0, 0.0, 0.0);
+                if (wa & (1u << j)) {
+                    const double4 ws = wshear[(int64_t)j * n + p];
+                    s0 = mk3(ws.x, ws.y, ws.z);
+                }
+                const HMOut h = hertz_mindlin(gap, dn, vc, P.matNu[0], P.matE[0], P.matNu[mj],
+                                              P.matE[mj], X.w, Vv.w, sf, dt, s0);
+                if (h.fatal) ++fatal;
+                const m3 Ri = inverse(R);
+                const d3 Fg = mul(Ri, -h.F);
+                F = F + Fg;
+                T = T + cross(kip, Fg);
+                wshear[(int64_t)j * n + p] = make_double4(h.s.x, h.s.y, h.s.z, 0.0);
+                wa_new |= 1u << j;
+            }
+        }
+        if (P.nwall && wa_new != wa) wact[p] = wa_new;
+        // apply_body_force! (kernels.jl:428-443)
+        F = F + mk3(P.g[0] * Vv.w, P.g[1] * Vv.w, P.g[2] * Vv.w);
+        const double nd = -P.damping;
+        F = F + mk3(nd * fabs(F.x) * jsign(vp.x), nd * fabs(F.y) * jsign(vp.y), nd * fabs(F.z) * jsign(vp.z));
+        T = T + mk3(nd * fabs(T.x) * jsign(wp.x), nd * fabs(T.y) * jsign(wp.y), nd * fabs(T.z) * jsign(wp.z));
+        if (write_forces) {
+            ftot[p] = make_double4(F.x, F.y, F.z, 0.0);
+            ttot[p] = make_double4(T.x, T.y, T.z, 0.0);
+        }
+        // update! (kernels.jl:445-485)
+        const d3 av = F / Vv.w;
+        const d3 knew = (kp + vp * dt) + (0.5 * av) * (dt * dt);
+        const d3 vnew = vp + av * dt;
+        const m3 Rq = quat_matrix(Q);
+        const d3 Tl = mul(Rq, T);
+        const d3 wl = mul(Rq, wp);
+        m3 I;
+        I.a[0][0] = W.w; I.a[0][1] = 0.0; I.a[0][2] = 0.0;
+        I.a[1][0] = 0.0; I.a[1][1] = W.w; I.a[1][2] = 0.0;
+        I.a[2][0] = 0.0; I.a[2][1] = 0.0; I.a[2][2] = W.w;
+        const m3 Iinv = inverse(I);
+        const d3 dwl = mul(Iinv, Tl - cross(wl, mul(I, wl)));
+        const m3 Rqi = inverse(Rq);
+        const d3 dw = mul(Rqi, dwl);
+        const double q0 = Q.x, q1 = Q.y, q2 = Q.z, q3 = Q.w;
+        const double dq0 = -0.5 * (q1 * wp.x + q2 * wp.y + q3 * wp.z);
+        const double dq1 = 0.5 * (q0 * wp.x + q2 * wp.z - q3 * wp.y);
+        const double dq2 = 0.5 * (q0 * wp.y + q1 * wp.z + q3 * wp.x);
+        const double dq3 = 0.5 * (q0 * wp.z + q1 * wp.y - q2 * wp.x);
+        const double n0 = q0 + dq0, n1 = q1 + dq1, n2 = q2 + dq2, n3 = q3 + dq3;
+        const double qn = sqrt(n0 * n0 + n1 * n1 + n2 * n2 + n3 * n3);
+        const d3 wnew = wp + dw * dt;
+        pos_n[p] = make_double4(knew.x, knew.y, knew.z, X.w);
+        vel_n[p] = make_double4(vnew.x, vnew.y, vnew.z, Vv.w);
+        omg_n[p] = make_double4(wnew.x, wnew.y, wnew.z, W.w);
+        quat_n[p] = make_double4(n0 / qn, n1 / qn, n2 / qn, n3 / qn);
+        acc[p] = make_double4(av.x, av.y, av.z, 0.0);
+        dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
+    }
+    wave_add(&ctr[CT_BROKEN], broken);
+    wave_add(&ctr[CT_BROKEN_TOTAL], broken);
+    wave_add(&ctr[CT_WALL], nwallc);
+    wave_add(&ctr[CT_FATAL], fatal);
+    wave_add(&ctr[CT_ACTIVE], active);
+    wave_add(&ctr[CT_ACTIVE_BOND], active_b);
+}
+
+// ------------------------------------------------------------------------------------------
+// read-back, state restore and intermediate arrays
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(TPB)
+k_debug_cells(const Params P, const double4* __restrict__ pos, int32_t* kid, uint32_t* hid,
+              double* kc, uint32_t* nbr) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= P.n) return;
+    const double4 x = pos[p];
+    uint32_t w[8];
+    int32_t c[3];
+    double ctr[3];
+    window_of(P.grid, mk3(x.x, x.y, x.z), w, c, ctr);
+    for (int a = 0; a < 3; ++a) { kid[3 * p + a] = c[a]; kc[3 * p + a] = ctr[a]; }
+    hid[p] = w[0] + 1u;
+    for (int k = 0; k < 8; ++k) nbr[8 * p + k] = w[k] + 1u;
+}
+
+// cp_list of the reference (kernels.jl:24-58): every window slot, aliases included.
+template <bool EMIT>
+__global__ void __launch_bounds__(TPB)
+k_cp_list(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx,
+          const uint32_t* __restrict__ bstart, uint32_t* __restrict__ cnt,
+          const uint32_t* __restrict__ off, int32_t* __restrict__ pairs) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= P.n) return;
+    const double4 x = pos[i];
+    uint32_t w[8];
+    window_of(P.grid, mk3(x.x, x.y, x.z), w);
+    uint32_t c = 0;
+    const uint32_t base = EMIT ? off[i] : 0u;
+    for (int wi = 0; wi < 8; ++wi) {
+        const uint32_t b = w[wi];
+        for (uint32_t k = bstart[b]; k < bstart[b + 1]; ++k) {
+            const uint32_t j = sidx[k];
+            if ((uint32_t)i < j) {
+                if (EMIT) {
+                    pairs[2 * ((int64_t)base + c)] = (int32_t)i + 1;
+                    pairs[2 * ((int64_t)base + c) + 1] = (int32_t)j + 1;
+                }
+                ++c;
+            }
+        }
+    }
+    if (!EMIT) cnt[i] = c;
+}
+
+// Active contacts owned by the lower index, in (i, j) order.
+template <bool EMIT>
+__global__ void __launch_bounds__(TPB)
+k_lo_rows(const Params P, const uint8_t* __restrict__ sel, const uint32_t* __restrict__ rcnt0,
+          const uint32_t* __restrict__ rcnt1, const uint32_t* __restrict__ pq0,
+          const uint32_t* __restrict__ pq1, const double* __restrict__ st0,
+          const double* __restrict__ st1, const double4* __restrict__ pos, uint32_t* __restrict__ cnt,
+          const uint32_t* __restrict__ off, dem_contact_t* __restrict__ out,
+          uint8_t* __restrict__ bonded) {
+    const int64_t p64 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const int64_t n = P.n;
+    if (p64 >= n) return;
+    const uint32_t p = (uint32_t)p64;
+    const int b = sel[p];
+    const uint32_t* pq = b ? pq1 : pq0;
+    const double* st = b ? st1 : st0;
+    const uint32_t rc = (b ? rcnt1 : rcnt0)[p];
+    const int64_t cs = (int64_t)P.K * n;
+    uint32_t c = 0;
+    const uint32_t base = EMIT ? off[p] : 0u;
+    for (uint32_t m = 0; m < rc; ++m) {
+        const int64_t e = (int64_t)m * n + p;
+        const uint32_t word = pq[e];
+        const uint32_t q = word & IDX_MASK;
+        if (q <= p || (word & BROKEN)) continue;
+        if (EMIT) {
+            dem_contact_t& o = out[base + c];
+            const bool bd = (word & BONDED) != 0;
+            o.i = (int32_t)p + 1; o.j = (int32_t)q + 1; o.midi = 1; o.midj = 1;
+            o.k[0] = st[S_KX * cs + e]; o.k[1] = st[S_KY * cs + e]; o.k[2] = st[S_KZ * cs + e];
+            o.F[0] = st[S_FX * cs + e]; o.F[1] = st[S_FY * cs + e]; o.F[2] = st[S_FZ * cs + e];
+            o.ti[0] = bd ? st[S_TIX * cs + e] : 0.0; o.ti[1] = bd ? st[S_TIY * cs + e] : 0.0;
+            o.ti[2] = bd ? st[S_TIZ * cs + e] : 0.0;
+            o.tj[0] = bd ? st[S_TJX * cs + e] : 0.0; o.tj[1] = bd ? st[S_TJY * cs + e] : 0.0;
+            o.tj[2] = bd ? st[S_TJZ * cs + e] : 0.0;
+            o.s[0] = st[S_SX * cs + e]; o.s[1] = st[S_SY * cs + e]; o.s[2] = st[S_SZ * cs + e];
+            bonded[base + c] = bd ? 1 : 0;
+        }
+        ++c;
+    }
+    if (!EMIT) cnt[p] = c;
+}
+
+__global__ void __launch_bounds__(TPB)
+k_write_rows(const Params P, int64_t m, const uint64_t* __restrict__ dst,
+             const uint32_t* __restrict__ word, const double* __restrict__ state,
+             uint32_t* __restrict__ pq, double* __restrict__ st) {
+    const int64_t e = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (e >= m) return;
+    const int64_t d = (int64_t)dst[e];
+    const int64_t cs = (int64_t)P.K * P.n;
+    pq[d] = word[e];
+    for (int c = 0; c < S_NCOMP; ++c) st[c * cs + d] = state[e * S_NCOMP + c];
+}
+
+inline dim3 grid_for(int64_t n, int tpb) { return dim3((uint32_t)((n + tpb - 1) / tpb)); }
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------
+// timing helpers
+// ------------------------------------------------------------------------------------------
+void timing_begin(dem_scene* s, int kernel) {
+    if (!s->timing || s->nev >= 64) return;
+    hipEventRecord(s->ev[2 * s->nev], s->stream);
+    s->ev_kernel[s->nev] = kernel;
+}
+void timing_end(dem_scene* s) {
+    if (!s->timing || s->nev >= 64) return;
+    hipEventRecord(s->ev[2 * s->nev + 1], s->stream);
+    s->nev++;
+}
+
+#define LAUNCH(s, kid, ...)                 \
+    do {                                    \
+        timing_begin(s, kid);               \
+        __VA_ARGS__;                        \
+        timing_end(s);                      \
+        hipError_t e_ = hipGetLastError();  \
+        if (e_ != hipSuccess) return e_;    \
+    } while (0)
+
+static hipError_t sort_particles(dem_scene* s, bool reset_counters) {
+    DevState& d = s->d;
+    const Params& P = s->P;
+    const int c = s->cur;
+    LAUNCH(s, K_HASH, hipLaunchKernelGGL(k_hash, grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P,
+                                         d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
+                                         reset_counters ? 1 : 0));
+    timing_begin(s, K_SCAN);
+    hipError_t e = launch_scan(d.scan, d.bcnt, d.bstart, (int64_t)P.grid.H + 1, true, nullptr, s->stream);
+    timing_end(s);
+    if (e != hipSuccess) return e;
+    LAUNCH(s, K_SCATTER, hipLaunchKernelGGL(k_scatter, grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
+                                            P, d.pos[c], d.hid, d.rank, d.bstart, d.sidx, d.spos,
+                                            s->precision == DEM_PRECISION_MIXED ? d.spos32 : nullptr));
+    return hipSuccess;
+}
+
+hipError_t run_contact_detection(dem_scene* s, bool init) {
+    DevState& d = s->d;
+    const Params& P = s->P;
+    const int c = s->cur;
+    hipError_t e = sort_particles(s, true);
+    if (e != hipSuccess) return e;
+    const bool mixed = s->precision == DEM_PRECISION_MIXED && !init;
+    const dim3 g = grid_for(P.n, TPB);
+    if (init) {
+        LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad<true, false>), g, dim3(TPB), 0, s->stream, P,
+                                             d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                             d.touch, d.ctr));
+    } else if (mixed) {
+        LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad<false, true>), g, dim3(TPB), 0, s->stream, P,
+                                              d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                              d.touch, d.ctr));
+    } else {
+        LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad<false, false>), g, dim3(TPB), 0, s->stream, P,
+                                              d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                              d.touch, d.ctr));
+    }
+    const size_t lds = (size_t)MERGE_TPB * P.K * sizeof(uint32_t);
+    const dim3 gm = grid_for(P.n, MERGE_TPB);
+    if (init) {
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true>), gm, dim3(MERGE_TPB), lds, s->stream,
+                                              P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
+                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
+    } else {
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false>), gm, dim3(MERGE_TPB), lds, s->stream,
+                                              P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
+                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
+    }
+    return hipSuccess;
+}
+
+hipError_t run_force_integrate(dem_scene* s, bool write_forces) {
+    DevState& d = s->d;
+    const Params& P = s->P;
+    const int c = s->cur, nx = 1 - s->cur;
+    const dim3 g = grid_for(P.n, TPB);
+    if (s->precision == DEM_PRECISION_MIXED) {
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<true>), g, dim3(TPB), 0, s->stream, P,
+                                              d.pos[c], d.vel[c], d.omg[c], d.quat[c], d.pos[nx],
+                                              d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
+                                              d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
+                                              d.st[0], d.st[1], d.wact, d.wshear, d.ctr,
+                                              write_forces ? 1 : 0));
+    } else {
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<false>), g, dim3(TPB), 0, s->stream, P,
+                                              d.pos[c], d.vel[c], d.omg[c], d.quat[c], d.pos[nx],
+                                              d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
+                                              d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
+                                              d.st[0], d.st[1], d.wact, d.wshear, d.ctr,
+                                              write_forces ? 1 : 0));
+    }
+    s->cur = nx;
+    return hipSuccess;
+}
+
+hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc, uint32_t* nbr) {
+    hipLaunchKernelGGL(k_debug_cells, grid_for(s->P.n, TPB), dim3(TPB), 0, s->stream, s->P,
+                       s->d.pos[s->cur], kid, hid, kc, nbr);
+    return hipGetLastError();
+}
+
+hipError_t run_debug_sort(dem_scene* s) { return sort_particles(s, false); }
+
+hipError_t run_debug_cp_list(dem_scene* s, int32_t** pairs_dev, int64_t* count) {
+    DevState& d = s->d;
+    const Params& P = s->P;
+    hipError_t e = sort_particles(s, false);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_cp_list<false>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P,
+                       d.pos[s->cur], d.sidx, d.bstart, d.scratch_u32, nullptr, nullptr);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    e = launch_scan(d.scan, d.scratch_u32, d.scratch2_u32, P.n, false, d.total, s->stream);
+    if (e != hipSuccess) return e;
+    uint64_t total = 0;
+    if ((e = hipMemcpyAsync(&total, d.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return e;
+    int32_t* pairs = nullptr;
+    if ((e = hipMalloc(&pairs, 2 * sizeof(int32_t) * (total ? total : 1))) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_cp_list<true>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P,
+                       d.pos[s->cur], d.sidx, d.bstart, nullptr, d.scratch2_u32, pairs);
+    if ((e = hipGetLastError()) != hipSuccess) { hipFree(pairs); return e; }
+    *pairs_dev = pairs;
+    *count = (int64_t)total;
+    return hipSuccess;
+}
+
+hipError_t run_read_contacts(dem_scene* s, dem_contact_t** out_dev, uint8_t** bonded_dev,
+                             int64_t* count) {
+    DevState& d = s->d;
+    const Params& P = s->P;
+    hipLaunchKernelGGL((k_lo_rows<false>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P, d.sel,
+                       d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.pos[s->cur],
+                       d.scratch_u32, nullptr, nullptr, nullptr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if ((e = launch_scan(d.scan, d.scratch_u32, d.scratch2_u32, P.n, false, d.total, s->stream)) != hipSuccess)
+        return e;
+    uint64_t total = 0;
+    if ((e = hipMemcpyAsync(&total, d.total, sizeof(uint64_t), hipMemcpyDeviceToHost, s->stream)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(s->stream)) != hipSuccess) return e;
+    dem_contact_t* out = nullptr;
+    uint8_t* bd = nullptr;
+    if ((e = hipMalloc(&out, sizeof(dem_contact_t) * (total ? total : 1))) != hipSuccess) return e;
+    if ((e = hipMalloc(&bd, total ? total : 1)) != hipSuccess) { hipFree(out); return e; }
+    hipLaunchKernelGGL((k_lo_rows<true>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P, d.sel,
+                       d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.pos[s->cur],
+                       nullptr, d.scratch2_u32, out, bd);
+    if ((e = hipGetLastError()) != hipSuccess) { hipFree(out); hipFree(bd); return e; }
+    *out_dev = out;
+    *bonded_dev = bd;
+    *count = (int64_t)total;
+    return hipSuccess;
+}
+
+hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint32_t* word_dev,
+                              const double* state_dev, int64_t m) {
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_write_rows, grid_for(m, TPB), dim3(TPB), 0, s->stream, s->P, m, dst_dev,
+                       word_dev, state_dev, s->d.pq[0], s->d.st[0]);
+    return hipGetLastError();
+}
+
+}  // namespace dem

@@ -1,0 +1,247 @@
+"""DEMPort executor API on the MI355X engine.
+
+``Scene`` owns one device-resident GlobalData (src/executor.jl:1-29) held by libdemhip;
+``Scene.initialize`` is ``initialize!`` (executor.jl:155-160), ``Scene.step(n)`` is n calls of
+``simulate!`` (executor.jl:162-172), and ``solve`` mirrors ``DEMPort.solve`` (executor.jl:174-394):
+TOML config, p4p input relative to the config file, optional bond initialisation, the
+save-interval loop and the p4p/p4c output.
+
+Every call goes through the C ABI of include/demhip.h; there is no Python or CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import logging
+import os
+import time
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+from .config import DEMConfig, from_toml
+from .p4io import read_p4p_rows, write_p4c_frame, write_p4p_frame
+from .types import CONTACT_DTYPE, GRAIN_DTYPE, GRAIN_IN_DTYPE, WALL_DTYPE, make_walls
+
+log = logging.getLogger("demhip")
+
+PRECISIONS = {"fp64": N.DEM_PRECISION_FP64, "mixed": N.DEM_PRECISION_MIXED}
+
+
+class DEMError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libdemhip error {code}: {msg}")
+        self.code = code
+
+
+def _ptr(arr: np.ndarray, ctype):
+    return arr.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+class Scene:
+    """One simulation scene on the GPU (GlobalData)."""
+
+    def __init__(self, cfg: DEMConfig, grains: np.ndarray, walls: Optional[Sequence] = None,
+                 precision: str = "fp64", contact_slots: int = 0, device: int = 0):
+        self._lib = N.load_library()
+        self.cfg = cfg
+        grains = np.ascontiguousarray(grains, dtype=GRAIN_IN_DTYPE)
+        self.n = len(grains)
+        if walls is None:  # executor.jl:313 -- the single configured wall
+            walls = [(cfg.wall_normal, cfg.wall_distance, 2)]
+        w = walls if isinstance(walls, np.ndarray) and walls.dtype == WALL_DTYPE else make_walls(walls)
+        self.walls = np.ascontiguousarray(w)
+        self.nwall = len(self.walls)
+        self._c_cfg = cfg.to_c()
+        opt = N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), 0)
+        h = ctypes.c_void_p()
+        rc = self._lib.dem_create(ctypes.byref(self._c_cfg), _ptr(self.walls, N.DemWall),
+                                  self.nwall, _ptr(grains, N.DemGrainIn), self.n,
+                                  ctypes.byref(opt), ctypes.byref(h))
+        if rc != 0:
+            raise DEMError(rc, self._lib.dem_last_error(None).decode())
+        self._h = h
+        self.precision = precision
+
+    # -- lifecycle ---------------------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.dem_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int) -> None:
+        if rc != 0:
+            raise DEMError(rc, self._lib.dem_last_error(self._h).decode())
+
+    # -- hot path ------------------------------------------------------------------------------
+    def initialize(self) -> None:
+        """initialize! (executor.jl:155-160): bonds between pairs closer than r0_i + r0_j."""
+        self._check(self._lib.dem_initialize(self._h))
+
+    def step(self, nsteps: int = 1) -> None:
+        """nsteps x simulate! (executor.jl:162-172)."""
+        self._check(self._lib.dem_step(self._h, int(nsteps)))
+
+    # -- read-back -----------------------------------------------------------------------------
+    def derived(self) -> dict:
+        d = N.DemDerived()
+        self._check(self._lib.dem_get_derived(self._h, ctypes.byref(d)))
+        return {f: getattr(d, f) for f, _ in N.DemDerived._fields_}
+
+    def counters(self) -> dict:
+        c = N.DemCounters()
+        self._check(self._lib.dem_get_counters(self._h, ctypes.byref(c)))
+        return {f: getattr(c, f) for f, _ in N.DemCounters._fields_}
+
+    def read_grains(self) -> np.ndarray:
+        out = np.zeros(self.n, dtype=GRAIN_DTYPE)
+        self._check(self._lib.dem_read_grains(self._h, _ptr(out, N.DemGrain), self.n))
+        return out
+
+    def write_grains(self, grains: np.ndarray) -> None:
+        g = np.ascontiguousarray(grains, dtype=GRAIN_DTYPE)
+        self._check(self._lib.dem_write_grains(self._h, _ptr(g, N.DemGrain), self.n))
+
+    def read_contacts(self):
+        """Active contacts sorted by (i, j) and their bonded flags."""
+        cnt = ctypes.c_int64(0)
+        self._check(self._lib.dem_read_contacts(self._h, None, None, 0, ctypes.byref(cnt)))
+        m = cnt.value
+        out = np.zeros(m, dtype=CONTACT_DTYPE)
+        bd = np.zeros(m, dtype=np.uint8)
+        if m:
+            self._check(self._lib.dem_read_contacts(self._h, _ptr(out, N.DemContact),
+                                                    _ptr(bd, ctypes.c_uint8), m, ctypes.byref(cnt)))
+            if cnt.value != m:
+                raise DEMError(-6, "contact count changed between calls")
+        return out, bd.astype(bool)
+
+    def write_contacts(self, contacts: np.ndarray, bonded: np.ndarray) -> None:
+        c = np.ascontiguousarray(contacts, dtype=CONTACT_DTYPE)
+        b = np.ascontiguousarray(bonded, dtype=np.uint8)
+        self._check(self._lib.dem_write_contacts(self._h, _ptr(c, N.DemContact),
+                                                 _ptr(b, ctypes.c_uint8), len(c)))
+
+    def read_wall_shear(self) -> np.ndarray:
+        out = np.zeros((self.n, self.nwall, 3), dtype=np.float64)
+        self._check(self._lib.dem_read_wall_shear(self._h, _ptr(out, ctypes.c_double), self.n))
+        return out
+
+    def write_wall_shear(self, s: np.ndarray) -> None:
+        a = np.ascontiguousarray(s, dtype=np.float64).reshape(self.n, self.nwall, 3)
+        self._check(self._lib.dem_write_wall_shear(self._h, _ptr(a, ctypes.c_double), self.n))
+
+    def read_forces(self):
+        F = np.zeros((self.n, 3))
+        T = np.zeros((self.n, 3))
+        self._check(self._lib.dem_read_forces(self._h, _ptr(F, ctypes.c_double),
+                                              _ptr(T, ctypes.c_double), self.n))
+        return F, T
+
+    # -- intermediate arrays of the contact pass -----------------------------------------------
+    def debug_cells(self):
+        kid = np.zeros((self.n, 3), dtype=np.int32)
+        hid = np.zeros(self.n, dtype=np.uint32)
+        kc = np.zeros((self.n, 3), dtype=np.float64)
+        nbr = np.zeros((self.n, 8), dtype=np.uint32)
+        self._check(self._lib.dem_debug_cells(self._h, _ptr(kid, ctypes.c_int32),
+                                              _ptr(hid, ctypes.c_uint32), _ptr(kc, ctypes.c_double),
+                                              _ptr(nbr, ctypes.c_uint32)))
+        return kid, hid, kc, nbr
+
+    def debug_hash_table(self):
+        H = self.derived()["hash_table_size"]
+        cnt = np.zeros(H, dtype=np.uint32)
+        start = np.zeros(H, dtype=np.uint32)
+        pid = np.zeros(self.n, dtype=np.uint32)
+        self._check(self._lib.dem_debug_hash_table(self._h, _ptr(cnt, ctypes.c_uint32),
+                                                   _ptr(start, ctypes.c_uint32),
+                                                   _ptr(pid, ctypes.c_uint32)))
+        return cnt, start, pid
+
+    def debug_cp_list(self) -> np.ndarray:
+        cnt = ctypes.c_int64(0)
+        self._check(self._lib.dem_debug_cp_list(self._h, None, 0, ctypes.byref(cnt)))
+        m = cnt.value
+        out = np.zeros((m, 2), dtype=np.int32)
+        if m:
+            self._check(self._lib.dem_debug_cp_list(self._h, _ptr(out, ctypes.c_int32), m,
+                                                    ctypes.byref(cnt)))
+        return out
+
+    # -- timing --------------------------------------------------------------------------------
+    def set_timing(self, on: bool) -> None:
+        self._check(self._lib.dem_set_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self) -> dict:
+        t = N.DemKernelTimes()
+        self._check(self._lib.dem_get_kernel_times(self._h, ctypes.byref(t)))
+        return {self._lib.dem_kernel_name(k).decode(): (t.ms[k], t.launches[k])
+                for k in range(N.DEM_NUM_KERNELS)}
+
+
+def device_scan(values: np.ndarray):
+    """Exclusive prefix sum with the engine's decoupled-lookback scan."""
+    lib = N.load_library()
+    a = np.ascontiguousarray(values, dtype=np.uint32)
+    out = np.zeros_like(a)
+    tot = ctypes.c_uint64(0)
+    rc = lib.dem_debug_scan(_ptr(a, ctypes.c_uint32), _ptr(out, ctypes.c_uint32), len(a),
+                            ctypes.byref(tot))
+    if rc != 0:
+        raise DEMError(rc, lib.dem_last_error(None).decode())
+    return out, tot.value
+
+
+def solve(cfg_filename: str, no_bond: bool = False, save_snapshot: bool = False,
+          save_information: bool = True, precision: str = "fp64", out_dir: str = ".",
+          max_steps: Optional[int] = None) -> Scene:
+    """DEMPort.solve (executor.jl:174-394) on the MI355X engine.
+
+    ``save_snapshot`` (GLMakie pictures, utils.jl:80-124) is accepted and ignored.  ``max_steps``
+    optionally stops the loop early (the reference's commented debug switch, executor.jl:360-362).
+    """
+    t0 = time.perf_counter()
+    cfg = from_toml(cfg_filename)
+    rows = read_p4p_rows(os.path.join(os.path.dirname(cfg_filename), cfg.init_particles))
+    scene = Scene(cfg, rows, precision=precision)
+    der = scene.derived()
+    log.info("Setting: hash_table_size=%d cell_size=%g", der["hash_table_size"], der["cell_size"])
+    total_steps = der["total_steps"] if max_steps is None else min(der["total_steps"], max_steps)
+    save_per_steps = max(der["save_per_steps"], 1)
+    if not no_bond:
+        scene.initialize()
+    p4p = p4c = None
+    if save_information:
+        p4p = open(os.path.join(out_dir, "output.p4p"), "w")
+        p4c = open(os.path.join(out_dir, "output.p4c"), "w")
+        write_p4p_frame(p4p, scene.read_grains(), 0.0)
+        c, b = scene.read_contacts()
+        write_p4c_frame(p4c, c, b, 0.0)
+    step = 0
+    while step < total_steps:
+        t1 = time.perf_counter()
+        scene.step(save_per_steps)
+        step += save_per_steps
+        log.info("Solving... step=%d dt=%.3fs", step, time.perf_counter() - t1)
+        if save_information:
+            write_p4p_frame(p4p, scene.read_grains(), step * cfg.dt)
+            c, b = scene.read_contacts()
+            write_p4c_frame(p4c, c, b, step * cfg.dt)
+    if save_information:
+        p4p.close()
+        p4c.close()
+    log.info("Total time: %.3f s", time.perf_counter() - t0)
+    return scene
