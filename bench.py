@@ -1,0 +1,177 @@
+"""Benchmark: particle-steps/s of the DEM step on one MI355X.
+
+Workload (default): the 1M-grain polydisperse pour -- log-normal radii (median 0.5 mm, sigma 0.25,
+clipped 0.25-1.0 mm), density 2500 kg/m^3, random non-overlapping seeding in a 0.1 x 0.1 x 0.3 m
+box with a floor and four side walls, Hertz-Mindlin only (E = 1e7 Pa, mu = 0.4), dt = 2e-7 s,
+synthetic input generated in-process (seed 3).  A step is one simulate! over all grains.
+
+Prints one JSON line.  ``value`` is the whole-job throughput (particle-steps/s) of the timed K
+steps, state resident in HBM before the timed region starts.  ``roofline`` describes the dominant
+kernel (largest share of the step time) measured with HIP events on the scene's stream over the
+same timed region.  ``cpu_baseline`` times the FP64 CPU restatement (the oracle) of the same step
+on a bounded sample on this host.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> float:
+    """Algorithmic HBM bytes of one launch, from the per-unit figures of the reference design:
+    per particle 360 B split over the kernels that touch particle arrays, per hash bucket 12 B,
+    per candidate pair 20 B (cp_list write 8 + read 8 + one 4-B state probe), per Hertz contact
+    104 B, per bonded contact 176 B, per wall contact 48 B."""
+    n, H, P = c["n"], c["hash_table_size"], c["candidate_pairs"]
+    Ch, Cb, W = c["hertz"], c["bonded"], c["wall_contacts"]
+    if kernel == "broad_phase":
+        # cp_list + state probe per candidate, cell key read, pid read, pair-range count/offset
+        return 20.0 * P + n * (4 + 4 + 16)
+    if kernel == "force_integrate":
+        # particle state read 128 + write 104, force/moment zero + read 96; contact terms
+        return n * (128 + 104 + 96) + 104.0 * Ch + 176.0 * Cb + 48.0 * W
+    if kernel == "scan":
+        return 12.0 * H
+    if kernel == "hash":
+        return n * 4.0
+    if kernel == "scatter":
+        return n * 4.0
+    return 0.0
+
+
+def build_workload(name: str, n: int):
+    from taichi_dem_julia_port_amd import scenes
+    if name == "pour":
+        cfg, g, w = scenes.pour(n)
+        desc = (f"{n}-grain polydisperse pour (log-normal r, median 0.5 mm, sigma 0.25, clipped "
+                "0.25-1.0 mm; 0.1x0.1x0.3 m box, floor + 4 side walls; E=1e7 Pa, mu=0.4, "
+                "dt=2e-7 s; no bonds)")
+        return cfg, g, w, False, desc
+    if name == "cube18k":
+        cfg, g, w = scenes.cube_18k()
+        return cfg, g, w, True, "18252-grain bonded cube impact (26x26x27, 20 m/s, dt=5e-7 s)"
+    if name == "cube900":
+        cfg, g, w = scenes.cube_small()
+        return cfg, g, w, True, "900-grain bonded cube impact (10x10x9, 10 m/s, dt=1e-6 s)"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def run_engine(cfg, g, w, bonded, precision, steps, warmup):
+    from taichi_dem_julia_port_amd.executor import Scene
+    s = Scene(cfg, g, w, precision=precision)
+    if bonded:
+        s.initialize()
+    s.step(warmup)
+    # timed region: K steps, HIP events around every launch on the scene's stream
+    s.set_timing(True)
+    t0 = time.perf_counter()
+    s.step(steps)  # returns after a stream synchronisation
+    t1 = time.perf_counter()
+    times = s.kernel_times()
+    s.set_timing(False)
+    # the same K steps again without events: the event overhead, for the record
+    t2 = time.perf_counter()
+    s.step(steps)
+    t3 = time.perf_counter()
+    c = s.counters()
+    d = s.derived()
+    s.close()
+    return {"elapsed": t1 - t0, "elapsed_no_events": t3 - t2, "kernels": times, "counters": c,
+            "derived": d}
+
+
+def cpu_baseline(cfg, g, w, bonded, steps):
+    """The oracle (serial FP64 C restatement of the reference step) on a bounded sample."""
+    from oracle.oracle import OracleScene
+    o = OracleScene(cfg, g, w)
+    if bonded:
+        o.initialize()
+    o.step(1)  # warm-up: allocations and first contact pass
+    t0 = time.perf_counter()
+    o.step(steps)
+    dt = time.perf_counter() - t0
+    o.close()
+    return len(g) * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--workload", default="pour", choices=["pour", "cube18k", "cube900"])
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--compare-precision", action="store_true",
+                    help="also time the other precision and report both")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs one scene per GPU on one GPU; multi-GPU "
+                                   "runs are not part of this build (use --gpus 1)"}))
+        sys.exit(2)
+
+    t_build = time.perf_counter()
+    cfg, g, w, bonded, desc = build_workload(args.workload, args.n)
+    t_build = time.perf_counter() - t_build
+    n = len(g)
+    r = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup)
+    value = n * args.steps / r["elapsed"]
+    kt = r["kernels"]
+    step_ms = {k: (ms / max(l, 1)) for k, (ms, l) in kt.items() if l}
+    total_kernel_ms = sum(ms for ms, _ in kt.values())
+    dom = max(((k, ms) for k, (ms, l) in kt.items() if l), key=lambda kv: kv[1])[0]
+    c = r["counters"]
+    alg = _algorithmic_bytes(dom, c, len(w), args.precision)
+    achieved = alg / (step_ms[dom] * 1e-3) / 1e9 if step_ms[dom] > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "algorithmic_bytes": alg,
+                "avg_launch_ms": round(step_ms[dom], 5)}
+    out = {
+        "metric": "particle-steps/s (1M grains, FP32 vs FP64)",
+        "value": round(value, 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": desc, "n": n, "precision": args.precision,
+                   "hash_table_size": r["derived"]["hash_table_size"],
+                   "contact_slots": r["derived"]["contact_slots"]},
+        "counters": {k: c[k] for k in ("candidate_pairs", "contacts", "bonded", "hertz",
+                                       "wall_contacts", "bonds_broken_total", "overflow")},
+        "kernel_ms_per_launch": {k: round(v, 5) for k, v in step_ms.items()},
+        "kernel_share": {k: round(ms / total_kernel_ms, 4) for k, (ms, l) in kt.items() if l},
+        "ms_per_step_without_events": round(1e3 * r["elapsed_no_events"] / args.steps, 5),
+        "roofline": roofline,
+    }
+    if args.compare_precision:
+        other = "mixed" if args.precision == "fp64" else "fp64"
+        r2 = run_engine(cfg, g, w, bonded, other, args.steps, args.warmup)
+        out[f"value_{other}"] = round(n * args.steps / r2["elapsed"], 1)
+        out[f"ms_per_step_{other}"] = round(1e3 * r2["elapsed"] / args.steps, 5)
+    if not args.no_cpu_baseline:
+        v, dt = cpu_baseline(cfg, g, w, bonded, args.cpu_steps)
+        out["cpu_baseline"] = {"value": round(v, 1), "unit": "particle-steps/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"same workload ({n} grains), {args.cpu_steps} steps after "
+                                         f"1 warm-up step, serial FP64 oracle, {dt:.1f} s"}
+    out["setup_s"] = round(t_build, 2)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -205,6 +205,9 @@ typedef struct dem_kernel_times {
 int32_t dem_set_timing(dem_scene_t* s, int32_t on);
 int32_t dem_get_kernel_times(dem_scene_t* s, dem_kernel_times_t* out);
 const char* dem_kernel_name(int32_t k);
+/* Plain device-to-device copy kernel over nbytes (rounded down to 32-byte units), run repeats
+ * times after one warm-up; reports the mean duration.  Used to calibrate HBM byte counters. */
+int32_t dem_debug_copy(int64_t nbytes, int32_t repeats, double* ms_per_copy);
 
 /* ---- synthetic inputs (host only; no counterpart in the reference) -------------------------
  * Loose polydisperse packing by random sequential addition: log-normal radii (median, sigma)

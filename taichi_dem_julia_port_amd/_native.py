@@ -158,6 +158,7 @@ SIGNATURES = [
     ("dem_set_timing", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("dem_get_kernel_times", ctypes.c_int32, [ctypes.c_void_p, P(DemKernelTimes)]),
     ("dem_kernel_name", ctypes.c_char_p, [ctypes.c_int32]),
+    ("dem_debug_copy", ctypes.c_int32, [ctypes.c_int64, ctypes.c_int32, P(ctypes.c_double)]),
     ("dem_generate_pour", ctypes.c_int32, [ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_uint64, P(DemGrainIn)]),

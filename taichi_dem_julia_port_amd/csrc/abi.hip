@@ -57,8 +57,8 @@ void free_scene(dem_scene* s) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (s->h_ctr) hipHostFree(s->h_ctr);
-    for (int k = 0; k < 128; ++k)
-        if (s->ev[k]) hipEventDestroy(s->ev[k]);
+    for (hipEvent_t e : s->ev)
+        if (e) hipEventDestroy(e);
     if (s->stream) hipStreamDestroy(s->stream);
     delete[] s->h_id; delete[] s->h_gid; delete[] s->h_mid; delete[] s->h_V; delete[] s->h_m;
     delete[] s->h_r; delete[] s->h_r0; delete[] s->h_I;
@@ -72,8 +72,10 @@ int32_t sync_counters(dem_scene* s) {
     return DEM_OK;
 }
 
-int32_t collect_timing(dem_scene* s) {
-    if (!s->timing || s->nev == 0) return DEM_OK;
+}  // namespace
+
+int32_t dem::timing_flush(dem_scene* s) {
+    if (s->nev == 0) return DEM_OK;
     HIPCHK(s, hipStreamSynchronize(s->stream));
     for (int k = 0; k < s->nev; ++k) {
         float ms = 0.f;
@@ -84,6 +86,10 @@ int32_t collect_timing(dem_scene* s) {
     s->nev = 0;
     return DEM_OK;
 }
+
+namespace {
+
+int32_t collect_timing(dem_scene* s) { return s->timing ? timing_flush(s) : DEM_OK; }
 
 // Reallocate empty contact rows with k slots per particle (before the first step only).
 int32_t resize_rows(dem_scene* s, int32_t k) {
@@ -331,10 +337,6 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
         HIPCHK(s, run_contact_detection(s, false));
         HIPCHK(s, run_force_integrate(s, t == nsteps - 1));
         s->steps++;
-        if (s->timing) {
-            int32_t rc = collect_timing(s);
-            if (rc) return rc;
-        }
     }
     int32_t rc = sync_counters(s);
     if (rc) return rc;
@@ -638,11 +640,41 @@ int32_t dem_debug_scan(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* t
     return DEM_OK;
 }
 
+int32_t dem_debug_copy(int64_t nbytes, int32_t repeats, double* ms_per_copy) {
+    if (nbytes <= 0 || repeats <= 0) return DEM_ERR_INVALID;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, nbytes);
+    if (e == hipSuccess) e = hipMalloc(&b, nbytes);
+    if (e == hipSuccess) e = hipMemset(a, 1, nbytes);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = run_copy(a, b, nbytes, nullptr);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, nullptr);
+    for (int r = 0; r < repeats && e == hipSuccess; ++r) e = run_copy(a, b, nbytes, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(e1, nullptr);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (e != hipSuccess) { g_create_err = std::string("dem_debug_copy: ") + hipGetErrorString(e); return DEM_ERR_HIP; }
+    if (ms_per_copy) *ms_per_copy = ms / repeats;
+    return DEM_OK;
+}
+
 int32_t dem_set_timing(dem_scene_t* s, int32_t on) {
     if (!s) return DEM_ERR_INVALID;
     HIPCHK(s, hipSetDevice(s->device));
-    if (on && !s->ev[0])
-        for (int k = 0; k < 128; ++k) HIPCHK(s, hipEventCreate(&s->ev[k]));
+    if (on && s->ev.empty()) {
+        const int cap = 4096;
+        s->ev.assign(2 * cap, nullptr);
+        s->ev_kernel.assign(cap, 0);
+        for (auto& e : s->ev) HIPCHK(s, hipEventCreate(&e));
+    }
+    if (!on && s->nev) { int32_t rc = timing_flush(s); if (rc) return rc; }
     s->timing = on != 0;
     s->nev = 0;
     for (int k = 0; k < DEM_NUM_KERNELS; ++k) { s->kms[k] = 0.0; s->klaunch[k] = 0; }

@@ -4,6 +4,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/demhip.h"
 #include "dem_device.hpp"
@@ -135,10 +136,10 @@ struct dem_scene {
     double* h_I = nullptr;
     unsigned long long* h_ctr = nullptr;  // pinned
     std::string err;
-    // timing
+    // timing: an event pair around every launch, collected when the pool fills or on request
     bool timing = false;
-    hipEvent_t ev[2 * 64] = {};
-    int ev_kernel[64] = {};
+    std::vector<hipEvent_t> ev;   // 2 * capacity
+    std::vector<int> ev_kernel;   // capacity
     int nev = 0;
     double kms[DEM_NUM_KERNELS] = {};
     int64_t klaunch[DEM_NUM_KERNELS] = {};
@@ -158,4 +159,6 @@ hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint3
                               const double* state_dev, int64_t m);
 void timing_begin(dem_scene* s, int kernel);
 void timing_end(dem_scene* s);
+int32_t timing_flush(dem_scene* s);
+hipError_t run_copy(const void* src, void* dst, int64_t nbytes, hipStream_t stream);  // abi.hip: collect elapsed times of recorded pairs
 }  // namespace dem

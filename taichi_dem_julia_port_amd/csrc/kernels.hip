@@ -649,6 +649,12 @@ k_write_rows(const Params P, int64_t m, const uint64_t* __restrict__ dst,
     for (int c = 0; c < S_NCOMP; ++c) st[c * cs + d] = state[e * S_NCOMP + c];
 }
 
+__global__ void __launch_bounds__(TPB)
+k_copy(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
+    for (int64_t k = (int64_t)blockIdx.x * TPB + threadIdx.x; k < n; k += (int64_t)gridDim.x * TPB)
+        dst[k] = src[k];
+}
+
 inline dim3 grid_for(int64_t n, int tpb) { return dim3((uint32_t)((n + tpb - 1) / tpb)); }
 
 }  // namespace
@@ -657,12 +663,13 @@ inline dim3 grid_for(int64_t n, int tpb) { return dim3((uint32_t)((n + tpb - 1) 
 // timing helpers
 // ------------------------------------------------------------------------------------------
 void timing_begin(dem_scene* s, int kernel) {
-    if (!s->timing || s->nev >= 64) return;
+    if (!s->timing) return;
+    if (s->nev >= (int)s->ev_kernel.size()) timing_flush(s);
     hipEventRecord(s->ev[2 * s->nev], s->stream);
     s->ev_kernel[s->nev] = kernel;
 }
 void timing_end(dem_scene* s) {
-    if (!s->timing || s->nev >= 64) return;
+    if (!s->timing) return;
     hipEventRecord(s->ev[2 * s->nev + 1], s->stream);
     s->nev++;
 }
@@ -818,6 +825,12 @@ hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint3
     if (m <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_write_rows, grid_for(m, TPB), dim3(TPB), 0, s->stream, s->P, m, dst_dev,
                        word_dev, state_dev, s->d.pq[0], s->d.st[0]);
+    return hipGetLastError();
+}
+
+hipError_t run_copy(const void* src, void* dst, int64_t nbytes, hipStream_t stream) {
+    const int64_t n = nbytes / (int64_t)sizeof(double4);
+    hipLaunchKernelGGL(k_copy, dim3(2048), dim3(TPB), 0, stream, (const double4*)src, (double4*)dst, n);
     return hipGetLastError();
 }
 
