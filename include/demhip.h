@@ -120,10 +120,15 @@ typedef struct dem_derived {
 
 typedef struct dem_options {
     int32_t precision;     /* enum dem_precision */
-    int32_t contact_slots; /* contacts per particle (both roles); 0 = max_coordinate_number */
+    int32_t contact_slots; /* contacts per particle (both roles); 0 = max(32, 2 max_coordinate_number) */
     int32_t device;        /* HIP device ordinal */
-    int32_t pad_;
+    int32_t flags;         /* DEM_FLAG_* */
 } dem_options_t;
+
+/* Broad phase owned by the lower particle index, as in the reference (kernels.jl:31), instead of
+ * the lower sorted slot; the engine also falls back to it when the configuration does not
+ * guarantee that touching pairs lie in both particles' windows. */
+#define DEM_FLAG_INDEX_OWNED_BROAD_PHASE 1
 
 /* Per-step counters (values of the last step unless marked cumulative). */
 typedef struct dem_counters {

@@ -92,7 +92,10 @@ class DemDerived(ctypes.Structure):
 
 class DemOptions(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("contact_slots", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("pad_", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+DEM_FLAG_INDEX_OWNED_BROAD_PHASE = 1
 
 
 class DemCounters(ctypes.Structure):

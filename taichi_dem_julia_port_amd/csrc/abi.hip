@@ -199,6 +199,11 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     der.contact_slots = K;
     der.total_steps = (int64_t)trunc(cfg->target_time / cfg->dt);
     der.save_per_steps = (int64_t)trunc(cfg->saving_interval_time / cfg->dt);
+    {
+        const double reach = 2.0 * rmax_r * std::max(1.0, cfg->particle_contact_radius_multiplier);
+        s->slot_owned = reach * (1.0 + 1e-9) < 0.5 * der.cell_size &&
+                        !(o.flags & DEM_FLAG_INDEX_OWNED_BROAD_PHASE);
+    }
     if (!(der.cell_size > 0.0) || !std::isfinite(der.cell_size)) {
         free_scene(s);
         return fail(nullptr, DEM_ERR_INVALID, "derived cell size is not positive and finite");
@@ -233,6 +238,13 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
         P.walls[j].mid = walls[j].mid;
     }
     P.margin_rel = 0.f;
+    {
+        const double E = cfg->particle_elastic_modulus, nu = cfg->particle_poisson_ratio;
+        P.ys_pp = (float)(1.0 / ((1.0 - nu * nu) / E + (1.0 - nu * nu) / E));
+        P.gs_pp = (float)(1.0 / (2.0 * (2.0 - nu) * (1.0 + nu) / E + 2.0 * (2.0 - nu) * (1.0 + nu) / E));
+        P.beta_pp = (float)pp.beta;
+        P.mu_pp = (float)pp.mu;
+    }
 
     // ---- device allocation -------------------------------------------------------------------
     DevState& d = s->d;
@@ -316,7 +328,7 @@ int32_t dem_initialize(dem_scene_t* s) {
         return fail(s, DEM_ERR_STATE, "dem_initialize must be called once, before the first step");
     HIPCHK(s, hipSetDevice(s->device));
     while (true) {
-        HIPCHK(s, run_contact_detection(s, true));
+        HIPCHK(s, run_contact_detection(s, true, true));
         int32_t rc = sync_counters(s);
         if (rc) return rc;
         if ((rc = collect_timing(s))) return rc;
@@ -334,7 +346,7 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
     if (!s || nsteps < 0) return DEM_ERR_INVALID;
     HIPCHK(s, hipSetDevice(s->device));
     for (int64_t t = 0; t < nsteps; ++t) {
-        HIPCHK(s, run_contact_detection(s, false));
+        HIPCHK(s, run_contact_detection(s, false, t == nsteps - 1));
         HIPCHK(s, run_force_integrate(s, t == nsteps - 1));
         s->steps++;
     }

@@ -57,6 +57,8 @@ struct Params {
     SurfaceP surf[2][2];
     WallP walls[MAX_WALLS];
     float margin_rel;  // FP32 candidate filter slack (mixed precision)
+    // particle-particle Hertz-Mindlin constants in FP32 (mixed precision)
+    float ys_pp, gs_pp, beta_pp, mu_pp;
 };
 
 struct ScanWorkspace {
@@ -119,6 +121,7 @@ struct dem_scene {
     int32_t K = 0;
     int cur = 0;
     bool initialized = false;
+    bool slot_owned = true;  // broad phase owned by sorted slot (needs touching => mutual windows)
     int64_t steps = 0;
     uint64_t broken_total_host = 0;
     dem_config_t cfg{};
@@ -148,7 +151,7 @@ struct dem_scene {
 namespace dem {
 // kernel launchers (kernels.hip)
 enum KernelId { K_HASH = 0, K_SCAN, K_SCATTER, K_BROAD, K_MERGE, K_FORCE, K_INIT, K_OTHER };
-hipError_t run_contact_detection(dem_scene* s, bool init);
+hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates);
 hipError_t run_force_integrate(dem_scene* s, bool write_forces);
 hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc, uint32_t* nbr);
 hipError_t run_debug_sort(dem_scene* s);

@@ -129,6 +129,95 @@ k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __rest
     if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
 }
 
+// Slot-owned broad phase.  A touching pair lies inside both particles' windows (the window
+// reaches at least cs/2 = 2 r_max past a particle on every axis, and r_i + r_j <= 2 max r <
+// 2 r_max), so it can be owned by the particle with the lower sorted slot: each thread scans
+// only slots k > t of its window buckets and reads the partner's index only for touching pairs.
+// The reference's pair rule -- (lo, hi) is a candidate iff hi's bucket is one of lo's window
+// buckets -- is then checked exactly: when the owner is hi, lo's window is recomputed.
+template <bool INIT, bool MIXED>
+__global__ void __launch_bounds__(TPB)
+k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
+             const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
+             const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
+             unsigned long long* ctr) {
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint64_t found = 0;
+    uint32_t overflow = 0;
+    if (t < P.n) {
+        const double4 me = spos[t];
+        const d3 xp = mk3(me.x, me.y, me.z);
+        uint32_t p = 0xffffffffu;  // own index, loaded on the first touching pair
+        float4 me32 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float slack = 0.f;
+        if (MIXED) {
+            me32 = spos32[t];
+            slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
+        }
+        uint32_t w[8];
+        window_of(P.grid, xp, w);
+        const uint32_t own = w[0];
+#pragma unroll 1
+        for (int wi = 0; wi < 8; ++wi) {
+            const uint32_t b = w[wi];
+            bool dup = false;
+            for (int wj = 0; wj < wi; ++wj) dup |= (w[wj] == b);
+            if (dup) continue;
+            uint32_t k = bstart[b];
+            const uint32_t s1 = bstart[b + 1];
+            if (k <= (uint32_t)t) k = (uint32_t)t + 1;
+            for (; k < s1; ++k) {
+                if (MIXED) {
+                    const float4 o = spos32[k];
+                    const float dx = me32.x - o.x, dy = me32.y - o.y, dz = me32.z - o.z;
+                    const float d2 = dx * dx + dy * dy + dz * dz;
+                    const float rs = me32.w + o.w;
+                    const float thr = rs * (1.0f + 64.0f * FLT_EPSILON) + slack;
+                    if (!INIT && d2 > thr * thr) continue;
+                }
+                const double4 o = spos[k];
+                if (p == 0xffffffffu) p = sidx[t];
+                const uint32_t q = sidx[k];
+                const uint32_t lo = p < q ? p : q, hi = p < q ? q : p;
+                const double4 xl = p < q ? me : o, xh = p < q ? o : me;
+                const double rl = INIT ? r0[lo] : xl.w, rh = INIT ? r0[hi] : xh.w;
+                // kernels.jl:103 (norm(k_i - k_j) < r_i + r_j), kernels.jl:72 with r0 at t = 0
+                if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
+                if (p > q) {  // owner is hi: is hi's bucket in lo's window?
+                    uint32_t wl[8];
+                    window_of(P.grid, mk3(o.x, o.y, o.z), wl);
+                    bool in = false;
+#pragma unroll
+                    for (int m = 0; m < 8; ++m) in |= (wl[m] == own);
+                    if (!in) continue;
+                }
+                ++found;
+                row_append(tcnt, touch, P.n, P.K, p, q, overflow);
+                row_append(tcnt, touch, P.n, P.K, q, p, overflow);
+            }
+        }
+    }
+    wave_add(&ctr[CT_TOUCH], found);
+    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+}
+
+// Candidate-pair count of the reference's cp_list (kernels.jl:24-37): a diagnostic, evaluated on
+// the last step of each dem_step call.
+__global__ void __launch_bounds__(TPB)
+k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx,
+             const uint32_t* __restrict__ bstart, unsigned long long* ctr) {
+    const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint64_t c = 0;
+    if (i < P.n) {
+        const double4 x = pos[i];
+        uint32_t w[8];
+        window_of(P.grid, mk3(x.x, x.y, x.z), w);
+        for (int wi = 0; wi < 8; ++wi)
+            for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += (uint32_t)i < sidx[k];
+    }
+    wave_add(&ctr[CT_CAND], c);
+}
+
 // ------------------------------------------------------------------------------------------
 // merge: persistent contact rows vs the new touching set (update_contacts! / init_bonds!)
 // ------------------------------------------------------------------------------------------
@@ -316,6 +405,57 @@ __device__ __forceinline__ HMOut hertz_mindlin(double gap, double dn, d3 vc, dou
     return o;
 }
 
+// ---- mixed precision: FP32 contact frame and Hertz-Mindlin law, FP64 spring state ----------
+struct f3 { float x, y, z; };
+__device__ __forceinline__ f3 mkf(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 tof(d3 a) { return mkf((float)a.x, (float)a.y, (float)a.z); }
+__device__ __forceinline__ d3 tod(f3 a) { return mk3((double)a.x, (double)a.y, (double)a.z); }
+__device__ __forceinline__ f3 fsub(f3 a, f3 b) { return mkf(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 fadd(f3 a, f3 b) { return mkf(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 fscl(f3 a, float s) { return mkf(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 fcross(f3 a, f3 b) {
+    return mkf(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float fnorm(f3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+struct fm3 { float a[3][3]; };
+__device__ __forceinline__ f3 fmul(const fm3& R, f3 u) {
+    return mkf(R.a[0][0] * u.x + R.a[0][1] * u.y + R.a[0][2] * u.z,
+               R.a[1][0] * u.x + R.a[1][1] * u.y + R.a[1][2] * u.z,
+               R.a[2][0] * u.x + R.a[2][1] * u.y + R.a[2][2] * u.z);
+}
+// R^T u: R is a rotation, so its transpose is its inverse to FP32 precision
+__device__ __forceinline__ f3 fmul_t(const fm3& R, f3 u) {
+    return mkf(R.a[0][0] * u.x + R.a[1][0] * u.y + R.a[2][0] * u.z,
+               R.a[0][1] * u.x + R.a[1][1] * u.y + R.a[2][1] * u.z,
+               R.a[0][2] * u.x + R.a[1][2] * u.y + R.a[2][2] * u.z);
+}
+// The same frame as contact_frame (normal -> local x), in FP32; near-degenerate normals
+// (|a x ex| < 1e-6) take the reference's axis-aligned branch.
+__device__ __forceinline__ fm3 contact_frame_f32(f3 a, float tol) {
+    const f3 v = mkf(0.0f, a.z, -a.y);  // a x (1, 0, 0)
+    const float s = fnorm(v);
+    const float c = a.x;
+    fm3 R;
+    if (s < tol) {
+        const float sg = c > 0.0f ? 1.0f : -1.0f;
+        R.a[0][0] = sg;   R.a[0][1] = 0.0f; R.a[0][2] = 0.0f;
+        R.a[1][0] = 0.0f; R.a[1][1] = 1.0f; R.a[1][2] = 0.0f;
+        R.a[2][0] = 0.0f; R.a[2][1] = 0.0f; R.a[2][2] = sg;
+        return R;
+    }
+    float K[3][3] = {{0.0f, -v.z, v.y}, {v.z, 0.0f, -v.x}, {-v.y, v.x, 0.0f}};
+    const float f = (1.0f - c) / (s * s);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const float k2 = K[r][0] * K[0][q] + K[r][1] * K[1][q] + K[r][2] * K[2][q];
+            R.a[r][q] = ((r == q) ? 1.0f : 0.0f) + K[r][q] + k2 * f;
+        }
+    return R;
+}
+
 // Neumaier-compensated FP32 accumulator (mixed precision force sums).
 struct CSum3 {
     float s[3] = {0.f, 0.f, 0.f};
@@ -327,6 +467,7 @@ struct CSum3 {
         s[k] = t;
     }
     __device__ __forceinline__ void add(d3 v) { add(0, (float)v.x); add(1, (float)v.y); add(2, (float)v.z); }
+    __device__ __forceinline__ void add(f3 v) { add(0, v.x); add(1, v.y); add(2, v.z); }
     __device__ __forceinline__ d3 value() const {
         return mk3((double)s[0] + (double)c[0], (double)s[1] + (double)c[1], (double)s[2] + (double)c[2]);
     }
@@ -373,11 +514,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             const d3 wj = plo ? mk3(Wq.x, Wq.y, Wq.z) : wp;
             const double ri = plo ? X.w : Xq.w, rj = plo ? Xq.w : X.w;
             const double mi = plo ? Vv.w : Vq.w, mj = plo ? Vq.w : Vv.w;
-            const d3 a = normalize(kj - ki);
-            const m3 R = contact_frame(a, P.tol);
             const double Li = norm(kj - ki);
             if (word & BONDED) {
                 // EBPM bond (kernels.jl:170-239)
+                const m3 R = contact_frame(normalize(kj - ki), P.tol);
                 const d3 kc = (ki + kj) * 0.5;
                 const d3 di = mul(R, vi * dt), dj = mul(R, vj * dt);
                 const d3 thi = mul(R, wi * dt), thj = mul(R, wj * dt);
@@ -430,8 +570,57 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
                 st[S_SX * cs + e] = 0.0; st[S_SY * cs + e] = 0.0; st[S_SZ * cs + e] = 0.0;
                 if (plo) { ++active; ++active_b; }
+            } else if (MIXED) {
+                // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance
+                const double gap = Li - ri - rj;
+                const double dn = fabs(gap);
+                const f3 d32 = tof(kj - ki);
+                const f3 a32 = fscl(d32, 1.0f / (float)Li);
+                const fm3 R32 = contact_frame_f32(a32, 1e-6f);
+                const float lever = (float)(ri - dn);
+                const f3 kip = fscl(a32, lever);
+                const f3 kjp = fsub(kip, d32);
+                const f3 vci = fadd(fcross(tof(wi), kip), tof(vi));
+                const f3 vcj = fadd(fcross(tof(wj), kjp), tof(vj));
+                const f3 vc = fmul(R32, fsub(vcj, vci));
+                const float Rs = 1.0f / (1.0f / (float)ri + 1.0f / (float)rj);
+                const float ms = 1.0f / (1.0f / (float)mi + 1.0f / (float)mj);
+                const float srd = sqrtf(Rs * (float)dn);
+                const float kn = (4.0f / 3.0f) * P.ys_pp * srd;
+                const float kt = 8.0f * P.gs_pp * srd;
+                const float gn = -2.0f * P.beta_pp * sqrtf((5.0f / 6.0f) * (2.0f * P.ys_pp * srd) * ms);
+                const float gt = -2.0f * P.beta_pp * sqrtf((5.0f / 6.0f) * kt * ms);
+                // the tangential spring accumulates in FP64: v dt increments vanish in FP32
+                d3 sv = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
+                sv = sv + mk3((double)vc.x * 0.0, (double)vc.y * dt, (double)vc.z * dt);
+                const float F1 = -kn * (float)gap - gn * vc.x;
+                const f3 Ft = fscl(tof(sv), -kt);
+                const float nFt = fnorm(Ft);
+                float F2, F3;
+                if (nFt >= P.mu_pp * F1) {
+                    const float ratio = P.mu_pp * F1 / nFt;
+                    F2 = ratio * Ft.y;
+                    F3 = ratio * Ft.z;
+                    sv = mk3(sv.x, (double)(F2 / kt), (double)(F3 / kt));
+                } else {
+                    F2 = Ft.y - gt * vc.y;
+                    F3 = Ft.z - gt * vc.z;
+                }
+                const f3 Fl = mkf(F1, F2, F3);
+                const f3 Fg = fmul_t(R32, mkf(-F1, -F2, -F3));
+                if (plo) { FH.add(Fg); TH.add(fcross(kip, Fg)); }
+                else { FH.add(mkf(-Fg.x, -Fg.y, -Fg.z)); TH.add(fcross(kjp, mkf(-Fg.x, -Fg.y, -Fg.z))); }
+                const d3 kc = ki + tod(kip);
+                st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                st[S_FX * cs + e] = (double)Fl.x; st[S_FY * cs + e] = (double)Fl.y; st[S_FZ * cs + e] = (double)Fl.z;
+                st[S_SX * cs + e] = sv.x; st[S_SY * cs + e] = sv.y; st[S_SZ * cs + e] = sv.z;
+                if (plo) {
+                    ++active;
+                    if (gap > 0 || gn < 0.0f || gt < 0.0f) ++fatal;
+                }
             } else {
                 // Hertz-Mindlin (kernels.jl:240-313)
+                const m3 R = contact_frame(normalize(kj - ki), P.tol);
                 const double gap = Li - ri - rj;
                 const double dn = fabs(gap);
                 const d3 kc = ki + normalize(kj - ki) * (ri - dn);
@@ -447,13 +636,8 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                                               P.matE[0], Rs, ms, sf, dt, s0);
                 const m3 Ri = inverse(R);
                 const d3 Fg = mul(Ri, -h.F);
-                if (MIXED) {
-                    if (plo) { FH.add(Fg); TH.add(cross(kip, Fg)); }
-                    else { FH.add(-Fg); TH.add(cross(kjp, -Fg)); }
-                } else {
-                    if (plo) { F = F + Fg; T = T + cross(kip, Fg); }
-                    else { F = F + (-Fg); T = T + cross(kjp, -Fg); }
-                }
+                if (plo) { F = F + Fg; T = T + cross(kip, Fg); }
+                else { F = F + (-Fg); T = T + cross(kjp, -Fg); }
                 st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
                 st[S_FX * cs + e] = h.F.x; st[S_FY * cs + e] = h.F.y; st[S_FZ * cs + e] = h.F.z;
                 st[S_SX * cs + e] = h.s.x; st[S_SY * cs + e] = h.s.y; st[S_SZ * cs + e] = h.s.z;
@@ -700,7 +884,7 @@ static hipError_t sort_particles(dem_scene* s, bool reset_counters) {
     return hipSuccess;
 }
 
-hipError_t run_contact_detection(dem_scene* s, bool init) {
+hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
@@ -708,7 +892,24 @@ hipError_t run_contact_detection(dem_scene* s, bool init) {
     if (e != hipSuccess) return e;
     const bool mixed = s->precision == DEM_PRECISION_MIXED && !init;
     const dim3 g = grid_for(P.n, TPB);
-    if (init) {
+    if (count_candidates)
+        LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
+                                              d.sidx, d.bstart, d.ctr));
+    if (s->slot_owned) {
+        if (init) {
+            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_slot<true, false>), g, dim3(TPB), 0, s->stream,
+                                                 P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                                 d.touch, d.ctr));
+        } else if (mixed) {
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_slot<false, true>), g, dim3(TPB), 0, s->stream,
+                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                                  d.touch, d.ctr));
+        } else {
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_slot<false, false>), g, dim3(TPB), 0, s->stream,
+                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                                  d.touch, d.ctr));
+        }
+    } else if (init) {
         LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad<true, false>), g, dim3(TPB), 0, s->stream, P,
                                              d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
                                              d.touch, d.ctr));

@@ -42,7 +42,8 @@ class Scene:
     """One simulation scene on the GPU (GlobalData)."""
 
     def __init__(self, cfg: DEMConfig, grains: np.ndarray, walls: Optional[Sequence] = None,
-                 precision: str = "fp64", contact_slots: int = 0, device: int = 0):
+                 precision: str = "fp64", contact_slots: int = 0, device: int = 0,
+                 index_owned_broad_phase: bool = False):
         self._lib = N.load_library()
         self.cfg = cfg
         grains = np.ascontiguousarray(grains, dtype=GRAIN_IN_DTYPE)
@@ -53,7 +54,8 @@ class Scene:
         self.walls = np.ascontiguousarray(w)
         self.nwall = len(self.walls)
         self._c_cfg = cfg.to_c()
-        opt = N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), 0)
+        flags = N.DEM_FLAG_INDEX_OWNED_BROAD_PHASE if index_owned_broad_phase else 0
+        opt = N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), flags)
         h = ctypes.c_void_p()
         rc = self._lib.dem_create(ctypes.byref(self._c_cfg), _ptr(self.walls, N.DemWall),
                                   self.nwall, _ptr(grains, N.DemGrainIn), self.n,
