@@ -24,25 +24,41 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
 def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> float:
-    """Algorithmic HBM bytes of one launch, from the per-unit figures of the reference design:
-    per particle 360 B split over the kernels that touch particle arrays, per hash bucket 12 B,
-    per candidate pair 20 B (cp_list write 8 + read 8 + one 4-B state probe), per Hertz contact
-    104 B, per bonded contact 176 B, per wall contact 48 B."""
-    n, H, P = c["n"], c["hash_table_size"], c["candidate_pairs"]
+    """Compulsory HBM bytes of one launch of ``kernel`` in this engine's design (each array the
+    kernel must touch, read or written once):
+      broad_phase     sorted mirror read once (32 B/grain, +16 B FP32 copy in mixed precision),
+                      cell starts (4 B/cell), sorted index of touching pairs (4 B), and the two
+                      row appends per touching pair (4 B counter + 4 B entry each)
+      force_integrate grain state read 128 B + written 128 B, row counts 5 B, wall flags 4 B;
+                      per Hertz contact 2 rows x (partner 4 + spring 24 r/w + k,F 48 w),
+                      per bond 2 rows x (partner 4 + F,ti,tj 72 r/w + k 24 w), per wall contact 32 B
+      hash            position read 32 B, key + rank written 8 B, one counter atomic 4 B
+      scatter         key + rank read 8 B, position read 32 B, sorted index + position written 36 B
+      scan            cell counts read + zeroed 8 B, starts written 4 B per cell
+      merge           touch count 4 B + row count and selector 5 B per grain"""
+    n, P = c["n"], c["candidate_pairs"]
+    cells = c.get("grid_cells") or c["hash_table_size"]
     Ch, Cb, W = c["hertz"], c["bonded"], c["wall_contacts"]
+    touching = Ch + Cb
     if kernel == "broad_phase":
-        # cp_list + state probe per candidate, cell key read, pid read, pair-range count/offset
-        return 20.0 * P + n * (4 + 4 + 16)
+        return n * (32.0 + (16.0 if precision == "mixed" else 0.0)) + 4.0 * cells + touching * (4 + 16)
     if kernel == "force_integrate":
-        # particle state read 128 + write 104, force/moment zero + read 96; contact terms
-        return n * (128 + 104 + 96) + 104.0 * Ch + 176.0 * Cb + 48.0 * W
-    if kernel == "scan":
-        return 12.0 * H
+        return n * (128 + 128 + 5 + 4) + 2 * Ch * (4 + 48 + 48) + 2 * Cb * (4 + 144 + 24) + 32.0 * W
     if kernel == "hash":
-        return n * 4.0
+        return n * 44.0
     if kernel == "scatter":
-        return n * 4.0
+        return n * 76.0
+    if kernel == "scan":
+        return 12.0 * cells
+    if kernel == "merge":
+        return n * 9.0
     return 0.0
+
+
+def _reference_equivalent_bytes(c: dict) -> float:
+    """Bytes the reference design moves for its broad phase: 20 B per cp_list entry (write 8 +
+    read 8 + one 4-B BitTable probe), the per-candidate-pair figure of the measurement plan."""
+    return 20.0 * c["candidate_pairs"]
 
 
 def build_workload(name: str, n: int):
@@ -112,12 +128,24 @@ def main():
                     help="also time the other precision and report both")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="first run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one scene per GPU on one GPU; multi-GPU "
                                    "runs are not part of this build (use --gpus 1)"}))
         sys.exit(2)
 
+    copy_cal = None
+    if args.copy_calibration_mb > 0:
+        import ctypes
+        from taichi_dem_julia_port_amd import _native as N
+        ms = ctypes.c_double(0.0)
+        nbytes = args.copy_calibration_mb << 20
+        rc = N.load_library().dem_debug_copy(nbytes, 3, ctypes.byref(ms))
+        if rc == 0:
+            copy_cal = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms.value,
+                        "GB_s": round(2 * nbytes / (ms.value * 1e-3) / 1e9, 1)}
     t_build = time.perf_counter()
     cfg, g, w, bonded, desc = build_workload(args.workload, args.n)
     t_build = time.perf_counter() - t_build
@@ -135,6 +163,18 @@ def main():
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": None, "algorithmic_bytes": alg,
                 "avg_launch_ms": round(step_ms[dom], 5)}
+    if dom == "broad_phase":
+        # candidates examined are re-read from L2/MALL, not HBM: the kernel's practical limiter
+        roofline["candidates_examined"] = c["examined"]
+        roofline["cache_GB_s"] = round(c["examined"] * (16 if args.precision == "mixed" else 32)
+                                       / (step_ms[dom] * 1e-3) / 1e9, 1)
+        roofline["reference_equivalent_GB_s"] = round(
+            _reference_equivalent_bytes(c) / (step_ms[dom] * 1e-3) / 1e9, 1)
+    per_kernel = {}
+    for k, ms in step_ms.items():
+        b = _algorithmic_bytes(k, c, len(w), args.precision)
+        if b > 0 and ms > 0:
+            per_kernel[k] = round(b / (ms * 1e-3) / 1e9, 1)
     out = {
         "metric": "particle-steps/s (1M grains, FP32 vs FP64)",
         "value": round(value, 1),
@@ -151,12 +191,14 @@ def main():
         "config": {"workload": desc, "n": n, "precision": args.precision,
                    "hash_table_size": r["derived"]["hash_table_size"],
                    "contact_slots": r["derived"]["contact_slots"]},
-        "counters": {k: c[k] for k in ("candidate_pairs", "contacts", "bonded", "hertz",
-                                       "wall_contacts", "bonds_broken_total", "overflow")},
+        "counters": {k: c[k] for k in ("candidate_pairs", "examined", "grid_cells", "contacts",
+                                       "bonded", "hertz", "wall_contacts", "bonds_broken_total",
+                                       "overflow")},
         "kernel_ms_per_launch": {k: round(v, 5) for k, v in step_ms.items()},
         "kernel_share": {k: round(ms / total_kernel_ms, 4) for k, (ms, l) in kt.items() if l},
         "ms_per_step_without_events": round(1e3 * r["elapsed_no_events"] / args.steps, 5),
         "roofline": roofline,
+        "kernel_GB_s": per_kernel,
     }
     if args.compare_precision:
         other = "mixed" if args.precision == "fp64" else "fp64"
@@ -170,6 +212,8 @@ def main():
                                "sample": f"same workload ({n} grains), {args.cpu_steps} steps after "
                                          f"1 warm-up step, serial FP64 oracle, {dt:.1f} s"}
     out["setup_s"] = round(t_build, 2)
+    if copy_cal:
+        out["copy_calibration"] = copy_cal
     print(json.dumps(out), flush=True)
 
 

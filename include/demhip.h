@@ -129,6 +129,11 @@ typedef struct dem_options {
  * the lower sorted slot; the engine also falls back to it when the configuration does not
  * guarantee that touching pairs lie in both particles' windows. */
 #define DEM_FLAG_INDEX_OWNED_BROAD_PHASE 1
+/* Sort and search on the reference's hashed cell grid (utils.jl:1-16) every step instead of the
+ * engine's dense grid of cs/2 cells (used when the domain is compact; same contact sets). */
+#define DEM_FLAG_REFERENCE_GRID 2
+/* Launch every step's kernels individually instead of replaying a captured two-step HIP graph. */
+#define DEM_FLAG_NO_GRAPH 4
 
 /* Per-step counters (values of the last step unless marked cumulative). */
 typedef struct dem_counters {
@@ -146,6 +151,8 @@ typedef struct dem_counters {
     int64_t bonds_broken_total;/* cumulative */
     int64_t fatal_gap;         /* unbonded active contacts evaluated with gap > 0 */
     int64_t overflow;          /* non-zero: contact capacity exceeded (results invalid) */
+    int64_t examined;          /* candidate pairs examined by the engine's own broad phase */
+    int64_t grid_cells;        /* cells of the engine's dense sort grid (0: reference grid used) */
 } dem_counters_t;
 
 typedef struct dem_scene dem_scene_t;

@@ -96,13 +96,15 @@ class DemOptions(ctypes.Structure):
 
 
 DEM_FLAG_INDEX_OWNED_BROAD_PHASE = 1
+DEM_FLAG_REFERENCE_GRID = 2
+DEM_FLAG_NO_GRAPH = 4
 
 
 class DemCounters(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "step", "n", "hash_table_size", "candidate_pairs", "contacts", "bonded", "hertz",
         "wall_contacts", "created", "deactivated", "bonds_broken", "bonds_broken_total",
-        "fatal_gap", "overflow")]
+        "fatal_gap", "overflow", "examined", "grid_cells")]
 
 
 DEM_NUM_KERNELS = 8

@@ -59,6 +59,8 @@ void free_scene(dem_scene* s) {
     if (s->h_ctr) hipHostFree(s->h_ctr);
     for (hipEvent_t e : s->ev)
         if (e) hipEventDestroy(e);
+    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
+    if (s->graph) hipGraphDestroy(s->graph);
     if (s->stream) hipStreamDestroy(s->stream);
     delete[] s->h_id; delete[] s->h_gid; delete[] s->h_mid; delete[] s->h_V; delete[] s->h_m;
     delete[] s->h_r; delete[] s->h_r0; delete[] s->h_I;
@@ -111,6 +113,8 @@ int32_t resize_rows(dem_scene* s, int32_t k) {
     s->K = k;
     s->P.K = k;
     s->der.contact_slots = k;
+    if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
     return DEM_OK;
 }
 
@@ -165,6 +169,7 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     s->K = K;
     s->device = o.device;
     s->cfg = *cfg;
+    s->use_graph = !(o.flags & DEM_FLAG_NO_GRAPH);
     if (hipSetDevice(o.device) != hipSuccess) {
         delete s;
         return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
@@ -238,6 +243,26 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
         P.walls[j].mid = walls[j].mid;
     }
     P.margin_rel = 0.f;
+    P.bond_mult = cfg->particle_contact_radius_multiplier;
+    // Dense sort/broad-phase grid with cells of cs/2 = 2 r_max when the slot-owned broad phase is
+    // valid and the domain holds at most max(4 n, 2^20) such cells; otherwise the reference's grid.
+    P.dense.cells = 0;
+    if (s->slot_owned && !(o.flags & DEM_FLAG_REFERENCE_GRID)) {
+        const double cell = 0.5 * der.cell_size;
+        double cells = 1.0;
+        int32_t dims[3];
+        for (int a = 0; a < 3; ++a) {
+            const double ext = cfg->domain_max[a] - cfg->domain_min[a];
+            const double dd = std::max(1.0, ceil(ext / cell));
+            dims[a] = (int32_t)std::min(dd, 2.0e9);
+            cells *= dd;
+        }
+        if (cells <= std::max(4.0 * (double)n, (double)(1 << 20)) && cells < 2.0e9) {
+            for (int a = 0; a < 3; ++a) { P.dense.origin[a] = cfg->domain_min[a]; P.dense.dims[a] = dims[a]; }
+            P.dense.inv_cell = 1.0 / cell;
+            P.dense.cells = (uint32_t)cells;
+        }
+    }
     {
         const double E = cfg->particle_elastic_modulus, nu = cfg->particle_poisson_ratio;
         P.ys_pp = (float)(1.0 / ((1.0 - nu * nu) / E + (1.0 - nu * nu) / E));
@@ -266,12 +291,13 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     }
     ALLOC(d.acc, n); ALLOC(d.dw, n); ALLOC(d.ftot, n); ALLOC(d.ttot, n); ALLOC(d.r0, n);
     ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall > 0 ? nwall : 1) * n);
-    ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.bcnt, H + 1); ALLOC(d.bstart, H + 1);
+    const int64_t nkeys = std::max<int64_t>(H, (int64_t)P.dense.cells);
+    ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.bcnt, nkeys + 1); ALLOC(d.bstart, nkeys + 1);
     ALLOC(d.sidx, n); ALLOC(d.spos, n);
     if (o.precision == DEM_PRECISION_MIXED) { ALLOC(d.spos32, n); }
     ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
     ALLOC(d.ctr, CT_NCOUNTERS);
-    const int64_t max_tiles = scan_tiles(std::max<int64_t>(H + 1, n + 1));
+    const int64_t max_tiles = scan_tiles(std::max<int64_t>(nkeys + 1, n + 1));
     ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2);
     d.scan.max_tiles = max_tiles;
     ALLOC(d.scratch_u32, n + 1); ALLOC(d.scratch2_u32, n + 1); ALLOC(d.total, 1);
@@ -342,10 +368,44 @@ int32_t dem_initialize(dem_scene_t* s) {
     return DEM_OK;
 }
 
+// Capture two plain steps (no output writes, no diagnostics) starting from the current buffer.
+static int32_t capture_graph(dem_scene* s) {
+    if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
+    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    const int start = s->cur;
+    HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = run_contact_detection(s, false, false);
+        if (e == hipSuccess) e = run_force_integrate(s, false);
+    }
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(s->stream, &g);
+    s->cur = start;  // capture only recorded the launches
+    if (e != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "graph capture", e); }
+    if (e2 != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "hipStreamEndCapture", e2); }
+    s->graph = g;
+    HIPCHK(s, hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
+    s->graph_parity = start;
+    return DEM_OK;
+}
+
 int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
     if (!s || nsteps < 0) return DEM_ERR_INVALID;
     HIPCHK(s, hipSetDevice(s->device));
-    for (int64_t t = 0; t < nsteps; ++t) {
+    int64_t t = 0;
+    // body: pairs of steps replayed from a captured graph (not while timing individual kernels)
+    if (s->use_graph && !s->timing && nsteps >= 3) {
+        if (!s->graph_exec || s->graph_parity != s->cur) {
+            int32_t rc = capture_graph(s);
+            if (rc) return rc;
+        }
+        for (; t + 2 < nsteps; t += 2) {
+            HIPCHK(s, hipGraphLaunch(s->graph_exec, s->stream));
+            s->steps += 2;
+        }
+    }
+    for (; t < nsteps; ++t) {
         HIPCHK(s, run_contact_detection(s, false, t == nsteps - 1));
         HIPCHK(s, run_force_integrate(s, t == nsteps - 1));
         s->steps++;
@@ -384,6 +444,8 @@ int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
     out->bonds_broken_total = (int64_t)c[CT_BROKEN_TOTAL];
     out->fatal_gap = (int64_t)c[CT_FATAL];
     out->overflow = (int64_t)c[CT_OVERFLOW];
+    out->examined = (int64_t)c[CT_EXAMINED];
+    out->grid_cells = (int64_t)s->P.dense.cells;
     return DEM_OK;
 }
 

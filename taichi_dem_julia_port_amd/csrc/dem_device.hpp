@@ -165,6 +165,46 @@ __device__ __forceinline__ void window_of(const HashGrid& g, d3 p, uint32_t w[8]
     window_of(g, p, w, c, ctr);
 }
 
+// Reference bucket of a position (utils.jl:1-2 minus one).
+__device__ __forceinline__ uint32_t ref_bucket(const HashGrid& g, d3 p) {
+    int32_t c[3];
+    cell_of(g, p, c);
+    return bucket_of(g, c[0], c[1], c[2]);
+}
+
+// Is the (lo, hi) pair in the reference's cp_list: is hi's bucket one of lo's window buckets?
+__device__ __forceinline__ bool ref_candidate(const HashGrid& g, d3 lo, d3 hi) {
+    uint32_t w[8];
+    window_of(g, lo, w);
+    const uint32_t b = ref_bucket(g, hi);
+    bool in = false;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) in |= (w[k] == b);
+    return in;
+}
+
+// Dense cell grid used for sorting and the broad phase when the domain is compact: cells of
+// side cell >= the largest contact distance, linear index x-fastest, positions outside the
+// domain clamped into the border cells (clamping keeps touching pairs in adjacent cells).
+struct DenseGrid {
+    double origin[3];
+    double inv_cell;
+    int32_t dims[3];
+    uint32_t cells;
+};
+
+__device__ __forceinline__ uint32_t dense_cell(const DenseGrid& g, d3 p, int32_t c[3]) {
+    const double r[3] = {p.x, p.y, p.z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        int32_t v = (int32_t)floor((r[a] - g.origin[a]) * g.inv_cell);
+        v = v < 0 ? 0 : v;
+        v = v >= g.dims[a] ? g.dims[a] - 1 : v;
+        c[a] = v;
+    }
+    return (uint32_t)c[0] + (uint32_t)g.dims[0] * ((uint32_t)c[1] + (uint32_t)g.dims[1] * (uint32_t)c[2]);
+}
+
 // ---- wave-level helpers ----------------------------------------------------------------------
 
 __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {

@@ -31,6 +31,7 @@ enum CounterIdx {
     CT_FATAL,        // gap > 0 or negative damping coefficient evaluations
     CT_ACTIVE,       // active contacts after the step (counted once per pair)
     CT_ACTIVE_BOND,  // active bonds after the step
+    CT_EXAMINED,     // candidate pairs examined by the engine's broad phase
     CT_NSTEP,        // counters below are reset at the start of every step
     CT_BROKEN_TOTAL = 12,  // cumulative
     CT_OVERFLOW = 13,      // sticky: a capacity overflow invalidates the scene
@@ -47,7 +48,9 @@ struct WallP { double n[3]; double d; int32_t mid; int32_t pad; };
 
 // Everything a kernel needs by value.
 struct Params {
-    HashGrid grid;
+    HashGrid grid;      // the reference's hashed cell grid (utils.jl:1-16)
+    DenseGrid dense;    // fine dense grid for sorting/broad phase (dense.cells == 0: unused)
+    double bond_mult;   // particle_contact_radius_multiplier (r0 = r * bond_mult)
     int64_t n;
     int32_t K;       // contact slots per particle
     int32_t nwall;
@@ -139,6 +142,12 @@ struct dem_scene {
     double* h_I = nullptr;
     unsigned long long* h_ctr = nullptr;  // pinned
     std::string err;
+    // two steps captured as one HIP graph (the particle buffers alternate, so two steps restore
+    // the buffer parity); replayed for the body of every dem_step call
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_parity = -1;   // value of cur the graph was captured from
+    bool use_graph = true;
     // timing: an event pair around every launch, collected when the pool fills or on request
     bool timing = false;
     std::vector<hipEvent_t> ev;   // 2 * capacity

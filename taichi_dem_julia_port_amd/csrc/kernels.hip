@@ -33,6 +33,7 @@ __device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 
 // ------------------------------------------------------------------------------------------
 // hashing and counting sort
 // ------------------------------------------------------------------------------------------
+template <bool DENSE>
 __global__ void __launch_bounds__(TPB)
 k_hash(const Params P, const double4* __restrict__ pos, uint32_t* __restrict__ hid,
        uint32_t* __restrict__ rank, uint32_t* __restrict__ bcnt, unsigned long long* ctr,
@@ -42,8 +43,8 @@ k_hash(const Params P, const double4* __restrict__ pos, uint32_t* __restrict__ h
     if (p >= P.n) return;
     const double4 x = pos[p];
     int32_t c[3];
-    cell_of(P.grid, mk3(x.x, x.y, x.z), c);
-    const uint32_t b = bucket_of(P.grid, c[0], c[1], c[2]);
+    const uint32_t b = DENSE ? dense_cell(P.dense, mk3(x.x, x.y, x.z), c)
+                             : ref_bucket(P.grid, mk3(x.x, x.y, x.z));
     hid[p] = b;
     rank[p] = atomicAdd(&bcnt[b], 1u);
 }
@@ -201,6 +202,84 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
     if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
 }
 
+// Dense-grid broad phase.  Cells have side cs/2 = 2 r_max, at least the largest contact distance,
+// so every touching pair lies in the 27-cell stencil; slots are sorted by cell index (x fastest),
+// so the three x-neighbours of a cell row form one contiguous slot range and the stencil is 9
+// ranges.  Pairs are owned by the lower sorted slot.  A squared-distance pre-test skips the square
+// root for candidates clearly apart; survivors get the reference's exact test
+// (norm(k_i - k_j) < r_i + r_j, kernels.jl:103; r0 at t = 0, kernels.jl:72) and the reference's
+// candidate rule (hi's bucket in lo's window, kernels.jl:24-58), so the contact set is the
+// reference's.
+template <bool INIT, bool MIXED>
+__global__ void __launch_bounds__(TPB)
+k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
+              const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
+              const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
+              unsigned long long* ctr) {
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint64_t found = 0, examined = 0;
+    uint32_t overflow = 0;
+    if (t < P.n) {
+        const DenseGrid& G = P.dense;
+        const double4 me = spos[t];
+        int32_t c[3];
+        dense_cell(G, mk3(me.x, me.y, me.z), c);
+        uint32_t p = 0xffffffffu;
+        float4 me32 = make_float4(0.f, 0.f, 0.f, 0.f);
+        float slack = 0.f;
+        if (MIXED) {
+            me32 = spos32[t];
+            slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
+        }
+        const double mult = INIT ? P.bond_mult * (1.0 + 1e-9) : 1.0;
+        const int32_t x0 = c[0] > 0 ? c[0] - 1 : 0;
+        const int32_t x1 = c[0] + 1 < G.dims[0] ? c[0] + 1 : G.dims[0] - 1;
+#pragma unroll 1
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int32_t z = c[2] + dz;
+            if (z < 0 || z >= G.dims[2]) continue;
+#pragma unroll 1
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int32_t y = c[1] + dy;
+                if (y < 0 || y >= G.dims[1]) continue;
+                const uint32_t row = (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
+                uint32_t k = bstart[row + (uint32_t)x0];
+                const uint32_t s1 = bstart[row + (uint32_t)x1 + 1];
+                if (k <= (uint32_t)t) k = (uint32_t)t + 1;
+                if (k < s1) examined += s1 - k;
+                for (; k < s1; ++k) {
+                    if (MIXED) {
+                        const float4 o = spos32[k];
+                        const float ex = me32.x - o.x, ey = me32.y - o.y, ez = me32.z - o.z;
+                        const float d2f = ex * ex + ey * ey + ez * ez;
+                        const float thr = (me32.w + o.w) * (1.0f + 64.0f * FLT_EPSILON) + slack;
+                        if (!INIT && d2f > thr * thr) continue;
+                    }
+                    const double4 o = spos[k];
+                    const double ex = me.x - o.x, ey = me.y - o.y, ez = me.z - o.z;
+                    const double d2 = ex * ex + ey * ey + ez * ez;
+                    const double rs = (me.w + o.w) * mult;
+                    if (d2 > rs * rs * (1.0 + 1e-12)) continue;  // conservative: sqrt(d2) >= rs
+                    if (p == 0xffffffffu) p = sidx[t];
+                    const uint32_t q = sidx[k];
+                    const bool plo = p < q;
+                    const uint32_t lo = plo ? p : q, hi = plo ? q : p;
+                    const double4 xl = plo ? me : o, xh = plo ? o : me;
+                    const double rl = INIT ? r0[lo] : xl.w, rh = INIT ? r0[hi] : xh.w;
+                    if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
+                    if (!ref_candidate(P.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z))) continue;
+                    ++found;
+                    row_append(tcnt, touch, P.n, P.K, p, q, overflow);
+                    row_append(tcnt, touch, P.n, P.K, q, p, overflow);
+                }
+            }
+        }
+    }
+    wave_add(&ctr[CT_EXAMINED], examined);
+    wave_add(&ctr[CT_TOUCH], found);
+    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+}
+
 // Candidate-pair count of the reference's cp_list (kernels.jl:24-37): a diagnostic, evaluated on
 // the last step of each dem_step call.
 __global__ void __launch_bounds__(TPB)
@@ -225,14 +304,8 @@ k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __
 // Is hi's bucket one of lo's 8 window buckets (the pair is in the reference's cp_list)?
 __device__ __forceinline__ bool in_window(const Params& P, const double4* __restrict__ pos,
                                           const uint32_t* __restrict__ hid, uint32_t lo, uint32_t hi) {
-    const double4 x = pos[lo];
-    uint32_t w[8];
-    window_of(P.grid, mk3(x.x, x.y, x.z), w);
-    const uint32_t b = hid[hi];
-    bool in = false;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) in |= (w[k] == b);
-    return in;
+    const double4 xl = pos[lo], xh = pos[hi];
+    return ref_candidate(P.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z));
 }
 
 template <bool INIT>
@@ -720,8 +793,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         vel_n[p] = make_double4(vnew.x, vnew.y, vnew.z, Vv.w);
         omg_n[p] = make_double4(wnew.x, wnew.y, wnew.z, W.w);
         quat_n[p] = make_double4(n0 / qn, n1 / qn, n2 / qn, n3 / qn);
-        acc[p] = make_double4(av.x, av.y, av.z, 0.0);
-        dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
+        if (write_forces) {  // a and dw are outputs of update!; the step never reads them back
+            acc[p] = make_double4(av.x, av.y, av.z, 0.0);
+            dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
+        }
     }
     wave_add(&ctr[CT_BROKEN], broken);
     wave_add(&ctr[CT_BROKEN_TOTAL], broken);
@@ -867,15 +942,22 @@ void timing_end(dem_scene* s) {
         if (e_ != hipSuccess) return e_;    \
     } while (0)
 
-static hipError_t sort_particles(dem_scene* s, bool reset_counters) {
+static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
-    LAUNCH(s, K_HASH, hipLaunchKernelGGL(k_hash, grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P,
-                                         d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
-                                         reset_counters ? 1 : 0));
+    if (dense) {
+        LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<true>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
+                                             P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
+                                             reset_counters ? 1 : 0));
+    } else {
+        LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<false>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
+                                             P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
+                                             reset_counters ? 1 : 0));
+    }
+    const int64_t nkeys = dense ? (int64_t)P.dense.cells : (int64_t)P.grid.H;
     timing_begin(s, K_SCAN);
-    hipError_t e = launch_scan(d.scan, d.bcnt, d.bstart, (int64_t)P.grid.H + 1, true, nullptr, s->stream);
+    hipError_t e = launch_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, true, nullptr, s->stream);
     timing_end(s);
     if (e != hipSuccess) return e;
     LAUNCH(s, K_SCATTER, hipLaunchKernelGGL(k_scatter, grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
@@ -888,14 +970,29 @@ hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates)
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
-    hipError_t e = sort_particles(s, true);
+    const bool dense = P.dense.cells > 0;
+    hipError_t e = sort_particles(s, true, dense);
     if (e != hipSuccess) return e;
     const bool mixed = s->precision == DEM_PRECISION_MIXED && !init;
     const dim3 g = grid_for(P.n, TPB);
-    if (count_candidates)
+    if (count_candidates && !dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
                                               d.sidx, d.bstart, d.ctr));
-    if (s->slot_owned) {
+    if (dense) {
+        if (init) {
+            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, false>), g, dim3(TPB), 0, s->stream,
+                                                 P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                                 d.touch, d.ctr));
+        } else if (mixed) {
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, true>), g, dim3(TPB), 0, s->stream,
+                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                                  d.touch, d.ctr));
+        } else {
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, false>), g, dim3(TPB), 0, s->stream,
+                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
+                                                  d.touch, d.ctr));
+        }
+    } else if (s->slot_owned) {
         if (init) {
             LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_slot<true, false>), g, dim3(TPB), 0, s->stream,
                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
@@ -933,6 +1030,12 @@ hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates)
                                               P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
                                               d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
     }
+    if (count_candidates && dense) {
+        // the cp_list length needs the reference's bucket sort: rebuild it after the merge
+        if ((e = sort_particles(s, false, false)) != hipSuccess) return e;
+        LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
+                                              d.sidx, d.bstart, d.ctr));
+    }
     return hipSuccess;
 }
 
@@ -966,12 +1069,12 @@ hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc
     return hipGetLastError();
 }
 
-hipError_t run_debug_sort(dem_scene* s) { return sort_particles(s, false); }
+hipError_t run_debug_sort(dem_scene* s) { return sort_particles(s, false, false); }
 
 hipError_t run_debug_cp_list(dem_scene* s, int32_t** pairs_dev, int64_t* count) {
     DevState& d = s->d;
     const Params& P = s->P;
-    hipError_t e = sort_particles(s, false);
+    hipError_t e = sort_particles(s, false, false);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_cp_list<false>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P,
                        d.pos[s->cur], d.sidx, d.bstart, d.scratch_u32, nullptr, nullptr);

@@ -43,7 +43,8 @@ class Scene:
 
     def __init__(self, cfg: DEMConfig, grains: np.ndarray, walls: Optional[Sequence] = None,
                  precision: str = "fp64", contact_slots: int = 0, device: int = 0,
-                 index_owned_broad_phase: bool = False):
+                 index_owned_broad_phase: bool = False, reference_grid: bool = False,
+                 graph: bool = True):
         self._lib = N.load_library()
         self.cfg = cfg
         grains = np.ascontiguousarray(grains, dtype=GRAIN_IN_DTYPE)
@@ -54,7 +55,9 @@ class Scene:
         self.walls = np.ascontiguousarray(w)
         self.nwall = len(self.walls)
         self._c_cfg = cfg.to_c()
-        flags = N.DEM_FLAG_INDEX_OWNED_BROAD_PHASE if index_owned_broad_phase else 0
+        flags = (N.DEM_FLAG_INDEX_OWNED_BROAD_PHASE if index_owned_broad_phase else 0) | \
+                (N.DEM_FLAG_REFERENCE_GRID if reference_grid else 0) | \
+                (0 if graph else N.DEM_FLAG_NO_GRAPH)
         opt = N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), flags)
         h = ctypes.c_void_p()
         rc = self._lib.dem_create(ctypes.byref(self._c_cfg), _ptr(self.walls, N.DemWall),

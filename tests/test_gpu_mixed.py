@@ -19,15 +19,16 @@ def _scene(*a, **k):
     return Scene(*a, **k)
 
 
+@pytest.mark.parametrize("refgrid", [False, True])
 @pytest.mark.parametrize("name", ["packed_4k", "cfg2_nobond", "cfg2_bonded"])
-def test_mixed_one_step_against_oracle(name):
-    print(f"[case] mixed {name}", flush=True)
+def test_mixed_one_step_against_oracle(name, refgrid):
+    print(f"[case] mixed {name} refgrid={refgrid}", flush=True)
     if name == "packed_4k":
         cfg, g, w = packed_sample(4000)
     else:
         cfg, g, w = reference_case(2)
     o = OracleScene(cfg, g, w)
-    with _scene(cfg, g, w, precision="mixed") as s:
+    with _scene(cfg, g, w, precision="mixed", reference_grid=refgrid) as s:
         if name == "cfg2_bonded":
             o.initialize()
             s.initialize()

@@ -113,15 +113,15 @@ def test_cp_list_multiset(name):
     assert np.array_equal(key(got), key(ref))
 
 
-@pytest.mark.parametrize("owner", ["slot", "index"])
+@pytest.mark.parametrize("variant", ["auto", "refgrid", "index"])
 @pytest.mark.parametrize("name", ["cfg1_carom", "cfg2_cube911", "cfg3_cube18112", "lattice_10x10x9"])
-def test_initialize_bond_set(name, owner):
+def test_initialize_bond_set(name, variant):
     Scene, _ = _engine()
     cfg, g, w = _build(name)
     o = OracleScene(cfg, g, w)
     o.initialize()
     co, bo = o.read_contacts()
-    with Scene(cfg, g, w, index_owned_broad_phase=(owner == "index")) as s:
+    with Scene(cfg, g, w, **VARIANTS[variant]) as s:
         s.initialize()
         c, b = s.read_contacts()
         cnt = s.counters()
@@ -151,14 +151,20 @@ def _compare_state(s, o, rel=FP64_REL):
 ONE_STEP_CASES = CASES + ["cfg2_nobond"]
 
 
-@pytest.mark.parametrize("owner", ["slot", "index"])
+# broad-phase variants: the engine's choice (dense cs/2 grid where the domain is compact), the
+# reference's hashed grid with slot-owned pairs, and index-owned pairs as in kernels.jl:31
+VARIANTS = {"auto": {}, "refgrid": {"reference_grid": True},
+            "index": {"index_owned_broad_phase": True}}
+
+
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
 @pytest.mark.parametrize("name", ONE_STEP_CASES)
-def test_one_step_parity(name, owner):
+def test_one_step_parity(name, variant):
     Scene, _ = _engine()
     cfg, g, w = _build("cfg2_cube911" if name == "cfg2_nobond" else name)
     bonded = name not in UNBONDED
     o = OracleScene(cfg, g, w)
-    with Scene(cfg, g, w, index_owned_broad_phase=(owner == "index")) as s:
+    with Scene(cfg, g, w, **VARIANTS[variant]) as s:
         if bonded:
             o.initialize()
             s.initialize()
@@ -167,13 +173,14 @@ def test_one_step_parity(name, owner):
         _compare_state(s, o)
 
 
+@pytest.mark.parametrize("variant", ["auto", "refgrid"])
 @pytest.mark.parametrize("name", ["cfg2_cube911", "lattice_10x10x9", "packed_4k"])
-def test_stepwise_parity_from_identical_states(name):
+def test_stepwise_parity_from_identical_states(name, variant):
     """Run 60 steps; every 20 steps copy the engine state into the oracle and compare the next step."""
     Scene, _ = _engine()
     cfg, g, w = _build(name)
     bonded = name not in UNBONDED
-    with Scene(cfg, g, w) as s:
+    with Scene(cfg, g, w, **VARIANTS[variant]) as s:
         if bonded:
             s.initialize()
         for _ in range(3):
@@ -188,13 +195,14 @@ def test_stepwise_parity_from_identical_states(name):
             _compare_state(s, o)
 
 
+@pytest.mark.parametrize("variant", ["auto", "refgrid"])
 @pytest.mark.parametrize("name", ["cfg2_cube911", "packed_4k"])
-def test_candidate_pair_counter(name):
+def test_candidate_pair_counter(name, variant):
     """P reported by the engine equals the length of the reference's cp_list (executor.jl:63)."""
     Scene, _ = _engine()
     cfg, g, w = _build(name)
     o = OracleScene(cfg, g, w)
-    with Scene(cfg, g, w) as s:
+    with Scene(cfg, g, w, **VARIANTS[variant]) as s:
         s.step(1)
         o.step(1)
         assert s.counters()["candidate_pairs"] == o.counters()["total_cps"]
