@@ -75,12 +75,18 @@ def build_workload(name: str, n: int):
     if name == "cube900":
         cfg, g, w = scenes.cube_small()
         return cfg, g, w, True, "900-grain bonded cube impact (10x10x9, 10 m/s, dt=1e-6 s)"
+    if name == "batch256":
+        members = scenes.batched_cubes(256, seed=4)
+        return members, None, None, True, (
+            "256 bonded-cube scenes advanced together (grains log-uniform 500-20000, E 1e7-1e9 Pa, "
+            "mu 0.2-0.8, bond strength 1e5-1e7 Pa, impact 2-30 m/s, dt=5e-7 s), one segmented "
+            "launch per stage")
     raise SystemExit(f"unknown workload {name}")
 
 
 def run_engine(cfg, g, w, bonded, precision, steps, warmup):
-    from taichi_dem_julia_port_amd.executor import Scene
-    s = Scene(cfg, g, w, precision=precision)
+    from taichi_dem_julia_port_amd.executor import Batch, Scene
+    s = Batch(cfg, precision=precision) if g is None else Scene(cfg, g, w, precision=precision)
     if bonded:
         s.initialize()
     s.step(warmup)
@@ -102,9 +108,41 @@ def run_engine(cfg, g, w, bonded, precision, steps, warmup):
             "derived": d}
 
 
+def one_at_a_time(members, precision, steps, sample=16, seed=0):
+    """Plain path beside a batch: the same scenes stepped one scene at a time (a seeded sample),
+    scaled to the whole set by particle count."""
+    import numpy as np
+    from taichi_dem_julia_port_amd.executor import Scene
+    rng = np.random.default_rng(seed)
+    idx = rng.choice(len(members), size=min(sample, len(members)), replace=False)
+    tot_n, tot_t = 0, 0.0
+    for k in idx:
+        cfg, g, w = members[k]
+        with Scene(cfg, g, w, precision=precision) as s:
+            s.initialize()
+            s.step(5)
+            t0 = time.perf_counter()
+            s.step(steps)
+            tot_t += time.perf_counter() - t0
+        tot_n += len(g)
+    return tot_n * steps / tot_t, [int(k) for k in idx]
+
+
 def cpu_baseline(cfg, g, w, bonded, steps):
     """The oracle (serial FP64 C restatement of the reference step) on a bounded sample."""
     from oracle.oracle import OracleScene
+    if g is None:  # a batch: time the oracle on its scenes one after another (first 8)
+        tot_n, tot_t = 0, 0.0
+        for (c, gg, ww) in cfg[:8]:
+            o = OracleScene(c, gg, ww)
+            o.initialize()
+            o.step(1)
+            t0 = time.perf_counter()
+            o.step(steps)
+            tot_t += time.perf_counter() - t0
+            tot_n += len(gg)
+            o.close()
+        return tot_n * steps / tot_t, tot_t
     o = OracleScene(cfg, g, w)
     if bonded:
         o.initialize()
@@ -121,7 +159,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--workload", default="pour", choices=["pour", "cube18k", "cube900"])
+    ap.add_argument("--workload", default="pour", choices=["pour", "cube18k", "cube900", "batch256"])
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--compare-precision", action="store_true",
@@ -149,7 +187,9 @@ def main():
     t_build = time.perf_counter()
     cfg, g, w, bonded, desc = build_workload(args.workload, args.n)
     t_build = time.perf_counter() - t_build
-    n = len(g)
+    n = len(g) if g is not None else sum(len(m[1]) for m in cfg)
+    if g is None:
+        w = cfg[0][2]
     r = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup)
     value = n * args.steps / r["elapsed"]
     kt = r["kernels"]
@@ -200,6 +240,12 @@ def main():
         "roofline": roofline,
         "kernel_GB_s": per_kernel,
     }
+    if g is None:
+        out["scenes"] = len(cfg)
+        out["scene_steps_per_s"] = round(len(cfg) * args.steps / r["elapsed"], 1)
+        v1, idx = one_at_a_time(cfg, args.precision, args.steps)
+        out["one_at_a_time"] = {"value": round(v1, 1), "unit": "particle-steps/s",
+                                "sample_scenes": idx, "speedup_batched": round(value / v1, 2)}
     if args.compare_precision:
         other = "mixed" if args.precision == "fp64" else "fp64"
         r2 = run_engine(cfg, g, w, bonded, other, args.steps, args.warmup)
@@ -207,10 +253,12 @@ def main():
         out[f"ms_per_step_{other}"] = round(1e3 * r2["elapsed"] / args.steps, 5)
     if not args.no_cpu_baseline:
         v, dt = cpu_baseline(cfg, g, w, bonded, args.cpu_steps)
+        sample = (f"same workload ({n} grains), {args.cpu_steps} steps after 1 warm-up step, "
+                  f"serial FP64 oracle, {dt:.1f} s") if g is not None else (
+                  f"first 8 scenes of the batch one after another, {args.cpu_steps} steps each "
+                  f"after initialize + 1 warm-up step, serial FP64 oracle, {dt:.1f} s")
         out["cpu_baseline"] = {"value": round(v, 1), "unit": "particle-steps/s", "cores": 1,
-                               "kind": "port",
-                               "sample": f"same workload ({n} grains), {args.cpu_steps} steps after "
-                                         f"1 warm-up step, serial FP64 oracle, {dt:.1f} s"}
+                               "kind": "port", "sample": sample}
     out["setup_s"] = round(t_build, 2)
     if copy_cal:
         out["copy_calibration"] = copy_cal

@@ -221,6 +221,25 @@ const char* dem_kernel_name(int32_t k);
  * times after one warm-up; reports the mean duration.  Used to calibrate HBM byte counters. */
 int32_t dem_debug_copy(int64_t nbytes, int32_t repeats, double* ms_per_copy);
 
+/* ---- segmented batches (no counterpart in the reference) -----------------------------------
+ * nscenes independent scenes share one set of device arrays and are advanced together by one
+ * launch per kernel stage (and one graph replay per two steps).  Scene k has counts[k] grains
+ * (concatenated in grains), config cfgs[k] and nwalls[k] walls (concatenated in walls).  Every
+ * other entry point applies to the whole batch; particle indices are global (scene k occupies
+ * offsets[k] .. offsets[k+1]-1, 0-based).  Each scene's result equals its result alone. */
+int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_t* nwalls,
+                         const dem_wall_t* walls, const int64_t* counts,
+                         const dem_grain_in_t* grains, const dem_options_t* opt, dem_scene_t** out);
+int32_t dem_batch_offsets(const dem_scene_t* s, int64_t* offsets);     /* nscenes + 1 values */
+/* Scenes with mask[k] == 0 get no bonds in dem_initialize (solve's no_bond). */
+int32_t dem_batch_set_bond_init(dem_scene_t* s, const uint8_t* mask);
+int32_t dem_batch_bonds_broken(dem_scene_t* s, int64_t* out);          /* cumulative, per scene */
+/* Move the complete dynamic state of scene src_scene of src (a batch or a single scene, index 0)
+ * into scene dst_scene of dst: grains, contact rows (partners re-indexed), wall history and the
+ * bond-break count.  Both must hold the same particles and walls; used to re-pack live scenes
+ * into a new batch when the serving loop admits or retires scenes. */
+int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, int32_t src_scene);
+
 /* ---- synthetic inputs (host only; no counterpart in the reference) -------------------------
  * Loose polydisperse packing by random sequential addition: log-normal radii (median, sigma)
  * clipped to [rmin, rmax], density rho, zero velocity, placed inside [lo, hi] without overlap;

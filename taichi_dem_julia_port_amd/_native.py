@@ -164,6 +164,14 @@ SIGNATURES = [
     ("dem_get_kernel_times", ctypes.c_int32, [ctypes.c_void_p, P(DemKernelTimes)]),
     ("dem_kernel_name", ctypes.c_char_p, [ctypes.c_int32]),
     ("dem_debug_copy", ctypes.c_int32, [ctypes.c_int64, ctypes.c_int32, P(ctypes.c_double)]),
+    ("dem_create_batch", ctypes.c_int32, [ctypes.c_int32, P(DemConfig), P(ctypes.c_int32), P(DemWall),
+                                          P(ctypes.c_int64), P(DemGrainIn), P(DemOptions),
+                                          P(ctypes.c_void_p)]),
+    ("dem_batch_offsets", ctypes.c_int32, [ctypes.c_void_p, P(ctypes.c_int64)]),
+    ("dem_batch_set_bond_init", ctypes.c_int32, [ctypes.c_void_p, P(ctypes.c_uint8)]),
+    ("dem_batch_bonds_broken", ctypes.c_int32, [ctypes.c_void_p, P(ctypes.c_int64)]),
+    ("dem_batch_adopt", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                         ctypes.c_int32]),
     ("dem_generate_pour", ctypes.c_int32, [ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_uint64, P(DemGrainIn)]),

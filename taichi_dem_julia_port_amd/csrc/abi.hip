@@ -53,7 +53,7 @@ void free_scene(dem_scene* s) {
                     d.acc, d.dw, d.ftot, d.ttot, d.r0, d.wact, d.wshear, d.hid, d.rank, d.bcnt,
                     d.bstart, d.sidx, d.spos, d.spos32, d.tcnt, d.touch, d.sel, d.rcnt[0], d.rcnt[1],
                     d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, d.scan.status, d.scan.counters,
-                    d.scratch_u32, d.scratch2_u32, d.total};
+                    d.scratch_u32, d.scratch2_u32, d.total, d.seg, d.sid, d.seg_broken};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (s->h_ctr) hipHostFree(s->h_ctr);
@@ -134,66 +134,51 @@ const char* dem_kernel_name(int32_t k) {
     return (k >= 0 && k < DEM_NUM_KERNELS) ? names[k] : "";
 }
 
-int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwall,
-                   const dem_grain_in_t* g, int64_t n, const dem_options_t* opt,
-                   dem_scene_t** out) {
-    g_create_err.clear();
-    if (!out) return fail(nullptr, DEM_ERR_INVALID, "out is NULL");
-    *out = nullptr;
+}  // extern "C"
+
+namespace {
+
+int32_t check_scene(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwall,
+                    const dem_grain_in_t* g, int64_t n) {
     if (!cfg || !g) return fail(nullptr, DEM_ERR_INVALID, "cfg and grains must be non-NULL");
     if (n <= 0 || n > (int64_t)IDX_MASK) return fail(nullptr, DEM_ERR_INVALID, "n out of range");
     if (nwall < 0 || nwall > MAX_WALLS || (nwall > 0 && !walls))
         return fail(nullptr, DEM_ERR_INVALID, "nwall must be in [0, 16] with a wall array");
     if (!(cfg->dt > 0.0)) return fail(nullptr, DEM_ERR_INVALID, "dt must be positive");
-    dem_options_t o = {DEM_PRECISION_FP64, 0, 0, 0};
-    if (opt) o = *opt;
-    if (o.precision != DEM_PRECISION_FP64 && o.precision != DEM_PRECISION_MIXED)
-        return fail(nullptr, DEM_ERR_INVALID, "unknown precision");
-    // A row holds a particle's contacts in both roles (i and j), so the default is twice the
-    // reference's per-particle average budget (contacts = n * max_coordinate_number,
-    // executor.jl:277), at least 32.
-    int32_t K = o.contact_slots > 0 ? o.contact_slots
-                                    : std::min(128, std::max(32, 2 * cfg->max_coordinate_number));
-    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
     for (int64_t i = 0; i < n; ++i)
         if (!(g[i].V > 0.0) || !(g[i].m > 0.0))
             return fail(nullptr, DEM_ERR_INVALID, "grain volume and mass must be positive");
     for (int j = 0; j < nwall; ++j)
         if (walls[j].mid < 1 || walls[j].mid > 2)
             return fail(nullptr, DEM_ERR_INVALID, "wall material id must be 1 or 2");
+    return DEM_OK;
+}
 
-    dem_scene* s = new dem_scene();
-    s->n = n;
-    s->nwall = nwall;
-    s->precision = o.precision;
-    s->K = K;
-    s->device = o.device;
-    s->cfg = *cfg;
-    s->use_graph = !(o.flags & DEM_FLAG_NO_GRAPH);
-    if (hipSetDevice(o.device) != hipSuccess) {
-        delete s;
-        return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
-    }
-
-    // ---- derivations (executor.jl:219-267) ------------------------------------------------
-    s->h_id = new int32_t[n]; s->h_gid = new int32_t[n]; s->h_mid = new int32_t[n];
-    s->h_V = new double[n]; s->h_m = new double[n]; s->h_r = new double[n];
-    s->h_r0 = new double[n]; s->h_I = new double[n];
+// The setup half of solve() for one scene (executor.jl:219-315): per-grain derivations into the
+// host arrays at [off, off + n), sizing, and the kernel parameters.  want_dense selects the
+// engine's dense sort grid; force_dense widens its cells instead of falling back to the
+// reference's grid when the domain is large.
+int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const dem_wall_t* walls,
+                     int32_t nwall, const dem_grain_in_t* g, int64_t n, bool want_dense,
+                     bool force_dense, int32_t flags, SceneP& S, dem_derived_t& der,
+                     bool& slot_owned) {
     double rmax_r = 0.0;
     for (int64_t i = 0; i < n; ++i) {
         const double r = grain_radius(g[i].V);
-        s->h_id[i] = g[i].id; s->h_gid[i] = g[i].gid; s->h_mid[i] = 1;
-        s->h_V[i] = g[i].V; s->h_m[i] = g[i].m; s->h_r[i] = r;
-        s->h_r0[i] = r * cfg->particle_contact_radius_multiplier;
-        s->h_I[i] = 2.0 / 5.0 * g[i].m * (r * r);
+        const int64_t k = off + i;
+        s->h_id[k] = g[i].id; s->h_gid[k] = g[i].gid; s->h_mid[k] = 1;
+        s->h_V[k] = g[i].V; s->h_m[k] = g[i].m; s->h_r[k] = r;
+        s->h_r0[k] = r * cfg->particle_contact_radius_multiplier;
+        s->h_I[k] = 2.0 / 5.0 * g[i].m * (r * r);
         if (i == 0 || r > rmax_r) rmax_r = r;
     }
-    dem_derived_t& der = s->der;
     der.r_max = rmax_r * cfg->particle_contact_radius_multiplier *
                 (1.0 + cfg->bond_tensile_strength / cfg->bond_elastic_modulus) *
                 cfg->neighboring_search_safety_factor;
     der.cell_size = 4.0 * der.r_max;
     der.material_density = g[n - 1].m / g[n - 1].V;
+    if (!(der.cell_size > 0.0) || !std::isfinite(der.cell_size))
+        return fail(nullptr, DEM_ERR_INVALID, "derived cell size is not positive and finite");
     uint32_t prod = 1u;
     for (int a = 0; a < 3; ++a)
         prod *= (uint32_t)(int32_t)ceil((cfg->domain_max[a] - cfg->domain_min[a]) / der.cell_size);
@@ -201,34 +186,26 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     int64_t H = der.grid_product_i32;
     H = std::min<int64_t>(std::max<int64_t>(H, 1 << 20), 1 << 22);
     der.hash_table_size = H;
-    der.contact_slots = K;
     der.total_steps = (int64_t)trunc(cfg->target_time / cfg->dt);
     der.save_per_steps = (int64_t)trunc(cfg->saving_interval_time / cfg->dt);
-    {
-        const double reach = 2.0 * rmax_r * std::max(1.0, cfg->particle_contact_radius_multiplier);
-        s->slot_owned = reach * (1.0 + 1e-9) < 0.5 * der.cell_size &&
-                        !(o.flags & DEM_FLAG_INDEX_OWNED_BROAD_PHASE);
-    }
-    if (!(der.cell_size > 0.0) || !std::isfinite(der.cell_size)) {
-        free_scene(s);
-        return fail(nullptr, DEM_ERR_INVALID, "derived cell size is not positive and finite");
-    }
+    // largest distance at which a pair can be in contact or bonded
+    const double reach = 2.0 * rmax_r * std::max(1.0, cfg->particle_contact_radius_multiplier);
+    slot_owned = reach * (1.0 + 1e-9) < 0.5 * der.cell_size &&
+                 !(flags & DEM_FLAG_INDEX_OWNED_BROAD_PHASE);
 
-    // ---- kernel parameters (executor.jl:283-315) --------------------------------------------
-    Params& P = s->P;
-    for (int a = 0; a < 3; ++a) P.grid.dmin[a] = cfg->domain_min[a];
-    P.grid.cs = der.cell_size;
-    P.grid.H = (uint64_t)H;
-    P.grid.Hmask = ((uint64_t)H & ((uint64_t)H - 1)) == 0 ? (uint64_t)H - 1 : 0;
-    P.n = n;
-    P.K = K;
-    P.nwall = nwall;
-    P.dt = cfg->dt;
-    P.tol = cfg->tolerance;
-    P.damping = cfg->global_damping;
-    for (int a = 0; a < 3; ++a) P.g[a] = cfg->gravity[a];
-    P.matE[0] = cfg->particle_elastic_modulus; P.matNu[0] = cfg->particle_poisson_ratio;
-    P.matE[1] = cfg->wall_elastic_modulus;     P.matNu[1] = cfg->wall_poisson_ratio;
+    memset(&S, 0, sizeof(S));
+    for (int a = 0; a < 3; ++a) S.grid.dmin[a] = cfg->domain_min[a];
+    S.grid.cs = der.cell_size;
+    S.grid.H = (uint64_t)H;
+    S.grid.Hmask = ((uint64_t)H & ((uint64_t)H - 1)) == 0 ? (uint64_t)H - 1 : 0;
+    S.nwall = nwall;
+    S.bond_init = 1;
+    S.dt = cfg->dt;
+    S.tol = cfg->tolerance;
+    S.damping = cfg->global_damping;
+    for (int a = 0; a < 3; ++a) S.g[a] = cfg->gravity[a];
+    S.matE[0] = cfg->particle_elastic_modulus; S.matNu[0] = cfg->particle_poisson_ratio;
+    S.matE[1] = cfg->wall_elastic_modulus;     S.matNu[1] = cfg->wall_poisson_ratio;
     auto beta_of = [](double e) { const double le = log(e); return le / sqrt(le * le + M_PI * M_PI); };
     SurfaceP pp{cfg->pp_coefficient_friction, cfg->pp_coefficient_restitution,
                 beta_of(cfg->pp_coefficient_restitution), cfg->bond_radius_ratio,
@@ -236,83 +213,87 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
                 cfg->bond_tensile_strength, cfg->bond_shear_strength};
     SurfaceP pw{cfg->pw_coefficient_friction, cfg->pw_coefficient_restitution,
                 beta_of(cfg->pw_coefficient_restitution), 0, 0, 0, 0, 0, 0};
-    P.surf[0][0] = pp; P.surf[0][1] = pw; P.surf[1][0] = pw; P.surf[1][1] = pw;
+    S.surf[0][0] = pp; S.surf[0][1] = pw; S.surf[1][0] = pw; S.surf[1][1] = pw;
     for (int j = 0; j < nwall; ++j) {
-        for (int a = 0; a < 3; ++a) P.walls[j].n[a] = walls[j].n[a];
-        P.walls[j].d = walls[j].d;
-        P.walls[j].mid = walls[j].mid;
+        for (int a = 0; a < 3; ++a) S.walls[j].n[a] = walls[j].n[a];
+        S.walls[j].d = walls[j].d;
+        S.walls[j].mid = walls[j].mid;
     }
-    P.margin_rel = 0.f;
-    P.bond_mult = cfg->particle_contact_radius_multiplier;
-    // Dense sort/broad-phase grid with cells of cs/2 = 2 r_max when the slot-owned broad phase is
-    // valid and the domain holds at most max(4 n, 2^20) such cells; otherwise the reference's grid.
-    P.dense.cells = 0;
-    if (s->slot_owned && !(o.flags & DEM_FLAG_REFERENCE_GRID)) {
-        const double cell = 0.5 * der.cell_size;
-        double cells = 1.0;
-        int32_t dims[3];
-        for (int a = 0; a < 3; ++a) {
-            const double ext = cfg->domain_max[a] - cfg->domain_min[a];
-            const double dd = std::max(1.0, ceil(ext / cell));
-            dims[a] = (int32_t)std::min(dd, 2.0e9);
-            cells *= dd;
-        }
-        if (cells <= std::max(4.0 * (double)n, (double)(1 << 20)) && cells < 2.0e9) {
-            for (int a = 0; a < 3; ++a) { P.dense.origin[a] = cfg->domain_min[a]; P.dense.dims[a] = dims[a]; }
-            P.dense.inv_cell = 1.0 / cell;
-            P.dense.cells = (uint32_t)cells;
-        }
-    }
+    S.bond_mult = cfg->particle_contact_radius_multiplier;
     {
         const double E = cfg->particle_elastic_modulus, nu = cfg->particle_poisson_ratio;
-        P.ys_pp = (float)(1.0 / ((1.0 - nu * nu) / E + (1.0 - nu * nu) / E));
-        P.gs_pp = (float)(1.0 / (2.0 * (2.0 - nu) * (1.0 + nu) / E + 2.0 * (2.0 - nu) * (1.0 + nu) / E));
-        P.beta_pp = (float)pp.beta;
-        P.mu_pp = (float)pp.mu;
+        S.ys_pp = (float)(1.0 / ((1.0 - nu * nu) / E + (1.0 - nu * nu) / E));
+        S.gs_pp = (float)(1.0 / (2.0 * (2.0 - nu) * (1.0 + nu) / E + 2.0 * (2.0 - nu) * (1.0 + nu) / E));
+        S.beta_pp = (float)pp.beta;
+        S.mu_pp = (float)pp.mu;
     }
+    // Dense sort/broad-phase grid: cells of at least cs/2 and of the contact reach, covering the
+    // domain with at most max(4 n, 2^20) cells (batches widen the cells to at most 8 n + 4096).
+    S.dense.cells = 0;
+    if (want_dense) {
+        double cell = std::max(0.5 * der.cell_size, reach * (1.0 + 1e-6));
+        const double budget = force_dense ? 8.0 * (double)n + 4096.0
+                                          : std::max(4.0 * (double)n, (double)(1 << 20));
+        for (int it = 0; it < 64; ++it) {
+            double cells = 1.0;
+            int32_t dims[3];
+            for (int a = 0; a < 3; ++a) {
+                const double ext = cfg->domain_max[a] - cfg->domain_min[a];
+                const double dd = std::max(1.0, ceil(ext / cell));
+                dims[a] = (int32_t)std::min(dd, 2.0e9);
+                cells *= dd;
+            }
+            if (cells <= budget && cells < 2.0e9) {
+                for (int a = 0; a < 3; ++a) { S.dense.origin[a] = cfg->domain_min[a]; S.dense.dims[a] = dims[a]; }
+                S.dense.inv_cell = 1.0 / cell;
+                S.dense.cells = (uint32_t)cells;
+                break;
+            }
+            if (!force_dense) break;
+            cell *= std::max(1.05, cbrt(cells / budget));
+        }
+        if (force_dense && S.dense.cells == 0)
+            return fail(nullptr, DEM_ERR_INVALID, "scene domain cannot be gridded");
+    }
+    return DEM_OK;
+}
 
-    // ---- device allocation -------------------------------------------------------------------
+// Device allocation and initial upload (executor.jl:229-242: q = 1, w = a = dw = 0) for the n
+// grains already derived into the host arrays.
+int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max, int64_t nkeys) {
     DevState& d = s->d;
-    const int64_t KN = (int64_t)K * n;
+    const int64_t n = s->n;
+    const int64_t KN = (int64_t)s->K * n;
     hipError_t e = hipSuccess;
 #define ALLOC(ptr, cnt)                                                                   \
     if (e == hipSuccess) {                                                                \
         e = dalloc(&(ptr), (size_t)(cnt));                                                \
-        if (e != hipSuccess) {                                                            \
-            std::string m_ = std::string("device allocation of " #ptr " failed: ") +     \
-                             hipGetErrorString(e);                                        \
-            free_scene(s);                                                                \
-            return fail(nullptr, DEM_ERR_HIP, m_);                                        \
-        }                                                                                 \
+        if (e != hipSuccess)                                                              \
+            return fail(nullptr, DEM_ERR_HIP, std::string("device allocation of " #ptr    \
+                                                          " failed: ") + hipGetErrorString(e)); \
     }
     for (int b = 0; b < 2; ++b) {
         ALLOC(d.pos[b], n); ALLOC(d.vel[b], n); ALLOC(d.omg[b], n); ALLOC(d.quat[b], n);
         ALLOC(d.rcnt[b], n); ALLOC(d.pq[b], KN); ALLOC(d.st[b], (int64_t)S_NCOMP * KN);
     }
     ALLOC(d.acc, n); ALLOC(d.dw, n); ALLOC(d.ftot, n); ALLOC(d.ttot, n); ALLOC(d.r0, n);
-    ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall > 0 ? nwall : 1) * n);
-    const int64_t nkeys = std::max<int64_t>(H, (int64_t)P.dense.cells);
+    ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall_max > 0 ? nwall_max : 1) * n);
     ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.bcnt, nkeys + 1); ALLOC(d.bstart, nkeys + 1);
     ALLOC(d.sidx, n); ALLOC(d.spos, n);
-    if (o.precision == DEM_PRECISION_MIXED) { ALLOC(d.spos32, n); }
+    if (s->precision == DEM_PRECISION_MIXED) { ALLOC(d.spos32, n); }
     ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
     ALLOC(d.ctr, CT_NCOUNTERS);
     const int64_t max_tiles = scan_tiles(std::max<int64_t>(nkeys + 1, n + 1));
     ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2);
     d.scan.max_tiles = max_tiles;
     ALLOC(d.scratch_u32, n + 1); ALLOC(d.scratch2_u32, n + 1); ALLOC(d.total, 1);
+    if (s->batched) { ALLOC(d.seg, s->nseg); ALLOC(d.sid, n); ALLOC(d.seg_broken, s->nseg); }
 #undef ALLOC
-    if (hipHostMalloc((void**)&s->h_ctr, sizeof(unsigned long long) * CT_NCOUNTERS) != hipSuccess) {
-        free_scene(s);
+    if (hipHostMalloc((void**)&s->h_ctr, sizeof(unsigned long long) * CT_NCOUNTERS) != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, "hipHostMalloc failed");
-    }
     memset(s->h_ctr, 0, sizeof(unsigned long long) * CT_NCOUNTERS);
-    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
-        free_scene(s);
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, "hipStreamCreate failed");
-    }
-
-    // ---- initial state (executor.jl:229-242): q = 1, w = a = dw = 0 ---------------------------
     std::vector<double4> hp(n), hv(n), hw(n), hq(n);
     for (int64_t i = 0; i < n; ++i) {
         hp[i] = make_double4(g[i].k[0], g[i].k[1], g[i].k[2], s->h_r[i]);
@@ -325,13 +306,249 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     if (e == hipSuccess) e = hipMemcpy(d.omg[0], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.quat[0], hq.data(), n * sizeof(double4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.r0, s->h_r0, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess && s->batched) {
+        std::vector<uint16_t> sid(n);
+        for (int k = 0; k < s->nseg; ++k)
+            for (int64_t i = s->seg_off[k]; i < s->seg_off[k + 1]; ++i) sid[i] = (uint16_t)k;
+        e = hipMemcpy(d.sid, sid.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(d.seg, s->seg_host.data(), s->nseg * sizeof(SceneP), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) {
-        std::string m = std::string("initial upload failed: ") + hipGetErrorString(e);
+    if (e != hipSuccess) return fail(nullptr, DEM_ERR_HIP, std::string("initial upload failed: ") + hipGetErrorString(e));
+    s->P.seg = d.seg;
+    s->P.sid = d.sid;
+    s->P.seg_broken = d.seg_broken;
+    return DEM_OK;
+}
+
+void alloc_host_fields(dem_scene* s, int64_t n) {
+    s->h_id = new int32_t[n]; s->h_gid = new int32_t[n]; s->h_mid = new int32_t[n];
+    s->h_V = new double[n]; s->h_m = new double[n]; s->h_r = new double[n];
+    s->h_r0 = new double[n]; s->h_I = new double[n];
+}
+
+int32_t options_of(const dem_options_t* opt, dem_options_t& o) {
+    o = dem_options_t{DEM_PRECISION_FP64, 0, 0, 0};
+    if (opt) o = *opt;
+    if (o.precision != DEM_PRECISION_FP64 && o.precision != DEM_PRECISION_MIXED)
+        return fail(nullptr, DEM_ERR_INVALID, "unknown precision");
+    return DEM_OK;
+}
+
+// A row holds a particle's contacts in both roles (i and j), so the default is twice the
+// reference's per-particle average budget (contacts = n * max_coordinate_number,
+// executor.jl:277), at least 32.
+int32_t slots_of(const dem_options_t& o, int32_t max_coord) {
+    return o.contact_slots > 0 ? o.contact_slots : std::min(128, std::max(32, 2 * max_coord));
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwall,
+                   const dem_grain_in_t* g, int64_t n, const dem_options_t* opt,
+                   dem_scene_t** out) {
+    g_create_err.clear();
+    if (!out) return fail(nullptr, DEM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int32_t rc = check_scene(cfg, walls, nwall, g, n);
+    if (rc) return rc;
+    dem_options_t o;
+    if ((rc = options_of(opt, o))) return rc;
+    const int32_t K = slots_of(o, cfg->max_coordinate_number);
+    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
+    if (hipSetDevice(o.device) != hipSuccess) return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
+
+    dem_scene* s = new dem_scene();
+    s->n = n;
+    s->nwall = nwall;
+    s->precision = o.precision;
+    s->K = K;
+    s->device = o.device;
+    s->cfg = *cfg;
+    s->use_graph = !(o.flags & DEM_FLAG_NO_GRAPH);
+    s->seg_off = {0, n};
+    alloc_host_fields(s, n);
+    const bool want_dense = !(o.flags & (DEM_FLAG_REFERENCE_GRID | DEM_FLAG_INDEX_OWNED_BROAD_PHASE));
+    if ((rc = derive_scene(s, 0, cfg, walls, nwall, g, n, want_dense, false, o.flags, s->P.sc,
+                           s->der, s->slot_owned))) {
         free_scene(s);
-        return fail(nullptr, DEM_ERR_HIP, m);
+        return rc;
+    }
+    s->der.contact_slots = K;
+    Params& P = s->P;
+    P.n = n;
+    P.K = K;
+    P.nseg = 1;
+    s->dense = P.sc.dense.cells > 0;
+    P.sort_keys = s->dense ? P.sc.dense.cells : (uint32_t)P.sc.grid.H;
+    const int64_t nkeys = std::max<int64_t>((int64_t)P.sc.grid.H, (int64_t)P.sc.dense.cells);
+    if ((rc = allocate_scene(s, g, nwall, nkeys))) {
+        free_scene(s);
+        return rc;
     }
     *out = s;
+    return DEM_OK;
+}
+
+int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_t* nwalls,
+                         const dem_wall_t* walls, const int64_t* counts,
+                         const dem_grain_in_t* grains, const dem_options_t* opt,
+                         dem_scene_t** out) {
+    g_create_err.clear();
+    if (!out) return fail(nullptr, DEM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (nscenes < 1 || nscenes > 65535 || !cfgs || !nwalls || !counts || !grains)
+        return fail(nullptr, DEM_ERR_INVALID, "need 1..65535 scenes with configs, wall counts and grains");
+    dem_options_t o;
+    int32_t rc = options_of(opt, o);
+    if (rc) return rc;
+    if (o.flags & (DEM_FLAG_REFERENCE_GRID | DEM_FLAG_INDEX_OWNED_BROAD_PHASE))
+        return fail(nullptr, DEM_ERR_INVALID, "batches always use the dense sort grid");
+    int64_t total = 0, woff = 0;
+    int32_t nwall_max = 0, maxcoord = 0;
+    for (int k = 0; k < nscenes; ++k) {
+        const int64_t nk = counts[k];
+        if ((rc = check_scene(&cfgs[k], walls ? walls + woff : nullptr, nwalls[k], grains + total, nk)))
+            return rc;
+        total += nk;
+        woff += nwalls[k];
+        nwall_max = std::max(nwall_max, nwalls[k]);
+        maxcoord = std::max(maxcoord, cfgs[k].max_coordinate_number);
+    }
+    if (total > (int64_t)IDX_MASK) return fail(nullptr, DEM_ERR_INVALID, "too many particles");
+    const int32_t K = slots_of(o, maxcoord);
+    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
+    if (hipSetDevice(o.device) != hipSuccess) return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
+
+    dem_scene* s = new dem_scene();
+    s->n = total;
+    s->nwall = nwall_max;
+    s->precision = o.precision;
+    s->K = K;
+    s->device = o.device;
+    s->cfg = cfgs[0];
+    s->use_graph = !(o.flags & DEM_FLAG_NO_GRAPH);
+    s->batched = true;
+    s->nseg = nscenes;
+    s->seg_off.assign(nscenes + 1, 0);
+    s->seg_host.resize(nscenes);
+    alloc_host_fields(s, total);
+    int64_t off = 0;
+    uint64_t keys = 0;
+    woff = 0;
+    for (int k = 0; k < nscenes; ++k) {
+        dem_derived_t der{};
+        bool slot_owned = false;
+        SceneP& S = s->seg_host[k];
+        if ((rc = derive_scene(s, off, &cfgs[k], walls ? walls + woff : nullptr, nwalls[k], grains + off,
+                               counts[k], true, true, o.flags, S, der, slot_owned))) {
+            free_scene(s);
+            return rc;
+        }
+        S.cell_base = (uint32_t)keys;
+        keys += S.dense.cells;
+        if (k == 0) s->der = der;
+        s->seg_off[k] = off;
+        off += counts[k];
+        woff += nwalls[k];
+    }
+    s->seg_off[nscenes] = off;
+    if (keys >= (1ull << 31)) {
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_INVALID, "batch sort grid too large");
+    }
+    s->der.contact_slots = K;
+    Params& P = s->P;
+    P.n = total;
+    P.K = K;
+    P.nseg = nscenes;
+    P.sc = s->seg_host[0];
+    s->dense = true;
+    P.sort_keys = (uint32_t)keys;
+    if ((rc = allocate_scene(s, grains, nwall_max, (int64_t)keys))) {
+        free_scene(s);
+        return rc;
+    }
+    *out = s;
+    return DEM_OK;
+}
+
+int32_t dem_batch_offsets(const dem_scene_t* s, int64_t* offsets) {
+    if (!s || !offsets) return DEM_ERR_INVALID;
+    for (int k = 0; k <= s->nseg; ++k) offsets[k] = s->seg_off[k];
+    return DEM_OK;
+}
+
+int32_t dem_batch_set_bond_init(dem_scene_t* s, const uint8_t* mask) {
+    if (!s || !mask) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    if (!s->batched) {
+        s->P.sc.bond_init = mask[0] ? 1 : 0;
+        return DEM_OK;
+    }
+    for (int k = 0; k < s->nseg; ++k) s->seg_host[k].bond_init = mask[k] ? 1 : 0;
+    s->P.sc = s->seg_host[0];
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(s->d.seg, s->seg_host.data(), s->nseg * sizeof(SceneP), hipMemcpyHostToDevice));
+    return DEM_OK;
+}
+
+int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, int32_t src_scene) {
+    if (!dst || !src || dst == src) return DEM_ERR_INVALID;
+    if (dst_scene < 0 || dst_scene >= dst->nseg || src_scene < 0 || src_scene >= src->nseg)
+        return fail(dst, DEM_ERR_INVALID, "scene index out of range");
+    if (dst->device != src->device) return fail(dst, DEM_ERR_INVALID, "scenes on different devices");
+    const int64_t n = src->seg_off[src_scene + 1] - src->seg_off[src_scene];
+    if (dst->seg_off[dst_scene + 1] - dst->seg_off[dst_scene] != n)
+        return fail(dst, DEM_ERR_INVALID, "scenes differ in particle count");
+    const SceneP& Ss = src->batched ? src->seg_host[src_scene] : src->P.sc;
+    const SceneP& Sd = dst->batched ? dst->seg_host[dst_scene] : dst->P.sc;
+    if (Ss.nwall != Sd.nwall) return fail(dst, DEM_ERR_INVALID, "scenes differ in wall count");
+    HIPCHK(dst, hipSetDevice(dst->device));
+    HIPCHK(dst, hipStreamSynchronize(src->stream));
+    HIPCHK(dst, hipStreamSynchronize(dst->stream));
+    uint32_t* ovf = nullptr;
+    HIPCHK(dst, hipMalloc(&ovf, sizeof(uint32_t)));
+    hipError_t e = hipMemset(ovf, 0, sizeof(uint32_t));
+    if (e == hipSuccess)
+        e = run_adopt(dst, dst->seg_off[dst_scene], src, src->seg_off[src_scene], n, Ss.nwall, ovf);
+    uint32_t o = 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(dst->stream);
+    if (e == hipSuccess) e = hipMemcpy(&o, ovf, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    hipFree(ovf);
+    if (e != hipSuccess) return hip_fail(dst, "dem_batch_adopt", e);
+    if (o) return fail(dst, DEM_ERR_CAPACITY, "destination has fewer contact slots than needed");
+    // cumulative bond breaks of the scene travel with it
+    unsigned long long nb = 0;
+    if (src->batched) {
+        HIPCHK(dst, hipMemcpy(&nb, src->d.seg_broken + src_scene, sizeof(nb), hipMemcpyDeviceToHost));
+    } else {
+        HIPCHK(dst, hipMemcpy(&nb, src->d.ctr + CT_BROKEN_TOTAL, sizeof(nb), hipMemcpyDeviceToHost));
+    }
+    if (dst->batched) {
+        HIPCHK(dst, hipMemcpy(dst->d.seg_broken + dst_scene, &nb, sizeof(nb), hipMemcpyHostToDevice));
+    } else {
+        HIPCHK(dst, hipMemcpy(dst->d.ctr + CT_BROKEN_TOTAL, &nb, sizeof(nb), hipMemcpyHostToDevice));
+    }
+    return DEM_OK;
+}
+
+int32_t dem_batch_bonds_broken(dem_scene_t* s, int64_t* out) {
+    if (!s || !out) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    if (!s->batched) {
+        int32_t rc = sync_counters(s);
+        if (rc) return rc;
+        out[0] = (int64_t)s->h_ctr[CT_BROKEN_TOTAL];
+        return DEM_OK;
+    }
+    std::vector<unsigned long long> v(s->nseg);
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(v.data(), s->d.seg_broken, s->nseg * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    for (int k = 0; k < s->nseg; ++k) out[k] = (int64_t)v[k];
     return DEM_OK;
 }
 
@@ -445,7 +662,7 @@ int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
     out->fatal_gap = (int64_t)c[CT_FATAL];
     out->overflow = (int64_t)c[CT_OVERFLOW];
     out->examined = (int64_t)c[CT_EXAMINED];
-    out->grid_cells = (int64_t)s->P.dense.cells;
+    out->grid_cells = s->dense ? (int64_t)s->P.sort_keys : 0;
     return DEM_OK;
 }
 
@@ -639,6 +856,7 @@ int32_t dem_read_forces(dem_scene_t* s, double* F, double* T, int64_t n) {
 int32_t dem_debug_cells(dem_scene_t* s, int32_t* kid, uint32_t* hid, double* kcenter,
                         uint32_t* nbr) {
     if (!s) return DEM_ERR_INVALID;
+    if (s->batched) return fail(s, DEM_ERR_STATE, "not available for batches");
     HIPCHK(s, hipSetDevice(s->device));
     const int64_t n = s->n;
     int32_t* dk = nullptr; uint32_t* dh = nullptr; double* dc = nullptr; uint32_t* dn = nullptr;
@@ -659,6 +877,7 @@ int32_t dem_debug_cells(dem_scene_t* s, int32_t* kid, uint32_t* hid, double* kce
 
 int32_t dem_debug_hash_table(dem_scene_t* s, uint32_t* counts, uint32_t* start, uint32_t* pid) {
     if (!s) return DEM_ERR_INVALID;
+    if (s->batched) return fail(s, DEM_ERR_STATE, "not available for batches");
     HIPCHK(s, hipSetDevice(s->device));
     HIPCHK(s, run_debug_sort(s));
     HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -677,6 +896,7 @@ int32_t dem_debug_hash_table(dem_scene_t* s, uint32_t* counts, uint32_t* start, 
 
 int32_t dem_debug_cp_list(dem_scene_t* s, int32_t* pairs, int64_t cap, int64_t* count) {
     if (!s || !count || cap < 0 || (cap > 0 && !pairs)) return DEM_ERR_INVALID;
+    if (s->batched) return fail(s, DEM_ERR_STATE, "not available for batches");
     HIPCHK(s, hipSetDevice(s->device));
     int32_t* dp = nullptr;
     int64_t total = 0;

@@ -46,23 +46,43 @@ struct SurfaceP {
 
 struct WallP { double n[3]; double d; int32_t mid; int32_t pad; };
 
-// Everything a kernel needs by value.
-struct Params {
+// Everything that may differ between the scenes of a batch.
+struct SceneP {
     HashGrid grid;      // the reference's hashed cell grid (utils.jl:1-16)
     DenseGrid dense;    // fine dense grid for sorting/broad phase (dense.cells == 0: unused)
-    double bond_mult;   // particle_contact_radius_multiplier (r0 = r * bond_mult)
-    int64_t n;
-    int32_t K;       // contact slots per particle
+    uint32_t cell_base; // first global sort key of this scene's dense cells
     int32_t nwall;
+    int32_t bond_init;  // scene takes part in dem_initialize
+    int32_t pad;
+    double bond_mult;   // particle_contact_radius_multiplier (r0 = r * bond_mult)
     double dt, tol, damping;
     double g[3];
     double matE[2], matNu[2];
     SurfaceP surf[2][2];
     WallP walls[MAX_WALLS];
-    float margin_rel;  // FP32 candidate filter slack (mixed precision)
     // particle-particle Hertz-Mindlin constants in FP32 (mixed precision)
     float ys_pp, gs_pp, beta_pp, mu_pp;
 };
+
+// Everything a kernel needs by value.  Single-scene kernels read scene 0 from the kernel
+// arguments (sc); batched kernels look a particle's scene up in the device table seg[sid[p]].
+struct Params {
+    int64_t n;
+    int32_t K;            // contact slots per particle
+    int32_t nseg;         // scenes (1 for a single scene)
+    uint32_t sort_keys;   // keys of the per-step sort (dense cells of all scenes, or H)
+    uint32_t pad;
+    SceneP sc;
+    const SceneP* seg;
+    const uint16_t* sid;
+    unsigned long long* seg_broken;  // cumulative bond breaks per scene
+};
+
+template <bool SEG>
+__device__ __forceinline__ const SceneP& scene_of(const Params& P, uint32_t p) {
+    if constexpr (SEG) return P.seg[P.sid[p]];
+    else return P.sc;
+}
 
 struct ScanWorkspace {
     uint64_t* status = nullptr;
@@ -111,6 +131,10 @@ struct DevState {
     uint32_t* scratch_u32 = nullptr;  // [n + 1]
     uint32_t* scratch2_u32 = nullptr; // [n + 1]
     uint64_t* total = nullptr;
+    // batched scenes
+    SceneP* seg = nullptr;
+    uint16_t* sid = nullptr;
+    unsigned long long* seg_broken = nullptr;
 };
 
 }  // namespace dem
@@ -125,6 +149,11 @@ struct dem_scene {
     int cur = 0;
     bool initialized = false;
     bool slot_owned = true;  // broad phase owned by sorted slot (needs touching => mutual windows)
+    bool dense = false;      // per-step sort on the dense cs/2 grid (else the reference's grid)
+    bool batched = false;    // several scenes in one set of arrays (per-particle scene ids)
+    int32_t nseg = 1;
+    std::vector<int64_t> seg_off;   // [nseg + 1] first particle of each scene
+    std::vector<dem::SceneP> seg_host;
     int64_t steps = 0;
     uint64_t broken_total_host = 0;
     dem_config_t cfg{};
@@ -172,5 +201,7 @@ hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint3
 void timing_begin(dem_scene* s, int kernel);
 void timing_end(dem_scene* s);
 int32_t timing_flush(dem_scene* s);
-hipError_t run_copy(const void* src, void* dst, int64_t nbytes, hipStream_t stream);  // abi.hip: collect elapsed times of recorded pairs
+hipError_t run_copy(const void* src, void* dst, int64_t nbytes, hipStream_t stream);
+hipError_t run_adopt(dem_scene* dst, int64_t dst_off, const dem_scene* src, int64_t src_off,
+                     int64_t n, int32_t nwall, uint32_t* overflow_dev);  // abi.hip: collect elapsed times of recorded pairs
 }  // namespace dem

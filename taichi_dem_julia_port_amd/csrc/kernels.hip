@@ -33,7 +33,7 @@ __device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 
 // ------------------------------------------------------------------------------------------
 // hashing and counting sort
 // ------------------------------------------------------------------------------------------
-template <bool DENSE>
+template <bool DENSE, bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_hash(const Params P, const double4* __restrict__ pos, uint32_t* __restrict__ hid,
        uint32_t* __restrict__ rank, uint32_t* __restrict__ bcnt, unsigned long long* ctr,
@@ -42,9 +42,10 @@ k_hash(const Params P, const double4* __restrict__ pos, uint32_t* __restrict__ h
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= P.n) return;
     const double4 x = pos[p];
+    const SceneP& S = scene_of<SEG>(P, (uint32_t)p);
     int32_t c[3];
-    const uint32_t b = DENSE ? dense_cell(P.dense, mk3(x.x, x.y, x.z), c)
-                             : ref_bucket(P.grid, mk3(x.x, x.y, x.z));
+    const uint32_t b = DENSE ? S.cell_base + dense_cell(S.dense, mk3(x.x, x.y, x.z), c)
+                             : ref_bucket(S.grid, mk3(x.x, x.y, x.z));
     hid[p] = b;
     rank[p] = atomicAdd(&bcnt[b], 1u);
 }
@@ -94,7 +95,7 @@ k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __rest
             slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
         }
         uint32_t w[8];
-        window_of(P.grid, xp, w);
+        window_of(P.sc.grid, xp, w);
 #pragma unroll 1
         for (int wi = 0; wi < 8; ++wi) {
             const uint32_t b = w[wi];
@@ -156,7 +157,7 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
             slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
         }
         uint32_t w[8];
-        window_of(P.grid, xp, w);
+        window_of(P.sc.grid, xp, w);
         const uint32_t own = w[0];
 #pragma unroll 1
         for (int wi = 0; wi < 8; ++wi) {
@@ -186,7 +187,7 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
                 if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
                 if (p > q) {  // owner is hi: is hi's bucket in lo's window?
                     uint32_t wl[8];
-                    window_of(P.grid, mk3(o.x, o.y, o.z), wl);
+                    window_of(P.sc.grid, mk3(o.x, o.y, o.z), wl);
                     bool in = false;
 #pragma unroll
                     for (int m = 0; m < 8; ++m) in |= (wl[m] == own);
@@ -210,7 +211,7 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
 // (norm(k_i - k_j) < r_i + r_j, kernels.jl:103; r0 at t = 0, kernels.jl:72) and the reference's
 // candidate rule (hi's bucket in lo's window, kernels.jl:24-58), so the contact set is the
 // reference's.
-template <bool INIT, bool MIXED>
+template <bool INIT, bool MIXED, bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
               const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
@@ -219,19 +220,21 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     uint64_t found = 0, examined = 0;
     uint32_t overflow = 0;
-    if (t < P.n) {
-        const DenseGrid& G = P.dense;
+    uint32_t p = 0xffffffffu;
+    if (SEG && t < P.n) p = sidx[t];
+    const SceneP& S = scene_of<SEG>(P, SEG && t < P.n ? p : 0u);
+    if (t < P.n && (!INIT || S.bond_init)) {
+        const DenseGrid& G = S.dense;
         const double4 me = spos[t];
         int32_t c[3];
         dense_cell(G, mk3(me.x, me.y, me.z), c);
-        uint32_t p = 0xffffffffu;
         float4 me32 = make_float4(0.f, 0.f, 0.f, 0.f);
         float slack = 0.f;
         if (MIXED) {
             me32 = spos32[t];
             slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
         }
-        const double mult = INIT ? P.bond_mult * (1.0 + 1e-9) : 1.0;
+        const double mult = INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0;
         const int32_t x0 = c[0] > 0 ? c[0] - 1 : 0;
         const int32_t x1 = c[0] + 1 < G.dims[0] ? c[0] + 1 : G.dims[0] - 1;
 #pragma unroll 1
@@ -242,7 +245,8 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
             for (int dy = -1; dy <= 1; ++dy) {
                 const int32_t y = c[1] + dy;
                 if (y < 0 || y >= G.dims[1]) continue;
-                const uint32_t row = (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
+                const uint32_t row = S.cell_base +
+                                     (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
                 uint32_t k = bstart[row + (uint32_t)x0];
                 const uint32_t s1 = bstart[row + (uint32_t)x1 + 1];
                 if (k <= (uint32_t)t) k = (uint32_t)t + 1;
@@ -267,7 +271,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
                     const double4 xl = plo ? me : o, xh = plo ? o : me;
                     const double rl = INIT ? r0[lo] : xl.w, rh = INIT ? r0[hi] : xh.w;
                     if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
-                    if (!ref_candidate(P.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z))) continue;
+                    if (!ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z))) continue;
                     ++found;
                     row_append(tcnt, touch, P.n, P.K, p, q, overflow);
                     row_append(tcnt, touch, P.n, P.K, q, p, overflow);
@@ -290,7 +294,7 @@ k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __
     if (i < P.n) {
         const double4 x = pos[i];
         uint32_t w[8];
-        window_of(P.grid, mk3(x.x, x.y, x.z), w);
+        window_of(P.sc.grid, mk3(x.x, x.y, x.z), w);
         for (int wi = 0; wi < 8; ++wi)
             for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += (uint32_t)i < sidx[k];
     }
@@ -302,13 +306,13 @@ k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __
 // ------------------------------------------------------------------------------------------
 
 // Is hi's bucket one of lo's 8 window buckets (the pair is in the reference's cp_list)?
-__device__ __forceinline__ bool in_window(const Params& P, const double4* __restrict__ pos,
-                                          const uint32_t* __restrict__ hid, uint32_t lo, uint32_t hi) {
+__device__ __forceinline__ bool in_window(const SceneP& S, const double4* __restrict__ pos,
+                                          uint32_t lo, uint32_t hi) {
     const double4 xl = pos[lo], xh = pos[hi];
-    return ref_candidate(P.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z));
+    return ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z));
 }
 
-template <bool INIT>
+template <bool INIT, bool SEG>
 __global__ void __launch_bounds__(MERGE_TPB)
 k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ hid,
         uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ touch, uint8_t* __restrict__ sel,
@@ -324,6 +328,7 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
 #define LT(m) lds[(m) * MERGE_TPB + threadIdx.x]
     if (p64 < n) {
         const uint32_t p = (uint32_t)p64;
+        const SceneP& S = scene_of<SEG>(P, p);
         uint32_t nt = tcnt[p];
         if (nt) tcnt[p] = 0u;
         if ((int)nt > K) { overflow = 1; nt = K; }
@@ -356,7 +361,7 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
                 if (eo & BROKEN) { changed = true; continue; }
                 if ((eo & BONDED) || INIT) { ++nout; continue; }
                 const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
-                if (in_window(P, pos, hid, lo, hi)) {  // in cp_list and not touching: deactivate
+                if (in_window(S, pos, lo, hi)) {  // in cp_list and not touching: deactivate
                     changed = true;
                     if (p < qo) ++deact;
                 } else {
@@ -396,7 +401,7 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
                     if (eo & BROKEN) continue;
                     if (!(eo & BONDED) && !INIT) {
                         const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
-                        if (in_window(P, pos, hid, lo, hi)) continue;
+                        if (in_window(S, pos, lo, hi)) continue;
                     }
                     pqn[dst] = eo;
 #pragma unroll
@@ -546,7 +551,7 @@ struct CSum3 {
     }
 };
 
-template <bool MIXED>
+template <bool MIXED, bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_force(const Params P, const double4* __restrict__ pos, const double4* __restrict__ vel,
         const double4* __restrict__ omg, const double4* __restrict__ quat,
@@ -562,7 +567,8 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
     uint64_t broken = 0, nwallc = 0, fatal = 0, active = 0, active_b = 0;
     if (p64 < n) {
         const uint32_t p = (uint32_t)p64;
-        const double dt = P.dt;
+        const SceneP& S = scene_of<SEG>(P, p);
+        const double dt = S.dt;
         const double4 X = pos[p], Vv = vel[p], W = omg[p], Q = quat[p];
         const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
         d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
@@ -590,11 +596,11 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             const double Li = norm(kj - ki);
             if (word & BONDED) {
                 // EBPM bond (kernels.jl:170-239)
-                const m3 R = contact_frame(normalize(kj - ki), P.tol);
+                const m3 R = contact_frame(normalize(kj - ki), S.tol);
                 const d3 kc = (ki + kj) * 0.5;
                 const d3 di = mul(R, vi * dt), dj = mul(R, vj * dt);
                 const d3 thi = mul(R, wi * dt), thj = mul(R, wj * dt);
-                const SurfaceP& sf = P.surf[0][0];
+                const SurfaceP& sf = S.surf[0][0];
                 const double rmin = ri < rj ? ri : rj;
                 const double rb = sf.rho * rmin;
                 const double L = Li;
@@ -626,7 +632,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const double ss = fabs(Ti.x) * rb / 2.0 / Ib + 4.0 / 3.0 / Ab * sqrt(Fi.y * Fi.y + Fi.z * Fi.z);
                 if (sc >= sf.sc || stt >= sf.st || ss >= sf.ss) {
                     pq[e] = word | BROKEN;  // both copies take the same decision
-                    if (plo) ++broken;
+                    if (plo) {
+                        ++broken;
+                        if (SEG) atomicAdd(&P.seg_broken[P.sid[p]], 1ull);
+                    }
                     continue;
                 }
                 const m3 Ri = inverse(R);
@@ -659,10 +668,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const float Rs = 1.0f / (1.0f / (float)ri + 1.0f / (float)rj);
                 const float ms = 1.0f / (1.0f / (float)mi + 1.0f / (float)mj);
                 const float srd = sqrtf(Rs * (float)dn);
-                const float kn = (4.0f / 3.0f) * P.ys_pp * srd;
-                const float kt = 8.0f * P.gs_pp * srd;
-                const float gn = -2.0f * P.beta_pp * sqrtf((5.0f / 6.0f) * (2.0f * P.ys_pp * srd) * ms);
-                const float gt = -2.0f * P.beta_pp * sqrtf((5.0f / 6.0f) * kt * ms);
+                const float kn = (4.0f / 3.0f) * S.ys_pp * srd;
+                const float kt = 8.0f * S.gs_pp * srd;
+                const float gn = -2.0f * S.beta_pp * sqrtf((5.0f / 6.0f) * (2.0f * S.ys_pp * srd) * ms);
+                const float gt = -2.0f * S.beta_pp * sqrtf((5.0f / 6.0f) * kt * ms);
                 // the tangential spring accumulates in FP64: v dt increments vanish in FP32
                 d3 sv = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
                 sv = sv + mk3((double)vc.x * 0.0, (double)vc.y * dt, (double)vc.z * dt);
@@ -670,8 +679,8 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const f3 Ft = fscl(tof(sv), -kt);
                 const float nFt = fnorm(Ft);
                 float F2, F3;
-                if (nFt >= P.mu_pp * F1) {
-                    const float ratio = P.mu_pp * F1 / nFt;
+                if (nFt >= S.mu_pp * F1) {
+                    const float ratio = S.mu_pp * F1 / nFt;
                     F2 = ratio * Ft.y;
                     F3 = ratio * Ft.z;
                     sv = mk3(sv.x, (double)(F2 / kt), (double)(F3 / kt));
@@ -693,7 +702,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 }
             } else {
                 // Hertz-Mindlin (kernels.jl:240-313)
-                const m3 R = contact_frame(normalize(kj - ki), P.tol);
+                const m3 R = contact_frame(normalize(kj - ki), S.tol);
                 const double gap = Li - ri - rj;
                 const double dn = fabs(gap);
                 const d3 kc = ki + normalize(kj - ki) * (ri - dn);
@@ -701,12 +710,12 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const d3 vci = cross(wi, kip) + vi;
                 const d3 vcj = cross(wj, kjp) + vj;
                 const d3 vc = mul(R, vcj - vci);
-                const SurfaceP& sf = P.surf[0][0];
+                const SurfaceP& sf = S.surf[0][0];
                 const double Rs = 1.0 / (1.0 / ri + 1.0 / rj);
                 const double ms = 1.0 / (1.0 / mi + 1.0 / mj);
                 const d3 s0 = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
-                const HMOut h = hertz_mindlin(gap, dn, vc, P.matNu[0], P.matE[0], P.matNu[0],
-                                              P.matE[0], Rs, ms, sf, dt, s0);
+                const HMOut h = hertz_mindlin(gap, dn, vc, S.matNu[0], S.matE[0], S.matNu[0],
+                                              S.matE[0], Rs, ms, sf, dt, s0);
                 const m3 Ri = inverse(R);
                 const d3 Fg = mul(Ri, -h.F);
                 if (plo) { F = F + Fg; T = T + cross(kip, Fg); }
@@ -725,13 +734,13 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             T = T + TH.value();
         }
         // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
-        uint32_t wa = P.nwall ? wact[p] : 0u, wa_new = 0u;
-        for (int j = 0; j < P.nwall; ++j) {
-            const WallP& wl = P.walls[j];
+        uint32_t wa = S.nwall ? wact[p] : 0u, wa_new = 0u;
+        for (int j = 0; j < S.nwall; ++j) {
+            const WallP& wl = S.walls[j];
             const d3 nrm = mk3(wl.n[0], wl.n[1], wl.n[2]);
             if (fabs(dot(kp, nrm) - wl.d) < X.w) {
                 ++nwallc;
-                const m3 R = contact_frame(nrm, P.tol);
+                const m3 R = contact_frame(nrm, S.tol);
                 const double L = dot(kp, nrm) - wl.d;
                 const double gap = fabs(L) - X.w;
                 const double dn = fabs(gap);
@@ -739,14 +748,14 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const d3 vci = cross(wp, kip) + vp;
                 const d3 vc = mul(R, -vci);
                 const int mj = wl.mid - 1;
-                const SurfaceP& sf = P.surf[0][mj];
+                const SurfaceP& sf = S.surf[0][mj];
                 d3 s0 = mk3(0.0, 0.0, 0.0);
                 if (wa & (1u << j)) {
                     const double4 ws = wshear[(int64_t)j * n + p];
                     s0 = mk3(ws.x, ws.y, ws.z);
                 }
-                const HMOut h = hertz_mindlin(gap, dn, vc, P.matNu[0], P.matE[0], P.matNu[mj],
-                                              P.matE[mj], X.w, Vv.w, sf, dt, s0);
+                const HMOut h = hertz_mindlin(gap, dn, vc, S.matNu[0], S.matE[0], S.matNu[mj],
+                                              S.matE[mj], X.w, Vv.w, sf, dt, s0);
                 if (h.fatal) ++fatal;
                 const m3 Ri = inverse(R);
                 const d3 Fg = mul(Ri, -h.F);
@@ -756,10 +765,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 wa_new |= 1u << j;
             }
         }
-        if (P.nwall && wa_new != wa) wact[p] = wa_new;
+        if (S.nwall && wa_new != wa) wact[p] = wa_new;
         // apply_body_force! (kernels.jl:428-443)
-        F = F + mk3(P.g[0] * Vv.w, P.g[1] * Vv.w, P.g[2] * Vv.w);
-        const double nd = -P.damping;
+        F = F + mk3(S.g[0] * Vv.w, S.g[1] * Vv.w, S.g[2] * Vv.w);
+        const double nd = -S.damping;
         F = F + mk3(nd * fabs(F.x) * jsign(vp.x), nd * fabs(F.y) * jsign(vp.y), nd * fabs(F.z) * jsign(vp.z));
         T = T + mk3(nd * fabs(T.x) * jsign(wp.x), nd * fabs(T.y) * jsign(wp.y), nd * fabs(T.z) * jsign(wp.z));
         if (write_forces) {
@@ -818,7 +827,7 @@ k_debug_cells(const Params P, const double4* __restrict__ pos, int32_t* kid, uin
     uint32_t w[8];
     int32_t c[3];
     double ctr[3];
-    window_of(P.grid, mk3(x.x, x.y, x.z), w, c, ctr);
+    window_of(P.sc.grid, mk3(x.x, x.y, x.z), w, c, ctr);
     for (int a = 0; a < 3; ++a) { kid[3 * p + a] = c[a]; kc[3 * p + a] = ctr[a]; }
     hid[p] = w[0] + 1u;
     for (int k = 0; k < 8; ++k) nbr[8 * p + k] = w[k] + 1u;
@@ -834,7 +843,7 @@ k_cp_list(const Params P, const double4* __restrict__ pos, const uint32_t* __res
     if (i >= P.n) return;
     const double4 x = pos[i];
     uint32_t w[8];
-    window_of(P.grid, mk3(x.x, x.y, x.z), w);
+    window_of(P.sc.grid, mk3(x.x, x.y, x.z), w);
     uint32_t c = 0;
     const uint32_t base = EMIT ? off[i] : 0u;
     for (int wi = 0; wi < 8; ++wi) {
@@ -914,6 +923,55 @@ k_copy(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n) {
         dst[k] = src[k];
 }
 
+// Copy one scene's complete dynamic state between two scenes/batches on the same device:
+// particle buffers, wall history, and contact rows with partner indices moved from src_off to
+// dst_off (rows land in buffer 0 of the destination).
+struct AdoptArgs {
+    int64_t n, src_off, dst_off, Ns, Nd;
+    int32_t Ks, Kd, nwall, src_cur, dst_cur;
+    const double4 *spos, *svel, *somg, *squat, *sacc, *sdw, *swshear;
+    double4 *dpos, *dvel, *domg, *dquat, *dacc, *ddw, *dwshear;
+    const uint32_t* swact;
+    uint32_t* dwact;
+    const uint8_t* ssel;
+    uint8_t* dsel;
+    const uint32_t *srcnt0, *srcnt1, *spq0, *spq1;
+    uint32_t *drcnt0, *drcnt1, *dpq0;
+    const double *sst0, *sst1;
+    double* dst0;
+    uint32_t* overflow;
+};
+
+__global__ void __launch_bounds__(TPB) k_adopt(const AdoptArgs a) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p >= a.n) return;
+    const int64_t ps = a.src_off + p, pd = a.dst_off + p;
+    a.dpos[pd] = a.spos[ps];
+    a.dvel[pd] = a.svel[ps];
+    a.domg[pd] = a.somg[ps];
+    a.dquat[pd] = a.squat[ps];
+    a.dacc[pd] = a.sacc[ps];
+    a.ddw[pd] = a.sdw[ps];
+    a.dwact[pd] = a.swact[ps];
+    for (int j = 0; j < a.nwall; ++j) a.dwshear[(int64_t)j * a.Nd + pd] = a.swshear[(int64_t)j * a.Ns + ps];
+    const int b = a.ssel[ps];
+    const uint32_t* pq = b ? a.spq1 : a.spq0;
+    const double* st = b ? a.sst1 : a.sst0;
+    uint32_t cnt = (b ? a.srcnt1 : a.srcnt0)[ps];
+    if ((int)cnt > a.Kd) { *a.overflow = 1u; cnt = (uint32_t)a.Kd; }
+    const int64_t css = (int64_t)a.Ks * a.Ns, csd = (int64_t)a.Kd * a.Nd;
+    for (uint32_t m = 0; m < cnt; ++m) {
+        const uint32_t w = pq[(int64_t)m * a.Ns + ps];
+        const uint32_t q = (uint32_t)((int64_t)(w & IDX_MASK) - a.src_off + a.dst_off);
+        a.dpq0[(int64_t)m * a.Nd + pd] = (w & ~IDX_MASK) | q;
+        for (int c = 0; c < S_NCOMP; ++c)
+            a.dst0[c * csd + (int64_t)m * a.Nd + pd] = st[c * css + (int64_t)m * a.Ns + ps];
+    }
+    a.drcnt0[pd] = cnt;
+    a.drcnt1[pd] = 0u;
+    a.dsel[pd] = 0;
+}
+
 inline dim3 grid_for(int64_t n, int tpb) { return dim3((uint32_t)((n + tpb - 1) / tpb)); }
 
 }  // namespace
@@ -942,20 +1000,21 @@ void timing_end(dem_scene* s) {
         if (e_ != hipSuccess) return e_;    \
     } while (0)
 
-static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) {
+template <bool SEG>
+static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
     if (dense) {
-        LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<true>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
-                                             P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
+        LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<true, SEG>), grid_for(P.n, TPB), dim3(TPB), 0,
+                                             s->stream, P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
                                              reset_counters ? 1 : 0));
-    } else {
-        LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<false>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
-                                             P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
+    } else {  // the reference's hashed grid: single scenes only
+        LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<false, false>), grid_for(P.n, TPB), dim3(TPB), 0,
+                                             s->stream, P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
                                              reset_counters ? 1 : 0));
     }
-    const int64_t nkeys = dense ? (int64_t)P.dense.cells : (int64_t)P.grid.H;
+    const int64_t nkeys = dense ? (int64_t)P.sort_keys : (int64_t)P.sc.grid.H;
     timing_begin(s, K_SCAN);
     hipError_t e = launch_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, true, nullptr, s->stream);
     timing_end(s);
@@ -966,12 +1025,19 @@ static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) 
     return hipSuccess;
 }
 
-hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates) {
+static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) {
+    return s->batched ? sort_impl<true>(s, reset_counters, dense)
+                      : sort_impl<false>(s, reset_counters, dense);
+}
+
+template <bool SEG>
+static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
-    const bool dense = P.dense.cells > 0;
-    hipError_t e = sort_particles(s, true, dense);
+    const bool dense = s->dense;
+    if (SEG && !dense) return hipErrorInvalidValue;
+    hipError_t e = sort_impl<SEG>(s, true, dense);
     if (e != hipSuccess) return e;
     const bool mixed = s->precision == DEM_PRECISION_MIXED && !init;
     const dim3 g = grid_for(P.n, TPB);
@@ -980,17 +1046,17 @@ hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates)
                                               d.sidx, d.bstart, d.ctr));
     if (dense) {
         if (init) {
-            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, false>), g, dim3(TPB), 0, s->stream,
-                                                 P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                                 d.touch, d.ctr));
+            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, false, SEG>), g, dim3(TPB), 0,
+                                                 s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
+                                                 d.tcnt, d.touch, d.ctr));
         } else if (mixed) {
-            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, true>), g, dim3(TPB), 0, s->stream,
-                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                                  d.touch, d.ctr));
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, true, SEG>), g, dim3(TPB), 0,
+                                                  s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
+                                                  d.tcnt, d.touch, d.ctr));
         } else {
-            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, false>), g, dim3(TPB), 0, s->stream,
-                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                                  d.touch, d.ctr));
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, false, SEG>), g, dim3(TPB), 0,
+                                                  s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
+                                                  d.tcnt, d.touch, d.ctr));
         }
     } else if (s->slot_owned) {
         if (init) {
@@ -1022,37 +1088,43 @@ hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates)
     const size_t lds = (size_t)MERGE_TPB * P.K * sizeof(uint32_t);
     const dim3 gm = grid_for(P.n, MERGE_TPB);
     if (init) {
-        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true>), gm, dim3(MERGE_TPB), lds, s->stream,
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG>), gm, dim3(MERGE_TPB), lds, s->stream,
                                               P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
                                               d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
     } else {
-        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false>), gm, dim3(MERGE_TPB), lds, s->stream,
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG>), gm, dim3(MERGE_TPB), lds, s->stream,
                                               P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
                                               d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
     }
-    if (count_candidates && dense) {
+    if (count_candidates && dense && !SEG) {
         // the cp_list length needs the reference's bucket sort: rebuild it after the merge
-        if ((e = sort_particles(s, false, false)) != hipSuccess) return e;
+        if ((e = sort_impl<false>(s, false, false)) != hipSuccess) return e;
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
                                               d.sidx, d.bstart, d.ctr));
     }
     return hipSuccess;
 }
 
-hipError_t run_force_integrate(dem_scene* s, bool write_forces) {
+hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates) {
+    return s->batched ? detect_impl<true>(s, init, count_candidates)
+                      : detect_impl<false>(s, init, count_candidates);
+}
+
+template <bool SEG>
+static hipError_t force_impl(dem_scene* s, bool write_forces) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur, nx = 1 - s->cur;
     const dim3 g = grid_for(P.n, TPB);
     if (s->precision == DEM_PRECISION_MIXED) {
-        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<true>), g, dim3(TPB), 0, s->stream, P,
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<true, SEG>), g, dim3(TPB), 0, s->stream, P,
                                               d.pos[c], d.vel[c], d.omg[c], d.quat[c], d.pos[nx],
                                               d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
                                               d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
                                               d.st[0], d.st[1], d.wact, d.wshear, d.ctr,
                                               write_forces ? 1 : 0));
     } else {
-        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<false>), g, dim3(TPB), 0, s->stream, P,
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<false, SEG>), g, dim3(TPB), 0, s->stream, P,
                                               d.pos[c], d.vel[c], d.omg[c], d.quat[c], d.pos[nx],
                                               d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
                                               d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
@@ -1061,6 +1133,10 @@ hipError_t run_force_integrate(dem_scene* s, bool write_forces) {
     }
     s->cur = nx;
     return hipSuccess;
+}
+
+hipError_t run_force_integrate(dem_scene* s, bool write_forces) {
+    return s->batched ? force_impl<true>(s, write_forces) : force_impl<false>(s, write_forces);
 }
 
 hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc, uint32_t* nbr) {
@@ -1129,6 +1205,26 @@ hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint3
     if (m <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_write_rows, grid_for(m, TPB), dim3(TPB), 0, s->stream, s->P, m, dst_dev,
                        word_dev, state_dev, s->d.pq[0], s->d.st[0]);
+    return hipGetLastError();
+}
+
+hipError_t run_adopt(dem_scene* dst, int64_t dst_off, const dem_scene* src, int64_t src_off,
+                     int64_t n, int32_t nwall, uint32_t* overflow_dev) {
+    AdoptArgs a;
+    a.n = n; a.src_off = src_off; a.dst_off = dst_off; a.Ns = src->n; a.Nd = dst->n;
+    a.Ks = src->K; a.Kd = dst->K; a.nwall = nwall; a.src_cur = src->cur; a.dst_cur = dst->cur;
+    const DevState& S = src->d;
+    DevState& D = dst->d;
+    a.spos = S.pos[src->cur]; a.svel = S.vel[src->cur]; a.somg = S.omg[src->cur]; a.squat = S.quat[src->cur];
+    a.sacc = S.acc; a.sdw = S.dw; a.swshear = S.wshear; a.swact = S.wact;
+    a.dpos = D.pos[dst->cur]; a.dvel = D.vel[dst->cur]; a.domg = D.omg[dst->cur]; a.dquat = D.quat[dst->cur];
+    a.dacc = D.acc; a.ddw = D.dw; a.dwshear = D.wshear; a.dwact = D.wact;
+    a.ssel = S.sel; a.dsel = D.sel;
+    a.srcnt0 = S.rcnt[0]; a.srcnt1 = S.rcnt[1]; a.spq0 = S.pq[0]; a.spq1 = S.pq[1];
+    a.drcnt0 = D.rcnt[0]; a.drcnt1 = D.rcnt[1]; a.dpq0 = D.pq[0];
+    a.sst0 = S.st[0]; a.sst1 = S.st[1]; a.dst0 = D.st[0];
+    a.overflow = overflow_dev;
+    hipLaunchKernelGGL(k_adopt, grid_for(n, TPB), dim3(TPB), 0, dst->stream, a);
     return hipGetLastError();
 }
 

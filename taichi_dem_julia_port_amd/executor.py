@@ -197,6 +197,76 @@ class Scene:
                 for k in range(N.DEM_NUM_KERNELS)}
 
 
+class Batch(Scene):
+    """Several independent scenes advanced together: one launch per kernel stage for all of them
+    (dem_create_batch).  Particle indices are global; ``offsets[k]:offsets[k+1]`` is scene k."""
+
+    def __init__(self, members, precision: str = "fp64", contact_slots: int = 0, device: int = 0,
+                 graph: bool = True):
+        self._lib = N.load_library()
+        members = list(members)
+        S = len(members)
+        cfgs = (N.DemConfig * S)()
+        nwalls = np.zeros(S, dtype=np.int32)
+        counts = np.zeros(S, dtype=np.int64)
+        walls, grains = [], []
+        for k, (cfg, g, w) in enumerate(members):
+            cfgs[k] = cfg.to_c()
+            if w is None:
+                w = [(cfg.wall_normal, cfg.wall_distance, 2)]
+            wa = w if isinstance(w, np.ndarray) and w.dtype == WALL_DTYPE else make_walls(w)
+            walls.append(wa)
+            nwalls[k] = len(wa)
+            g = np.ascontiguousarray(g, dtype=GRAIN_IN_DTYPE)
+            grains.append(g)
+            counts[k] = len(g)
+        allw = np.ascontiguousarray(np.concatenate(walls)) if sum(map(len, walls)) else \
+            np.zeros(1, dtype=WALL_DTYPE)
+        allg = np.ascontiguousarray(np.concatenate(grains))
+        flags = 0 if graph else N.DEM_FLAG_NO_GRAPH
+        opt = N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), flags)
+        h = ctypes.c_void_p()
+        rc = self._lib.dem_create_batch(S, cfgs, _ptr(nwalls, ctypes.c_int32), _ptr(allw, N.DemWall),
+                                        _ptr(counts, ctypes.c_int64), _ptr(allg, N.DemGrainIn),
+                                        ctypes.byref(opt), ctypes.byref(h))
+        if rc != 0:
+            raise DEMError(rc, self._lib.dem_last_error(None).decode())
+        self._h = h
+        self.cfg = members[0][0]
+        self.n = int(counts.sum())
+        self.nscenes = S
+        self.nwall = int(nwalls.max()) if S else 0
+        self.walls = allw
+        self.precision = precision
+        off = np.zeros(S + 1, dtype=np.int64)
+        self._check(self._lib.dem_batch_offsets(self._h, _ptr(off, ctypes.c_int64)))
+        self.offsets = off
+
+    def set_bond_init(self, mask) -> None:
+        m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))
+        self._check(self._lib.dem_batch_set_bond_init(self._h, _ptr(m, ctypes.c_uint8)))
+
+    def bonds_broken(self) -> np.ndarray:
+        out = np.zeros(self.nscenes, dtype=np.int64)
+        self._check(self._lib.dem_batch_bonds_broken(self._h, _ptr(out, ctypes.c_int64)))
+        return out
+
+    def split_grains(self, grains: np.ndarray):
+        return [grains[self.offsets[k]:self.offsets[k + 1]] for k in range(self.nscenes)]
+
+    def split_contacts(self, contacts: np.ndarray, bonded: np.ndarray):
+        """Per-scene contact lists with scene-local 1-based indices."""
+        scene = np.searchsorted(self.offsets, contacts["i"].astype(np.int64) - 1, side="right") - 1
+        out = []
+        for k in range(self.nscenes):
+            m = scene == k
+            c = contacts[m].copy()
+            c["i"] -= self.offsets[k]
+            c["j"] -= self.offsets[k]
+            out.append((c, bonded[m]))
+        return out
+
+
 def device_scan(values: np.ndarray):
     """Exclusive prefix sum with the engine's decoupled-lookback scan."""
     lib = N.load_library()

@@ -112,3 +112,25 @@ def pour_config(box=POUR_BOX, dt: float = 2e-7, **overrides) -> DEMConfig:
 def pour(n: int = 1_000_000, seed: int = 3, box=POUR_BOX, **overrides):
     """The polydisperse pour (no bonds: call Scene.step without Scene.initialize)."""
     return pour_config(box, **overrides), pour_grains(n, box, seed), pour_walls(box)
+
+
+def batched_cubes(nscenes: int = 256, seed: int = 4, nmin: int = 500, nmax: int = 20000,
+                  dt: float = 5e-7, base_gap: float = 1e-3):
+    """The batched ensemble: bonded cubes with particle counts log-uniform in [nmin, nmax], per
+    scene E in [1e7, 1e9] Pa, mu in [0.2, 0.8], bond strength in [1e5, 1e7] Pa (tensile and
+    shear) and impact speed in [2, 30] m/s.  Returns a list of (cfg, grains, walls)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nscenes):
+        n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
+        nx = max(2, int(round(n ** (1.0 / 3.0))))
+        nz = max(2, int(round(n / (nx * nx))))
+        E = math.exp(rng.uniform(math.log(1e7), math.log(1e9)))
+        mu = rng.uniform(0.2, 0.8)
+        sb = math.exp(rng.uniform(math.log(1e5), math.log(1e7)))
+        v = rng.uniform(2.0, 30.0)
+        cfg, g, w = bonded_cube(nx, nx, nz, speed=v, dt=dt, seed=1000 + k, base_gap=base_gap,
+                                particle_elastic_modulus=E, pp_coefficient_friction=mu,
+                                bond_tensile_strength=sb, bond_shear_strength=sb)
+        out.append((cfg, g, w))
+    return out
