@@ -194,6 +194,9 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
                  !(flags & DEM_FLAG_INDEX_OWNED_BROAD_PHASE);
 
     memset(&S, 0, sizeof(S));
+    // A touching (or bond-distance) pair lies in both particles' reference windows when the
+    // contact reach is below cs/2 with margin; otherwise the broad phase checks the rule per pair.
+    S.ref_check = reach * (1.0 + 1e-6) < 0.5 * der.cell_size ? 0 : 1;
     for (int a = 0; a < 3; ++a) S.grid.dmin[a] = cfg->domain_min[a];
     S.grid.cs = der.cell_size;
     S.grid.H = (uint64_t)H;

@@ -53,7 +53,7 @@ struct SceneP {
     uint32_t cell_base; // first global sort key of this scene's dense cells
     int32_t nwall;
     int32_t bond_init;  // scene takes part in dem_initialize
-    int32_t pad;
+    int32_t ref_check;  // touching pairs must be checked against the reference window rule
     double bond_mult;   // particle_contact_radius_multiplier (r0 = r * bond_mult)
     double dt, tol, damping;
     double g[3];

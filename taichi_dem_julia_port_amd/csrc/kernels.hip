@@ -218,7 +218,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
               const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
               unsigned long long* ctr) {
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    uint64_t found = 0, examined = 0;
+    uint64_t found = 0;
     uint32_t overflow = 0;
     uint32_t p = 0xffffffffu;
     if (SEG && t < P.n) p = sidx[t];
@@ -250,7 +250,6 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
                 uint32_t k = bstart[row + (uint32_t)x0];
                 const uint32_t s1 = bstart[row + (uint32_t)x1 + 1];
                 if (k <= (uint32_t)t) k = (uint32_t)t + 1;
-                if (k < s1) examined += s1 - k;
                 for (; k < s1; ++k) {
                     if (MIXED) {
                         const float4 o = spos32[k];
@@ -271,7 +270,10 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
                     const double4 xl = plo ? me : o, xh = plo ? o : me;
                     const double rl = INIT ? r0[lo] : xl.w, rh = INIT ? r0[hi] : xh.w;
                     if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
-                    if (!ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z))) continue;
+                    // when the contact reach is below cs/2 every touching pair passes this test
+                    if (S.ref_check &&
+                        !ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
+                        continue;
                     ++found;
                     row_append(tcnt, touch, P.n, P.K, p, q, overflow);
                     row_append(tcnt, touch, P.n, P.K, q, p, overflow);
@@ -279,9 +281,43 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
             }
         }
     }
-    wave_add(&ctr[CT_EXAMINED], examined);
     wave_add(&ctr[CT_TOUCH], found);
     if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+}
+
+// Candidate pairs the dense broad phase examines (slots k > t of the 9 stencil ranges): a
+// diagnostic, evaluated on the last step of each dem_step call.  (Counting inside the broad-phase
+// loop itself costs more than the loop.)
+template <bool SEG>
+__global__ void __launch_bounds__(TPB)
+k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
+                 const uint32_t* __restrict__ bstart, unsigned long long* ctr) {
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    uint64_t c = 0;
+    if (t < P.n) {
+        const SceneP& S = scene_of<SEG>(P, SEG ? sidx[t] : 0u);
+        const DenseGrid& G = S.dense;
+        const double4 me = spos[t];
+        int32_t cc[3];
+        dense_cell(G, mk3(me.x, me.y, me.z), cc);
+        const int32_t x0 = cc[0] > 0 ? cc[0] - 1 : 0;
+        const int32_t x1 = cc[0] + 1 < G.dims[0] ? cc[0] + 1 : G.dims[0] - 1;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int32_t z = cc[2] + dz;
+            if (z < 0 || z >= G.dims[2]) continue;
+            for (int dy = -1; dy <= 1; ++dy) {
+                const int32_t y = cc[1] + dy;
+                if (y < 0 || y >= G.dims[1]) continue;
+                const uint32_t row = S.cell_base +
+                                     (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
+                uint32_t k = bstart[row + (uint32_t)x0];
+                const uint32_t s1 = bstart[row + (uint32_t)x1 + 1];
+                if (k <= (uint32_t)t) k = (uint32_t)t + 1;
+                if (k < s1) c += s1 - k;
+            }
+        }
+    }
+    wave_add(&ctr[CT_EXAMINED], c);
 }
 
 // Candidate-pair count of the reference's cp_list (kernels.jl:24-37): a diagnostic, evaluated on
@@ -1096,6 +1132,9 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
                                               P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
                                               d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
     }
+    if (count_candidates && dense)
+        LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
+                                              d.sidx, d.spos, d.bstart, d.ctr));
     if (count_candidates && dense && !SEG) {
         // the cp_list length needs the reference's bucket sort: rebuild it after the merge
         if ((e = sort_impl<false>(s, false, false)) != hipSuccess) return e;

@@ -30,6 +30,10 @@ def _cases():
         "lattice_10x10x9": lambda: scenes.cube_small(),
         "pour_4k": lambda: pour_sample(4000),
         "packed_4k": lambda: packed_sample(4000),
+        # safety factor < 1: contact reach exceeds cs/2, so the reference window rule is checked
+        # per touching pair and the reference-grid path is index-owned
+        "lattice_safety09": lambda: scenes.bonded_cube(8, 8, 6, speed=10.0, dt=1e-6, base_gap=2e-5,
+                                                       neighboring_search_safety_factor=0.9),
     }
 
 
