@@ -84,9 +84,10 @@ def build_workload(name: str, n: int):
     raise SystemExit(f"unknown workload {name}")
 
 
-def run_engine(cfg, g, w, bonded, precision, steps, warmup):
+def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
     from taichi_dem_julia_port_amd.executor import Batch, Scene
-    s = Batch(cfg, precision=precision) if g is None else Scene(cfg, g, w, precision=precision)
+    s = (Batch(cfg, precision=precision, graph=graph) if g is None
+         else Scene(cfg, g, w, precision=precision, graph=graph))
     if bonded:
         s.initialize()
     s.step(warmup)
@@ -200,6 +201,8 @@ def main():
                     help="also time the other precision and report both")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="launch every step eagerly (no hipGraph replay)")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -227,7 +230,8 @@ def main():
     n = len(g) if g is not None else sum(len(m[1]) for m in cfg)
     if g is None:
         w = cfg[0][2]
-    r = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup)
+    r = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup,
+                   graph=not args.no_graph)
     value = n * args.steps / r["elapsed"]
     kt = r["kernels"]
     step_ms = {k: (ms / max(l, 1)) for k, (ms, l) in kt.items() if l}
@@ -274,6 +278,7 @@ def main():
         "kernel_ms_per_launch": {k: round(v, 5) for k, v in step_ms.items()},
         "kernel_share": {k: round(ms / total_kernel_ms, 4) for k, (ms, l) in kt.items() if l},
         "ms_per_step_with_kernel_events": round(1e3 * r["elapsed_events"] / args.steps, 5),
+        "graph": not args.no_graph,
         "roofline": roofline,
         "kernel_GB_s": per_kernel,
     }
@@ -285,7 +290,8 @@ def main():
                                 "sample_scenes": idx, "speedup_batched": round(value / v1, 2)}
     if args.compare_precision:
         other = "mixed" if args.precision == "fp64" else "fp64"
-        r2 = run_engine(cfg, g, w, bonded, other, args.steps, args.warmup)
+        r2 = run_engine(cfg, g, w, bonded, other, args.steps, args.warmup,
+                        graph=not args.no_graph)
         out[f"value_{other}"] = round(n * args.steps / r2["elapsed"], 1)
         out[f"ms_per_step_{other}"] = round(1e3 * r2["elapsed"] / args.steps, 5)
     if not args.no_cpu_baseline:

@@ -184,7 +184,8 @@ int32_t dem_write_grains(dem_scene_t* s, const dem_grain_t* in, int64_t n);
 int32_t dem_read_contacts(dem_scene_t* s, dem_contact_t* out, uint8_t* bonded, int64_t cap,
                           int64_t* count);
 /* Replace the contact set (every given contact active; bonded per flag).  Used to restore a
- * saved state.  Duplicate (i, j) pairs are rejected with DEM_ERR_DUPLICATE. */
+ * saved state.  Duplicate (i, j) pairs are rejected with DEM_ERR_DUPLICATE.  A bond's tangential
+ * spring s is stored as zero, the value every step of the reference leaves it at. */
 int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_t* bonded,
                            int64_t n);
 /* Wall shear history wall_contacts[j, i].s (kernels.jl:392,412-422) as [n][nwall][3]. */

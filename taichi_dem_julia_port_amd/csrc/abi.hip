@@ -57,6 +57,7 @@ void free_scene(dem_scene* s) {
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (s->h_ctr) hipHostFree(s->h_ctr);
+    if (s->h_ctr_raw) hipHostFree(s->h_ctr_raw);
     for (hipEvent_t e : s->ev)
         if (e) hipEventDestroy(e);
     if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
@@ -68,9 +69,14 @@ void free_scene(dem_scene* s) {
 }
 
 int32_t sync_counters(dem_scene* s) {
-    HIPCHK(s, hipMemcpyAsync(s->h_ctr, s->d.ctr, sizeof(unsigned long long) * CT_NCOUNTERS,
+    HIPCHK(s, hipMemcpyAsync(s->h_ctr_raw, s->d.ctr, sizeof(unsigned long long) * CTR_WORDS,
                              hipMemcpyDeviceToHost, s->stream));
     HIPCHK(s, hipStreamSynchronize(s->stream));
+    for (int k = 0; k < CT_NCOUNTERS; ++k) {
+        unsigned long long v = 0;
+        for (int c = 0; c < CTR_COPIES; ++c) v += s->h_ctr_raw[c * CTR_STRIDE + k];
+        s->h_ctr[k] = v;
+    }
     return DEM_OK;
 }
 
@@ -109,7 +115,7 @@ int32_t resize_rows(dem_scene* s, int32_t k) {
     }
     HIPCHK(s, hipMemset(d.sel, 0, s->n));
     HIPCHK(s, hipMemset(d.tcnt, 0, s->n * sizeof(uint32_t)));
-    HIPCHK(s, hipMemset(d.ctr, 0, CT_NCOUNTERS * sizeof(unsigned long long)));
+    HIPCHK(s, hipMemset(d.ctr, 0, CTR_WORDS * sizeof(unsigned long long)));
     s->K = k;
     s->P.K = k;
     s->der.contact_slots = k;
@@ -282,10 +288,9 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     ALLOC(d.acc, n); ALLOC(d.dw, n); ALLOC(d.ftot, n); ALLOC(d.ttot, n); ALLOC(d.r0, n);
     ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall_max > 0 ? nwall_max : 1) * n);
     ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.bcnt, nkeys + 1); ALLOC(d.bstart, nkeys + 1);
-    ALLOC(d.sidx, n); ALLOC(d.spos, n);
-    if (s->precision == DEM_PRECISION_MIXED) { ALLOC(d.spos32, n); }
+    ALLOC(d.sidx, n); ALLOC(d.spos, n); ALLOC(d.spos32, n);
     ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
-    ALLOC(d.ctr, CT_NCOUNTERS);
+    ALLOC(d.ctr, CTR_WORDS);
     const int64_t max_tiles = scan_tiles(std::max<int64_t>(nkeys + 1, n + 1));
     ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2);
     d.scan.max_tiles = max_tiles;
@@ -295,6 +300,9 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     if (hipHostMalloc((void**)&s->h_ctr, sizeof(unsigned long long) * CT_NCOUNTERS) != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, "hipHostMalloc failed");
     memset(s->h_ctr, 0, sizeof(unsigned long long) * CT_NCOUNTERS);
+    if (hipHostMalloc((void**)&s->h_ctr_raw, sizeof(unsigned long long) * CTR_WORDS) != hipSuccess)
+        return fail(nullptr, DEM_ERR_HIP, "hipHostMalloc failed");
+    memset(s->h_ctr_raw, 0, sizeof(unsigned long long) * CTR_WORDS);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, "hipStreamCreate failed");
     std::vector<double4> hp(n), hv(n), hw(n), hq(n);
@@ -529,12 +537,20 @@ int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, i
     if (src->batched) {
         HIPCHK(dst, hipMemcpy(&nb, src->d.seg_broken + src_scene, sizeof(nb), hipMemcpyDeviceToHost));
     } else {
-        HIPCHK(dst, hipMemcpy(&nb, src->d.ctr + CT_BROKEN_TOTAL, sizeof(nb), hipMemcpyDeviceToHost));
+        int32_t rc = sync_counters(src);
+        if (rc) return fail(dst, rc, "dem_batch_adopt: reading the source counters failed");
+        nb = src->h_ctr[CT_BROKEN_TOTAL];
     }
     if (dst->batched) {
         HIPCHK(dst, hipMemcpy(dst->d.seg_broken + dst_scene, &nb, sizeof(nb), hipMemcpyHostToDevice));
     } else {
-        HIPCHK(dst, hipMemcpy(dst->d.ctr + CT_BROKEN_TOTAL, &nb, sizeof(nb), hipMemcpyHostToDevice));
+        // the total goes into copy 0, the other copies restart from zero
+        std::vector<unsigned long long> raw(CTR_WORDS);
+        HIPCHK(dst, hipMemcpy(raw.data(), dst->d.ctr, CTR_WORDS * sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost));
+        for (int c = 0; c < CTR_COPIES; ++c) raw[c * CTR_STRIDE + CT_BROKEN_TOTAL] = c == 0 ? nb : 0ull;
+        HIPCHK(dst, hipMemcpy(dst->d.ctr, raw.data(), CTR_WORDS * sizeof(unsigned long long),
+                              hipMemcpyHostToDevice));
     }
     return DEM_OK;
 }

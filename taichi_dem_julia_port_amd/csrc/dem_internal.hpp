@@ -38,6 +38,17 @@ enum CounterIdx {
     CT_NCOUNTERS = 16
 };
 
+// Counters are kept in CTR_COPIES copies, 256 B apart, and a block adds to copy blockIdx % CTR_COPIES:
+// device-scope atomics on one address serialise, and one copy per wave of a million-particle
+// launch would cost more than the launch.  The host sums the copies.
+constexpr int CTR_COPIES = 64;
+constexpr int CTR_STRIDE = 32;
+constexpr int CTR_WORDS = CTR_COPIES * CTR_STRIDE;
+
+__device__ __forceinline__ unsigned long long* ctr_at(unsigned long long* ctr, int k) {
+    return ctr + (size_t)(blockIdx.x % CTR_COPIES) * CTR_STRIDE + k;
+}
+
 struct SurfaceP {
     double mu, e, beta;          // friction, restitution, log(e)/sqrt(log(e)^2 + pi^2)
     double rho, E, nu;           // bond radius ratio, modulus, Poisson ratio
@@ -115,7 +126,7 @@ struct DevState {
     uint32_t* bstart = nullptr;  // [H + 1] exclusive bucket starts
     uint32_t* sidx = nullptr;    // sorted slot -> particle
     double4* spos = nullptr;     // sorted copy of pos
-    float4* spos32 = nullptr;    // sorted FP32 copy (mixed precision)
+    float4* spos32 = nullptr;    // sorted FP32 copy (broad-phase pre-test)
     // broad phase output
     uint32_t* tcnt = nullptr;    // [n]
     uint32_t* touch = nullptr;   // [K][n]
@@ -169,7 +180,8 @@ struct dem_scene {
     double* h_r = nullptr;
     double* h_r0 = nullptr;
     double* h_I = nullptr;
-    unsigned long long* h_ctr = nullptr;  // pinned
+    unsigned long long* h_ctr = nullptr;      // pinned, summed over the copies
+    unsigned long long* h_ctr_raw = nullptr;  // pinned, CTR_WORDS
     std::string err;
     // two steps captured as one HIP graph (the particle buffers alternate, so two steps restore
     // the buffer parity); replayed for the body of every dem_step call

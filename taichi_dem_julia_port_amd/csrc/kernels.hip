@@ -38,7 +38,9 @@ __global__ void __launch_bounds__(TPB)
 k_hash(const Params P, const double4* __restrict__ pos, uint32_t* __restrict__ hid,
        uint32_t* __restrict__ rank, uint32_t* __restrict__ bcnt, unsigned long long* ctr,
        int reset_counters) {
-    if (reset_counters && blockIdx.x == 0 && threadIdx.x < CT_NSTEP) ctr[threadIdx.x] = 0ull;
+    if (reset_counters && blockIdx.x == 0)
+        for (int i = threadIdx.x; i < CTR_COPIES * CT_NSTEP; i += TPB)
+            ctr[(i / CT_NSTEP) * CTR_STRIDE + i % CT_NSTEP] = 0ull;
     const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (p >= P.n) return;
     const double4 x = pos[p];
@@ -60,7 +62,7 @@ k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __res
     const double4 x = pos[p];
     sidx[slot] = (uint32_t)p;
     spos[slot] = x;
-    if (spos32) spos32[slot] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
+    spos32[slot] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -126,9 +128,9 @@ k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __rest
             }
         }
     }
-    wave_add(&ctr[CT_CAND], cand);
-    wave_add(&ctr[CT_TOUCH], found);
-    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+    wave_add(ctr_at(ctr, CT_CAND), cand);
+    wave_add(ctr_at(ctr, CT_TOUCH), found);
+    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
 }
 
 // Slot-owned broad phase.  A touching pair lies inside both particles' windows (the window
@@ -199,90 +201,211 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
             }
         }
     }
-    wave_add(&ctr[CT_TOUCH], found);
-    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+    wave_add(ctr_at(ctr, CT_TOUCH), found);
+    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
 }
 
-// Dense-grid broad phase.  Cells have side cs/2 = 2 r_max, at least the largest contact distance,
-// so every touching pair lies in the 27-cell stencil; slots are sorted by cell index (x fastest),
-// so the three x-neighbours of a cell row form one contiguous slot range and the stencil is 9
-// ranges.  Pairs are owned by the lower sorted slot.  A squared-distance pre-test skips the square
-// root for candidates clearly apart; survivors get the reference's exact test
+// Blocks are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its
+// own L2.  Renumbering so that XCD x runs the x-th contiguous eighth of the logical blocks keeps
+// the neighbour cells that adjacent blocks share inside one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t NXCD = 8;
+    const uint32_t x = b % NXCD, local = b / NXCD;
+    const uint32_t q = nb / NXCD, r = nb % NXCD;
+    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
+}
+
+// Dense-grid broad phase.  Cells have side >= the largest contact distance, so every touching
+// pair lies in the 27-cell stencil; slots are sorted by cell index (x fastest), so the three
+// x-neighbours of a cell row form one contiguous slot range.  Pairs are owned by the lower sorted
+// slot, and the four stencil rows below the particle's own (dz = -1, or dz = 0 and dy = -1) hold
+// only lower slots, so each thread walks five slot ranges, (dy, dz) = (0,0), (1,0), (-1,1),
+// (0,1), (1,1), flattened into one candidate stream.
+//
+// A block's 256 consecutive slots lie in cells [cf, cl], so the j-th range of each of its threads
+// lies inside the slots of cells [cf + off_j - 1, cl + off_j + 2) (off_j the linear offset of the
+// row).  The block stages the FP32 copies of those five unions in LDS once and every thread then
+// pre-tests its candidates from LDS, eight per batch: |d|^2 against (r_i + r_j)^2 with a margin
+// that covers the FP32 rounding of coordinates and radii, so no touching pair is rejected.  The
+// few survivors get the FP64 squared-distance test, the reference's exact test
 // (norm(k_i - k_j) < r_i + r_j, kernels.jl:103; r0 at t = 0, kernels.jl:72) and the reference's
 // candidate rule (hi's bucket in lo's window, kernels.jl:24-58), so the contact set is the
-// reference's.
-template <bool INIT, bool MIXED, bool SEG>
+// reference's in either precision.  Blocks whose unions exceed the stage, or that straddle two
+// scenes of a batch, read the FP32 copies from global memory instead.
+constexpr int BROAD_BATCH = 8;
+constexpr int STAGE_CAP = 2048;  // float4 per block (32 KiB)
+
+template <bool INIT, bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
               const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
               const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
-              unsigned long long* ctr) {
-    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    uint64_t found = 0;
-    uint32_t overflow = 0;
+              unsigned long long* ctr, int diag) {
+    __shared__ float4 stage[STAGE_CAP];
+    __shared__ uint32_t u_lo[5], u_off[6];  // union slot ranges and their LDS offsets
+    __shared__ uint32_t b_cell[2], b_scene[2];
+    const int64_t n = P.n;
+    const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TPB;
+    const int64_t t = t0 + threadIdx.x;
+    const int64_t tl = (t0 + TPB < n ? t0 + TPB : n) - 1;
+    const bool live = t < n;
+    // this thread's particle; threads t0 and tl also give the block's first and last cells
+    double4 me = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 me32 = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t p = 0xffffffffu;
-    if (SEG && t < P.n) p = sidx[t];
-    const SceneP& S = scene_of<SEG>(P, SEG && t < P.n ? p : 0u);
-    if (t < P.n && (!INIT || S.bond_init)) {
-        const DenseGrid& G = S.dense;
-        const double4 me = spos[t];
-        int32_t c[3];
-        dense_cell(G, mk3(me.x, me.y, me.z), c);
-        float4 me32 = make_float4(0.f, 0.f, 0.f, 0.f);
-        float slack = 0.f;
-        if (MIXED) {
-            me32 = spos32[t];
-            slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
-        }
-        const double mult = INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0;
+    if (live) {
+        me = spos[t];
+        me32 = spos32[t];
+        if (SEG) p = sidx[t];
+    }
+    const SceneP& S = scene_of<SEG>(P, SEG && live ? p : 0u);
+    const DenseGrid& G = S.dense;
+    int32_t c[3] = {0, 0, 0};
+    const uint32_t mycell = live ? dense_cell(G, mk3(me.x, me.y, me.z), c) : 0u;
+    if (threadIdx.x == 0) { b_cell[0] = mycell; b_scene[0] = SEG ? (uint32_t)P.sid[p] : 0u; }
+    if (t == tl) { b_cell[1] = mycell; b_scene[1] = SEG ? (uint32_t)P.sid[p] : 0u; }
+    // this thread's five slot ranges [lo_j, lo_j + len_j), all above t
+    const bool work = live && (!INIT || S.bond_init);
+    uint32_t lo[5], len[5];
+    {
         const int32_t x0 = c[0] > 0 ? c[0] - 1 : 0;
         const int32_t x1 = c[0] + 1 < G.dims[0] ? c[0] + 1 : G.dims[0] - 1;
-#pragma unroll 1
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int32_t z = c[2] + dz;
-            if (z < 0 || z >= G.dims[2]) continue;
-#pragma unroll 1
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int32_t y = c[1] + dy;
-                if (y < 0 || y >= G.dims[1]) continue;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const int dy = j == 0 ? 0 : (j == 1 ? 1 : j - 3);
+            const int dz = j < 2 ? 0 : 1;
+            const int32_t y = c[1] + dy, z = c[2] + dz;
+            lo[j] = 0u; len[j] = 0u;
+            if (work && y >= 0 && y < G.dims[1] && z < G.dims[2]) {
                 const uint32_t row = S.cell_base +
                                      (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
-                uint32_t k = bstart[row + (uint32_t)x0];
-                const uint32_t s1 = bstart[row + (uint32_t)x1 + 1];
-                if (k <= (uint32_t)t) k = (uint32_t)t + 1;
-                for (; k < s1; ++k) {
-                    if (MIXED) {
-                        const float4 o = spos32[k];
-                        const float ex = me32.x - o.x, ey = me32.y - o.y, ez = me32.z - o.z;
-                        const float d2f = ex * ex + ey * ey + ez * ez;
-                        const float thr = (me32.w + o.w) * (1.0f + 64.0f * FLT_EPSILON) + slack;
-                        if (!INIT && d2f > thr * thr) continue;
-                    }
-                    const double4 o = spos[k];
-                    const double ex = me.x - o.x, ey = me.y - o.y, ez = me.z - o.z;
-                    const double d2 = ex * ex + ey * ey + ez * ez;
-                    const double rs = (me.w + o.w) * mult;
-                    if (d2 > rs * rs * (1.0 + 1e-12)) continue;  // conservative: sqrt(d2) >= rs
-                    if (p == 0xffffffffu) p = sidx[t];
-                    const uint32_t q = sidx[k];
-                    const bool plo = p < q;
-                    const uint32_t lo = plo ? p : q, hi = plo ? q : p;
-                    const double4 xl = plo ? me : o, xh = plo ? o : me;
-                    const double rl = INIT ? r0[lo] : xl.w, rh = INIT ? r0[hi] : xh.w;
-                    if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
-                    // when the contact reach is below cs/2 every touching pair passes this test
-                    if (S.ref_check &&
-                        !ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
-                        continue;
-                    ++found;
-                    row_append(tcnt, touch, P.n, P.K, p, q, overflow);
-                    row_append(tcnt, touch, P.n, P.K, q, p, overflow);
-                }
+                uint32_t a = bstart[row + (uint32_t)x0];
+                const uint32_t e = bstart[row + (uint32_t)x1 + 1];
+                if (a <= (uint32_t)t) a = (uint32_t)t + 1;
+                lo[j] = a;
+                len[j] = e > a ? e - a : 0u;
             }
         }
     }
-    wave_add(&ctr[CT_TOUCH], found);
-    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+    __syncthreads();
+    // union ranges (threads 0-4), then their LDS offsets (thread 0)
+    const bool one_scene = !SEG || b_scene[0] == b_scene[1];
+    if (threadIdx.x < 5) {
+        const int j = threadIdx.x;
+        uint32_t ulo = 0u, uhi = 0u;
+        if (one_scene) {
+            const SceneP& Sb = SEG ? P.seg[b_scene[0]] : P.sc;
+            const DenseGrid& Gb = Sb.dense;
+            const int dy = j == 0 ? 0 : (j == 1 ? 1 : j - 3);
+            const int dz = j < 2 ? 0 : 1;
+            const int64_t off = (int64_t)Gb.dims[0] * ((int64_t)dy + (int64_t)Gb.dims[1] * dz);
+            int64_t a = (int64_t)b_cell[0] + off - 1, b = (int64_t)b_cell[1] + off + 2;
+            const int64_t cells = (int64_t)Gb.cells;
+            a = a < 0 ? 0 : (a > cells ? cells : a);
+            b = b < 0 ? 0 : (b > cells ? cells : b);
+            if (b > a) {
+                ulo = bstart[Sb.cell_base + (uint32_t)a];
+                uhi = bstart[Sb.cell_base + (uint32_t)b];
+                if (j == 0 && ulo <= (uint32_t)t0) ulo = (uint32_t)t0 + 1;
+                if (uhi < ulo) uhi = ulo;
+            }
+        }
+        u_lo[j] = ulo;
+        u_off[j + 1] = uhi - ulo;  // length for now
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0u;
+        u_off[0] = 0u;
+        for (int j = 0; j < 5; ++j) { acc += u_off[j + 1]; u_off[j + 1] = acc; }
+        if (!one_scene || acc > (uint32_t)STAGE_CAP) u_off[5] = 0xffffffffu;  // not staged
+    }
+    __syncthreads();
+    const bool staged = u_off[5] != 0xffffffffu;
+    if (staged) {
+        for (uint32_t i = threadIdx.x; i < u_off[5]; i += TPB) {
+            const int j = (i >= u_off[1]) + (i >= u_off[2]) + (i >= u_off[3]) + (i >= u_off[4]);
+            stage[i] = spos32[u_lo[j] + (i - u_off[j])];
+        }
+    }
+    __syncthreads();
+
+    uint64_t found = 0;
+    uint32_t overflow = 0;
+    if (work) {
+        const float slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
+        const double mult = INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0;
+        const float fmult = (float)mult * (1.0f + 64.0f * FLT_EPSILON);
+        // where each range starts in the stage (inside its union by construction; checked so a
+        // stage read can never stray)
+        uint32_t lb[5];
+        bool in_stage = staged;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            lb[j] = 0u;
+            if (staged && len[j]) {
+                const uint32_t ulen = u_off[j + 1] - u_off[j];
+                if (lo[j] < u_lo[j] || lo[j] - u_lo[j] + len[j] > ulen) in_stage = false;
+                lb[j] = u_off[j] + (lo[j] - u_lo[j]);
+            }
+        }
+        const uint32_t o1 = len[0], o2 = o1 + len[1], o3 = o2 + len[2], o4 = o3 + len[3];
+        const uint32_t total = o4 + len[4];
+        auto pick = [&](uint32_t i, const uint32_t* v) -> uint32_t {
+            return i < o1 ? v[0] + i
+                 : i < o2 ? v[1] + (i - o1)
+                 : i < o3 ? v[2] + (i - o2)
+                 : i < o4 ? v[3] + (i - o3)
+                          : v[4] + (i - o4);
+        };
+#pragma unroll 1
+        for (uint32_t base = 0; base < total; base += BROAD_BATCH) {
+            float4 o[BROAD_BATCH];
+            if (in_stage) {
+#pragma unroll
+                for (int u = 0; u < BROAD_BATCH; ++u)
+                    o[u] = base + u < total ? stage[pick(base + u, lb)] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+            } else {
+#pragma unroll
+                for (int u = 0; u < BROAD_BATCH; ++u)
+                    o[u] = base + u < total ? spos32[pick(base + u, lo)] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
+            }
+            uint32_t hits = 0u;
+#pragma unroll
+            for (int u = 0; u < BROAD_BATCH; ++u) {
+                const float ex = me32.x - o[u].x, ey = me32.y - o[u].y, ez = me32.z - o[u].z;
+                const float d2f = ex * ex + ey * ey + ez * ez;
+                const float thr = (me32.w + o[u].w) * fmult + slack;
+                if (base + u < total && !(d2f > thr * thr)) hits |= 1u << u;
+            }
+#pragma unroll 1
+            while (hits) {
+                const int u = __builtin_ctz(hits);
+                hits &= hits - 1u;
+                const uint32_t k = pick(base + (uint32_t)u, lo);
+                const double4 ov = spos[k];
+                const double ex = me.x - ov.x, ey = me.y - ov.y, ez = me.z - ov.z;
+                const double rs = (me.w + ov.w) * mult;
+                if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) continue;  // sqrt(d2) >= rs
+                if (p == 0xffffffffu) p = sidx[t];
+                const uint32_t q = sidx[k];
+                const bool plo = p < q;
+                const uint32_t il = plo ? p : q, ih = plo ? q : p;
+                const double4 xl = plo ? me : ov, xh = plo ? ov : me;
+                const double rl = INIT ? r0[il] : xl.w, rh = INIT ? r0[ih] : xh.w;
+                if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
+                // when the contact reach is below cs/2 every touching pair passes this test
+                if (S.ref_check &&
+                    !ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
+                    continue;
+                ++found;
+                row_append(tcnt, touch, n, P.K, p, q, overflow);
+                row_append(tcnt, touch, n, P.K, q, p, overflow);
+            }
+        }
+    }
+    if (diag) wave_add(ctr_at(ctr, CT_TOUCH), found);
+    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
 }
 
 // Candidate pairs the dense broad phase examines (slots k > t of the 9 stencil ranges): a
@@ -317,7 +440,7 @@ k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const double
             }
         }
     }
-    wave_add(&ctr[CT_EXAMINED], c);
+    wave_add(ctr_at(ctr, CT_EXAMINED), c);
 }
 
 // Candidate-pair count of the reference's cp_list (kernels.jl:24-37): a diagnostic, evaluated on
@@ -334,7 +457,7 @@ k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __
         for (int wi = 0; wi < 8; ++wi)
             for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += (uint32_t)i < sidx[k];
     }
-    wave_add(&ctr[CT_CAND], c);
+    wave_add(ctr_at(ctr, CT_CAND), c);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -348,135 +471,191 @@ __device__ __forceinline__ bool in_window(const SceneP& S, const double4* __rest
     return ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z));
 }
 
+// Per-particle merge of the sorted old row with the sorted new touching set.  `old_at(m)` and
+// `touch_at(m)` read the m-th old-row word and the m-th touching partner.
+template <bool INIT, typename OldAt, typename TouchAt>
+__device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __restrict__ pos, uint32_t p,
+                                           int64_t n, int K, uint32_t nold, uint32_t nu, int b,
+                                           OldAt old_at, TouchAt touch_at, uint8_t* __restrict__ sel,
+                                           uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1,
+                                           uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1,
+                                           double* __restrict__ st0, double* __restrict__ st1,
+                                           uint64_t& created, uint64_t& deact, uint64_t& entries,
+                                           uint32_t& overflow) {
+    const uint32_t INF = 0xffffffffu;
+    // pass 1: decide the new row
+    uint32_t io = 0, in = 0, nout = 0;
+    bool changed = false;
+    while (io < nold || in < nu) {
+        const uint32_t eo = io < nold ? old_at(io) : INF;
+        const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
+        const uint32_t qn = in < nu ? touch_at(in) : INF;
+        if (qo < qn) {  // only in the old row
+            ++io;
+            if (eo & BROKEN) { changed = true; continue; }
+            if ((eo & BONDED) || INIT) { ++nout; continue; }
+            const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
+            if (in_window(S, pos, lo, hi)) {  // in cp_list and not touching: deactivate
+                changed = true;
+                if (p < qo) ++deact;
+            } else {
+                ++nout;  // not a candidate this step: untouched (kernels.jl:97-119)
+            }
+        } else if (qn < qo) {  // only in the new set: append a zeroed contact
+            ++in; ++nout; changed = true;
+            if (p < qn) ++created;
+        } else {  // in both
+            ++io; ++in; ++nout;
+            if (eo & BROKEN) {  // bond removed at the end of last step, re-created as Hertz
+                changed = true;
+                if (p < qn) ++created;
+            } else if (INIT && !(eo & BONDED)) {
+                changed = true;
+            }
+        }
+    }
+    if ((int)nout > K) overflow = 1;
+    entries = nout;
+    if (!changed || (int)nout > K) return;
+    // pass 2: write the new row into the other buffer
+    const int nb = 1 - b;
+    uint32_t* pqn = nb ? pq1 : pq0;
+    const double* so = b ? st1 : st0;
+    double* sn = nb ? st1 : st0;
+    const int64_t cs = (int64_t)K * n;  // component stride
+    io = 0; in = 0;
+    uint32_t o = 0;
+    while (io < nold || in < nu) {
+        const uint32_t eo = io < nold ? old_at(io) : INF;
+        const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
+        const uint32_t qn = in < nu ? touch_at(in) : INF;
+        const int64_t dst = (int64_t)o * n + p;
+        if (qo < qn) {
+            const int64_t src = (int64_t)io * n + p;
+            ++io;
+            if (eo & BROKEN) continue;
+            if (!(eo & BONDED) && !INIT) {
+                const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
+                if (in_window(S, pos, lo, hi)) continue;
+            }
+            pqn[dst] = eo;
+#pragma unroll
+            for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
+            ++o;
+        } else if (qn < qo) {
+            ++in;
+            pqn[dst] = qn | (INIT ? BONDED : 0u);
+#pragma unroll
+            for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+            ++o;
+        } else {
+            const int64_t src = (int64_t)io * n + p;
+            ++io; ++in;
+            if (eo & BROKEN) {
+                pqn[dst] = qn;
+#pragma unroll
+                for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+            } else if (INIT && !(eo & BONDED)) {
+                pqn[dst] = eo | BONDED;
+#pragma unroll
+                for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+            } else {
+                pqn[dst] = eo;
+#pragma unroll
+                for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
+            }
+            ++o;
+        }
+    }
+    (nb ? rcnt1 : rcnt0)[p] = o;
+    sel[p] = (uint8_t)nb;
+}
+
+// Sort v[0..nt) ascending (insertion sort: rows are short) and drop repeats; returns the count.
+template <typename Get, typename Set>
+__device__ __forceinline__ uint32_t sort_unique(uint32_t nt, Get get, Set set) {
+    for (uint32_t a = 1; a < nt; ++a) {
+        const uint32_t v = get(a);
+        int b = (int)a - 1;
+        while (b >= 0 && get(b) > v) { set(b + 1, get(b)); --b; }
+        set(b + 1, v);
+    }
+    uint32_t nu = 0;
+    for (uint32_t a = 0; a < nt; ++a)
+        if (nu == 0 || get(a) != get(nu - 1)) { set(nu, get(a)); ++nu; }
+    return nu;
+}
+
+// The merge kernel.  Rows of up to MERGE_LDS touching partners and old entries are staged in LDS
+// with batched independent loads; longer rows are merged in place in global memory.
+constexpr int MERGE_LDS = 16;
+
 template <bool INIT, bool SEG>
 __global__ void __launch_bounds__(MERGE_TPB)
 k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ hid,
-        uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ touch, uint8_t* __restrict__ sel,
+        uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch, uint8_t* __restrict__ sel,
         uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1, uint32_t* __restrict__ pq0,
         uint32_t* __restrict__ pq1, double* __restrict__ st0, double* __restrict__ st1,
-        unsigned long long* ctr) {
-    extern __shared__ uint32_t lds[];
+        unsigned long long* ctr, int diag) {
+    __shared__ uint32_t lt[MERGE_LDS * MERGE_TPB];  // touching partners
+    __shared__ uint32_t lw[MERGE_LDS * MERGE_TPB];  // old-row words
     const int64_t p64 = (int64_t)blockIdx.x * MERGE_TPB + threadIdx.x;
     const int64_t n = P.n;
     const int K = P.K;
     uint64_t created = 0, deact = 0, entries = 0;
     uint32_t overflow = 0;
-#define LT(m) lds[(m) * MERGE_TPB + threadIdx.x]
     if (p64 < n) {
         const uint32_t p = (uint32_t)p64;
-        const SceneP& S = scene_of<SEG>(P, p);
         uint32_t nt = tcnt[p];
-        if (nt) tcnt[p] = 0u;
-        if ((int)nt > K) { overflow = 1; nt = K; }
-        for (uint32_t m = 0; m < nt; ++m) LT(m) = touch[(int64_t)m * n + p];
-        // insertion sort + de-duplication of the new touching partners
-        for (uint32_t a = 1; a < nt; ++a) {
-            const uint32_t v = LT(a);
-            int b = (int)a - 1;
-            while (b >= 0 && LT(b) > v) { LT(b + 1) = LT(b); --b; }
-            LT(b + 1) = v;
-        }
-        uint32_t nu = 0;
-        for (uint32_t a = 0; a < nt; ++a)
-            if (nu == 0 || LT(a) != LT(nu - 1)) { LT(nu) = LT(a); ++nu; }
-
         const int b = sel[p];
-        const uint32_t* pqo = b ? pq1 : pq0;
-        const uint32_t nold = (b ? rcnt1 : rcnt0)[p];
-        const uint32_t INF = 0xffffffffu;
-
-        // pass 1: decide the new row
-        uint32_t io = 0, in = 0, nout = 0;
-        bool changed = false;
-        while (io < nold || in < nu) {
-            const uint32_t eo = io < nold ? pqo[(int64_t)io * n + p] : INF;
-            const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
-            const uint32_t qn = in < nu ? LT(in) : INF;
-            if (qo < qn) {  // only in the old row
-                ++io;
-                if (eo & BROKEN) { changed = true; continue; }
-                if ((eo & BONDED) || INIT) { ++nout; continue; }
-                const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
-                if (in_window(S, pos, lo, hi)) {  // in cp_list and not touching: deactivate
-                    changed = true;
-                    if (p < qo) ++deact;
-                } else {
-                    ++nout;  // not a candidate this step: untouched (kernels.jl:97-119)
+        const uint32_t c0 = rcnt0[p], c1 = rcnt1[p];
+        const uint32_t nold = b ? c1 : c0;
+        if (nt | nold) {
+            const SceneP& S = scene_of<SEG>(P, p);
+            if (nt) tcnt[p] = 0u;
+            if ((int)nt > K) { overflow = 1; nt = K; }
+            const uint32_t* pqo = b ? pq1 : pq0;
+            if (nt <= (uint32_t)MERGE_LDS && nold <= (uint32_t)MERGE_LDS) {
+                const int tid = threadIdx.x;
+                for (uint32_t m = 0; m < nt; m += 8) {
+                    uint32_t v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = m + u < nt ? touch[(int64_t)(m + u) * n + p] : 0u;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) if (m + u < nt) lt[(m + u) * MERGE_TPB + tid] = v[u];
                 }
-            } else if (qn < qo) {  // only in the new set: append a zeroed contact
-                ++in; ++nout; changed = true;
-                if (p < qn) ++created;
-            } else {  // in both
-                ++io; ++in; ++nout;
-                if (eo & BROKEN) {  // bond removed at the end of last step, re-created as Hertz
-                    changed = true;
-                    if (p < qn) ++created;
-                } else if (INIT && !(eo & BONDED)) {
-                    changed = true;
+                for (uint32_t m = 0; m < nold; m += 8) {
+                    uint32_t v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) v[u] = m + u < nold ? pqo[(int64_t)(m + u) * n + p] : 0u;
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) if (m + u < nold) lw[(m + u) * MERGE_TPB + tid] = v[u];
                 }
+                const uint32_t nu = sort_unique(
+                    nt, [&](uint32_t m) { return lt[m * MERGE_TPB + tid]; },
+                    [&](uint32_t m, uint32_t v) { lt[m * MERGE_TPB + tid] = v; });
+                merge_rows<INIT>(S, pos, p, n, K, nold, nu, b,
+                                 [&](uint32_t m) { return lw[m * MERGE_TPB + tid]; },
+                                 [&](uint32_t m) { return lt[m * MERGE_TPB + tid]; }, sel, rcnt0,
+                                 rcnt1, pq0, pq1, st0, st1, created, deact, entries, overflow);
+            } else {
+                // long rows: the touching list is sorted in place (it is rebuilt every step)
+                const uint32_t nu = sort_unique(
+                    nt, [&](uint32_t m) { return touch[(int64_t)m * n + p]; },
+                    [&](uint32_t m, uint32_t v) { touch[(int64_t)m * n + p] = v; });
+                merge_rows<INIT>(S, pos, p, n, K, nold, nu, b,
+                                 [&](uint32_t m) { return pqo[(int64_t)m * n + p]; },
+                                 [&](uint32_t m) { return touch[(int64_t)m * n + p]; }, sel, rcnt0,
+                                 rcnt1, pq0, pq1, st0, st1, created, deact, entries, overflow);
             }
-        }
-        if ((int)nout > K) { overflow = 1; }
-        entries = nout;
-        if (changed && (int)nout <= K) {
-            const int nb = 1 - b;
-            uint32_t* pqn = nb ? pq1 : pq0;
-            const double* so = b ? st1 : st0;
-            double* sn = nb ? st1 : st0;
-            const int64_t cs = (int64_t)K * n;  // component stride
-            io = 0; in = 0;
-            uint32_t o = 0;
-            while (io < nold || in < nu) {
-                const uint32_t eo = io < nold ? pqo[(int64_t)io * n + p] : INF;
-                const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
-                const uint32_t qn = in < nu ? LT(in) : INF;
-                const int64_t dst = (int64_t)o * n + p;
-                if (qo < qn) {
-                    const int64_t src = (int64_t)io * n + p;
-                    ++io;
-                    if (eo & BROKEN) continue;
-                    if (!(eo & BONDED) && !INIT) {
-                        const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
-                        if (in_window(S, pos, lo, hi)) continue;
-                    }
-                    pqn[dst] = eo;
-#pragma unroll
-                    for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
-                    ++o;
-                } else if (qn < qo) {
-                    ++in;
-                    pqn[dst] = qn | (INIT ? BONDED : 0u);
-#pragma unroll
-                    for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
-                    ++o;
-                } else {
-                    const int64_t src = (int64_t)io * n + p;
-                    ++io; ++in;
-                    if (eo & BROKEN) {
-                        pqn[dst] = qn;
-#pragma unroll
-                        for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
-                    } else if (INIT && !(eo & BONDED)) {
-                        pqn[dst] = eo | BONDED;
-#pragma unroll
-                        for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
-                    } else {
-                        pqn[dst] = eo;
-#pragma unroll
-                        for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
-                    }
-                    ++o;
-                }
-            }
-            (nb ? rcnt1 : rcnt0)[p] = o;
-            sel[p] = (uint8_t)nb;
         }
     }
-#undef LT
-    wave_add(&ctr[CT_CREATED], created);
-    wave_add(&ctr[CT_DEACT], deact);
-    wave_add(&ctr[CT_ENTRIES], entries);
-    if (overflow) atomicAdd(&ctr[CT_OVERFLOW], 1ull);
+    if (diag) {
+        wave_add(ctr_at(ctr, CT_CREATED), created);
+        wave_add(ctr_at(ctr, CT_DEACT), deact);
+        wave_add(ctr_at(ctr, CT_ENTRIES), entries);
+    }
+    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -686,7 +865,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
                 st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
                 st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
-                st[S_SX * cs + e] = 0.0; st[S_SY * cs + e] = 0.0; st[S_SZ * cs + e] = 0.0;
+                // s of a bond stays at the zero it was created (or restored) with; kernels.jl:230-238 stores zero
                 if (plo) { ++active; ++active_b; }
             } else if (MIXED) {
                 // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance
@@ -843,12 +1022,14 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
         }
     }
-    wave_add(&ctr[CT_BROKEN], broken);
-    wave_add(&ctr[CT_BROKEN_TOTAL], broken);
-    wave_add(&ctr[CT_WALL], nwallc);
-    wave_add(&ctr[CT_FATAL], fatal);
-    wave_add(&ctr[CT_ACTIVE], active);
-    wave_add(&ctr[CT_ACTIVE_BOND], active_b);
+    wave_add(ctr_at(ctr, CT_BROKEN), broken);
+    wave_add(ctr_at(ctr, CT_BROKEN_TOTAL), broken);
+    if (write_forces) {
+        wave_add(ctr_at(ctr, CT_WALL), nwallc);
+        wave_add(ctr_at(ctr, CT_FATAL), fatal);
+        wave_add(ctr_at(ctr, CT_ACTIVE), active);
+        wave_add(ctr_at(ctr, CT_ACTIVE_BOND), active_b);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -950,7 +1131,10 @@ k_write_rows(const Params P, int64_t m, const uint64_t* __restrict__ dst,
     const int64_t d = (int64_t)dst[e];
     const int64_t cs = (int64_t)P.K * P.n;
     pq[d] = word[e];
-    for (int c = 0; c < S_NCOMP; ++c) st[c * cs + d] = state[e * S_NCOMP + c];
+    // a bond's tangential spring is zero in every reachable state (kernels.jl:230-238)
+    const bool bond = (word[e] & BONDED) != 0u;
+    for (int c = 0; c < S_NCOMP; ++c)
+        st[c * cs + d] = (bond && c >= S_SX) ? 0.0 : state[e * S_NCOMP + c];
 }
 
 __global__ void __launch_bounds__(TPB)
@@ -1056,8 +1240,7 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense) {
     timing_end(s);
     if (e != hipSuccess) return e;
     LAUNCH(s, K_SCATTER, hipLaunchKernelGGL(k_scatter, grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
-                                            P, d.pos[c], d.hid, d.rank, d.bstart, d.sidx, d.spos,
-                                            s->precision == DEM_PRECISION_MIXED ? d.spos32 : nullptr));
+                                            P, d.pos[c], d.hid, d.rank, d.bstart, d.sidx, d.spos, d.spos32));
     return hipSuccess;
 }
 
@@ -1068,6 +1251,8 @@ static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) 
 
 template <bool SEG>
 static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
+    // per-step diagnostic counters are only read after the last step of a call
+    const int diag = count_candidates ? 1 : 0;
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
@@ -1082,17 +1267,13 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
                                               d.sidx, d.bstart, d.ctr));
     if (dense) {
         if (init) {
-            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, false, SEG>), g, dim3(TPB), 0,
+            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, SEG>), g, dim3(TPB), 0,
                                                  s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
-                                                 d.tcnt, d.touch, d.ctr));
-        } else if (mixed) {
-            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, true, SEG>), g, dim3(TPB), 0,
-                                                  s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
-                                                  d.tcnt, d.touch, d.ctr));
+                                                 d.tcnt, d.touch, d.ctr, diag));
         } else {
-            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, false, SEG>), g, dim3(TPB), 0,
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, SEG>), g, dim3(TPB), 0,
                                                   s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
-                                                  d.tcnt, d.touch, d.ctr));
+                                                  d.tcnt, d.touch, d.ctr, diag));
         }
     } else if (s->slot_owned) {
         if (init) {
@@ -1121,16 +1302,16 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
                                               d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
                                               d.touch, d.ctr));
     }
-    const size_t lds = (size_t)MERGE_TPB * P.K * sizeof(uint32_t);
+    const size_t lds = 0;
     const dim3 gm = grid_for(P.n, MERGE_TPB);
     if (init) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG>), gm, dim3(MERGE_TPB), lds, s->stream,
                                               P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
-                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
+                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, diag));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG>), gm, dim3(MERGE_TPB), lds, s->stream,
                                               P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
-                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr));
+                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, diag));
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
