@@ -50,8 +50,9 @@ void free_scene(dem_scene* s) {
     if (!s) return;
     DevState& d = s->d;
     void* ptrs[] = {d.pos[0], d.pos[1], d.vel[0], d.vel[1], d.omg[0], d.omg[1], d.quat[0], d.quat[1],
-                    d.acc, d.dw, d.ftot, d.ttot, d.r0, d.wact, d.wshear, d.hid, d.rank, d.bcnt,
-                    d.bstart, d.sidx, d.spos, d.spos32, d.tcnt, d.touch, d.sel, d.rcnt[0], d.rcnt[1],
+                    d.acc, d.dw, d.ftot, d.ttot, d.r0, d.wact, d.wshear, d.hid, d.rank, d.x32, d.bcnt,
+                    d.bstart, d.sidxb[0], d.sidxb[1], d.skey, d.spos, d.spos32, d.tcnt, d.touch, d.sel,
+                    d.rcnt[0], d.rcnt[1],
                     d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, d.scan.status, d.scan.counters,
                     d.scratch_u32, d.scratch2_u32, d.total, d.seg, d.sid, d.seg_broken};
     for (void* p : ptrs)
@@ -287,8 +288,10 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     }
     ALLOC(d.acc, n); ALLOC(d.dw, n); ALLOC(d.ftot, n); ALLOC(d.ttot, n); ALLOC(d.r0, n);
     ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall_max > 0 ? nwall_max : 1) * n);
-    ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.bcnt, nkeys + 1); ALLOC(d.bstart, nkeys + 1);
-    ALLOC(d.sidx, n); ALLOC(d.spos, n); ALLOC(d.spos32, n);
+    ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.x32, n); ALLOC(d.bcnt, nkeys + 1);
+    ALLOC(d.bstart, nkeys + 1); ALLOC(d.sidxb[0], n); ALLOC(d.sidxb[1], n); ALLOC(d.skey, n);
+    ALLOC(d.spos, n); ALLOC(d.spos32, n);
+    d.sidx = d.sidxb[0];
     ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
     ALLOC(d.ctr, CTR_WORDS);
     const int64_t max_tiles = scan_tiles(std::max<int64_t>(nkeys + 1, n + 1));
@@ -317,6 +320,12 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     if (e == hipSuccess) e = hipMemcpy(d.omg[0], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.quat[0], hq.data(), n * sizeof(double4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.r0, s->h_r0, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {  // the first sort starts from file order
+        std::vector<uint32_t> iota(n);
+        for (int64_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
+        e = hipMemcpy(d.sidxb[0], iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d.sidxb[1], iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess && s->batched) {
         std::vector<uint16_t> sid(n);
         for (int k = 0; k < s->nseg; ++k)

@@ -119,13 +119,18 @@ struct DevState {
     double* r0 = nullptr;
     uint32_t* wact = nullptr;               // bit w: particle touched wall w last step
     double4* wshear = nullptr;              // [nwall][n] shear displacement
-    // cell hashing / counting sort
+    // cell hashing / counting sort.  The sort walks the particles in the previous step's sorted
+    // order (sidxb[1 - cur]) and writes this step's order to sidxb[cur]; hid, rank and x32 are
+    // indexed by the previous sorted slot.
     uint32_t* hid = nullptr;     // 0-based bucket
     uint32_t* rank = nullptr;    // slot within the bucket
+    float4* x32 = nullptr;       // FP32 position and radius
     uint32_t* bcnt = nullptr;    // [H + 1] bucket counts (kept zero between steps)
     uint32_t* bstart = nullptr;  // [H + 1] exclusive bucket starts
-    uint32_t* sidx = nullptr;    // sorted slot -> particle
-    double4* spos = nullptr;     // sorted copy of pos
+    uint32_t* sidxb[2] = {nullptr, nullptr};
+    uint32_t* sidx = nullptr;    // sorted slot -> particle (sidxb[cur] after a sort)
+    uint32_t* skey = nullptr;    // sorted slot -> bucket
+    double4* spos = nullptr;     // sorted copy of pos (reference-grid broad phases only)
     float4* spos32 = nullptr;    // sorted FP32 copy (broad-phase pre-test)
     // broad phase output
     uint32_t* tcnt = nullptr;    // [n]

@@ -33,36 +33,67 @@ __device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 
 // ------------------------------------------------------------------------------------------
 // hashing and counting sort
 // ------------------------------------------------------------------------------------------
+// Counting sort, pass 1.  Thread t takes the particle in slot t of the previous step's sorted
+// order, so the lanes of a wave mostly share cells: each run of equal buckets among adjacent lanes
+// claims its slots with one atomic.  Any permutation is a valid starting order; a coherent one
+// only makes the pass cheaper.
 template <bool DENSE, bool SEG>
 __global__ void __launch_bounds__(TPB)
-k_hash(const Params P, const double4* __restrict__ pos, uint32_t* __restrict__ hid,
-       uint32_t* __restrict__ rank, uint32_t* __restrict__ bcnt, unsigned long long* ctr,
-       int reset_counters) {
+k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx_prev,
+       uint32_t* __restrict__ hid, uint32_t* __restrict__ rank, float4* __restrict__ x32,
+       uint32_t* __restrict__ bcnt, unsigned long long* ctr, int reset_counters) {
     if (reset_counters && blockIdx.x == 0)
         for (int i = threadIdx.x; i < CTR_COPIES * CT_NSTEP; i += TPB)
             ctr[(i / CT_NSTEP) * CTR_STRIDE + i % CT_NSTEP] = 0ull;
-    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (p >= P.n) return;
-    const double4 x = pos[p];
-    const SceneP& S = scene_of<SEG>(P, (uint32_t)p);
-    int32_t c[3];
-    const uint32_t b = DENSE ? S.cell_base + dense_cell(S.dense, mk3(x.x, x.y, x.z), c)
-                             : ref_bucket(S.grid, mk3(x.x, x.y, x.z));
-    hid[p] = b;
-    rank[p] = atomicAdd(&bcnt[b], 1u);
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    const bool live = t < P.n;
+    uint32_t b = 0xffffffffu;
+    double4 x = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (live) {
+        const uint32_t p = sidx_prev[t];
+        x = pos[p];
+        const SceneP& S = scene_of<SEG>(P, p);
+        int32_t c[3];
+        b = DENSE ? S.cell_base + dense_cell(S.dense, mk3(x.x, x.y, x.z), c)
+                  : ref_bucket(S.grid, mk3(x.x, x.y, x.z));
+    }
+    // runs of equal buckets among adjacent live lanes (live lanes are a prefix of the wave)
+    const int lane = threadIdx.x & 63;
+    const uint32_t bprev = __shfl_up(b, 1, 64);
+    const bool head = live && (lane == 0 || bprev != b);
+    const uint64_t heads = __ballot(head);
+    const uint64_t nlive = (uint64_t)__popcll(__ballot(live));
+    const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+    const int h = 63 - __clzll(heads & upto);  // this lane's run head (valid for live lanes)
+    const uint64_t above = heads & ~upto;
+    const uint32_t end = above ? (uint32_t)__ffsll((unsigned long long)above) - 1u : (uint32_t)nlive;
+    uint32_t base = 0u;
+    if (head) base = atomicAdd(&bcnt[b], end - (uint32_t)lane);
+    base = __shfl(base, live ? h : lane, 64);
+    if (live) {
+        hid[t] = b;
+        rank[t] = base + (uint32_t)(lane - h);
+        x32[t] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
+    }
 }
 
+// Counting sort, pass 2: scatter slot t of the previous order to its new slot.  The order moves
+// little from step to step, so the writes are close to sequential.
 __global__ void __launch_bounds__(TPB)
-k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ hid,
-          const uint32_t* __restrict__ rank, const uint32_t* __restrict__ bstart,
-          uint32_t* __restrict__ sidx, double4* __restrict__ spos, float4* __restrict__ spos32) {
-    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
-    if (p >= P.n) return;
-    const uint32_t slot = bstart[hid[p]] + rank[p];
-    const double4 x = pos[p];
-    sidx[slot] = (uint32_t)p;
-    spos[slot] = x;
-    spos32[slot] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
+k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx_prev,
+          const uint32_t* __restrict__ hid, const uint32_t* __restrict__ rank,
+          const float4* __restrict__ x32, const uint32_t* __restrict__ bstart,
+          uint32_t* __restrict__ sidx, uint32_t* __restrict__ skey, double4* __restrict__ spos,
+          float4* __restrict__ spos32) {
+    const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (t >= P.n) return;
+    const uint32_t b = hid[t];
+    const uint32_t slot = bstart[b] + rank[t];
+    const uint32_t p = sidx_prev[t];
+    sidx[slot] = p;
+    skey[slot] = b;
+    spos32[slot] = x32[t];
+    if (spos) spos[slot] = pos[p];  // the reference-grid broad phases read FP64 sorted positions
 }
 
 // ------------------------------------------------------------------------------------------
@@ -237,10 +268,11 @@ constexpr int STAGE_CAP = 2048;  // float4 per block (32 KiB)
 
 template <bool INIT, bool SEG>
 __global__ void __launch_bounds__(TPB)
-k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
-              const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
-              const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
-              unsigned long long* ctr, int diag) {
+k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
+              const double4* __restrict__ pos, const float4* __restrict__ spos32,
+              const uint32_t* __restrict__ bstart, const double* __restrict__ r0,
+              uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch, unsigned long long* ctr,
+              int diag) {
     __shared__ float4 stage[STAGE_CAP];
     __shared__ uint32_t u_lo[5], u_off[6];  // union slot ranges and their LDS offsets
     __shared__ uint32_t b_cell[2], b_scene[2];
@@ -249,19 +281,24 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
     const int64_t t = t0 + threadIdx.x;
     const int64_t tl = (t0 + TPB < n ? t0 + TPB : n) - 1;
     const bool live = t < n;
-    // this thread's particle; threads t0 and tl also give the block's first and last cells
-    double4 me = make_double4(0.0, 0.0, 0.0, 0.0);
+    // this thread's particle and cell; threads t0 and tl also give the block's first and last
+    // cells.  The FP64 position is gathered only for candidates that pass the FP32 pre-test.
     float4 me32 = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t p = 0xffffffffu;
+    uint32_t p = 0u, key = 0u;
     if (live) {
-        me = spos[t];
         me32 = spos32[t];
-        if (SEG) p = sidx[t];
+        p = sidx[t];
+        key = skey[t];
     }
-    const SceneP& S = scene_of<SEG>(P, SEG && live ? p : 0u);
+    const SceneP& S = scene_of<SEG>(P, p);
     const DenseGrid& G = S.dense;
-    int32_t c[3] = {0, 0, 0};
-    const uint32_t mycell = live ? dense_cell(G, mk3(me.x, me.y, me.z), c) : 0u;
+    const uint32_t mycell = live ? key - S.cell_base : 0u;
+    int32_t c[3];
+    c[0] = (int32_t)(mycell % (uint32_t)G.dims[0]);
+    c[1] = (int32_t)((mycell / (uint32_t)G.dims[0]) % (uint32_t)G.dims[1]);
+    c[2] = (int32_t)(mycell / ((uint32_t)G.dims[0] * (uint32_t)G.dims[1]));
+    double4 me = make_double4(0.0, 0.0, 0.0, 0.0);
+    bool have_me = false;
     if (threadIdx.x == 0) { b_cell[0] = mycell; b_scene[0] = SEG ? (uint32_t)P.sid[p] : 0u; }
     if (t == tl) { b_cell[1] = mycell; b_scene[1] = SEG ? (uint32_t)P.sid[p] : 0u; }
     // this thread's five slot ranges [lo_j, lo_j + len_j), all above t
@@ -383,12 +420,12 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
                 const int u = __builtin_ctz(hits);
                 hits &= hits - 1u;
                 const uint32_t k = pick(base + (uint32_t)u, lo);
-                const double4 ov = spos[k];
+                const uint32_t q = sidx[k];
+                if (!have_me) { me = pos[p]; have_me = true; }
+                const double4 ov = pos[q];
                 const double ex = me.x - ov.x, ey = me.y - ov.y, ez = me.z - ov.z;
                 const double rs = (me.w + ov.w) * mult;
                 if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) continue;  // sqrt(d2) >= rs
-                if (p == 0xffffffffu) p = sidx[t];
-                const uint32_t q = sidx[k];
                 const bool plo = p < q;
                 const uint32_t il = plo ? p : q, ih = plo ? q : p;
                 const double4 xl = plo ? me : ov, xh = plo ? ov : me;
@@ -413,16 +450,18 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const double4* 
 // loop itself costs more than the loop.)
 template <bool SEG>
 __global__ void __launch_bounds__(TPB)
-k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
+k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
                  const uint32_t* __restrict__ bstart, unsigned long long* ctr) {
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     uint64_t c = 0;
     if (t < P.n) {
         const SceneP& S = scene_of<SEG>(P, SEG ? sidx[t] : 0u);
         const DenseGrid& G = S.dense;
-        const double4 me = spos[t];
+        const uint32_t cell = skey[t] - S.cell_base;
         int32_t cc[3];
-        dense_cell(G, mk3(me.x, me.y, me.z), cc);
+        cc[0] = (int32_t)(cell % (uint32_t)G.dims[0]);
+        cc[1] = (int32_t)((cell / (uint32_t)G.dims[0]) % (uint32_t)G.dims[1]);
+        cc[2] = (int32_t)(cell / ((uint32_t)G.dims[0] * (uint32_t)G.dims[1]));
         const int32_t x0 = cc[0] > 0 ? cc[0] - 1 : 0;
         const int32_t x1 = cc[0] + 1 < G.dims[0] ? cc[0] + 1 : G.dims[0] - 1;
         for (int dz = -1; dz <= 1; ++dz) {
@@ -1225,14 +1264,17 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
+    // this step's order goes to sidxb[cur], starting from the previous step's in sidxb[1 - cur]
+    const uint32_t* prev = d.sidxb[1 - c];
+    d.sidx = d.sidxb[c];
     if (dense) {
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<true, SEG>), grid_for(P.n, TPB), dim3(TPB), 0,
-                                             s->stream, P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
-                                             reset_counters ? 1 : 0));
+                                             s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
+                                             d.bcnt, d.ctr, reset_counters ? 1 : 0));
     } else {  // the reference's hashed grid: single scenes only
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<false, false>), grid_for(P.n, TPB), dim3(TPB), 0,
-                                             s->stream, P, d.pos[c], d.hid, d.rank, d.bcnt, d.ctr,
-                                             reset_counters ? 1 : 0));
+                                             s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
+                                             d.bcnt, d.ctr, reset_counters ? 1 : 0));
     }
     const int64_t nkeys = dense ? (int64_t)P.sort_keys : (int64_t)P.sc.grid.H;
     timing_begin(s, K_SCAN);
@@ -1240,7 +1282,8 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense) {
     timing_end(s);
     if (e != hipSuccess) return e;
     LAUNCH(s, K_SCATTER, hipLaunchKernelGGL(k_scatter, grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
-                                            P, d.pos[c], d.hid, d.rank, d.bstart, d.sidx, d.spos, d.spos32));
+                                            P, d.pos[c], prev, d.hid, d.rank, d.x32, d.bstart, d.sidx,
+                                            d.skey, dense ? nullptr : d.spos, d.spos32));
     return hipSuccess;
 }
 
@@ -1268,12 +1311,12 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     if (dense) {
         if (init) {
             LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, SEG>), g, dim3(TPB), 0,
-                                                 s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
-                                                 d.tcnt, d.touch, d.ctr, diag));
+                                                 s->stream, P, d.sidx, d.skey, d.pos[c], d.spos32,
+                                                 d.bstart, d.r0, d.tcnt, d.touch, d.ctr, diag));
         } else {
             LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, SEG>), g, dim3(TPB), 0,
-                                                  s->stream, P, d.sidx, d.spos, d.spos32, d.bstart, d.r0,
-                                                  d.tcnt, d.touch, d.ctr, diag));
+                                                  s->stream, P, d.sidx, d.skey, d.pos[c], d.spos32,
+                                                  d.bstart, d.r0, d.tcnt, d.touch, d.ctr, diag));
         }
     } else if (s->slot_owned) {
         if (init) {
@@ -1315,7 +1358,7 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
-                                              d.sidx, d.spos, d.bstart, d.ctr));
+                                              d.sidx, d.skey, d.bstart, d.ctr));
     if (count_candidates && dense && !SEG) {
         // the cp_list length needs the reference's bucket sort: rebuild it after the merge
         if ((e = sort_impl<false>(s, false, false)) != hipSuccess) return e;
