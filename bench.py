@@ -69,6 +69,12 @@ def build_workload(name: str, n: int):
                 "0.25-1.0 mm; 0.1x0.1x0.3 m box, floor + 4 side walls; E=1e7 Pa, mu=0.4, "
                 "dt=2e-7 s; no bonds)")
         return cfg, g, w, False, desc
+    if name == "pour_packed":
+        cfg, g, w = scenes.pour_packed(n)
+        desc = (f"{n}-grain pour, contact-heavy state: the pour's seeded grains with radii "
+                "inflated 1.25x after placement and velocities uniform in +-0.05 m/s (Hertz-Mindlin "
+                "only, E=1e7 Pa, mu=0.4, dt=2e-7 s)")
+        return cfg, g, w, False, desc
     if name == "cube18k":
         cfg, g, w = scenes.cube_18k()
         return cfg, g, w, True, "18252-grain bonded cube impact (26x26x27, 20 m/s, dt=5e-7 s)"
@@ -109,6 +115,51 @@ def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
     s.close()
     return {"elapsed": t3 - t2, "elapsed_events": t1 - t0, "kernels": times, "counters": c,
             "derived": d}
+
+
+def precision_error(cfg, g, w, bonded, steps=1000):
+    """Mixed precision against FP64 after `steps` steps from the same initial state (untimed):
+    the largest position difference relative to the largest |k| and to the largest radius.
+    Beside it, the same figures for FP64 against FP64 started from x coordinates moved by one ulp
+    and by 1e-9 relative: the divergence the dynamics alone produce from a small difference (a
+    contact that forms or breaks one step apart changes a grain's path by far more than the
+    difference that decided it)."""
+    import numpy as np
+    from taichi_dem_julia_port_amd.executor import Batch, Scene
+
+    def bump(gr, rel):
+        gr = gr.copy()
+        x = gr["k"][:, 0]
+        gr["k"][:, 0] = np.nextafter(x, np.inf) if rel == 0 else x * (1.0 + rel)
+        return gr
+
+    def run(prec, perturb=None):
+        if g is None:
+            members = [(c, gg if perturb is None else bump(gg, perturb), ww) for (c, gg, ww) in cfg]
+            s = Batch(members, precision=prec)
+        else:
+            s = Scene(cfg, g if perturb is None else bump(g, perturb), w, precision=prec)
+        if bonded:
+            s.initialize()
+        s.step(steps)
+        out = s.read_grains()
+        s.close()
+        return out
+
+    a, b = run("fp64"), run("mixed")
+    c, e = run("fp64", 0.0), run("fp64", 1e-9)
+    scale_k = max(float(np.abs(a["k"]).max()), 1e-300)
+    scale_r = max(float(a["r"].max()), 1e-300)
+
+    def err(x, y):
+        per = np.abs(x["k"] - y["k"]).max(axis=1) if len(x) else np.zeros(1)
+        d = float(per.max())
+        return {"max_abs_m": d, "relative_to_max_position": d / scale_k,
+                "relative_to_max_radius": d / scale_r,
+                "p99_abs_m": float(np.percentile(per, 99)),
+                "grains_above_1e-6_of_max_position": int((per > 1e-6 * scale_k).sum())}
+    return {"steps": steps, "mixed_vs_fp64": err(a, b), "fp64_one_ulp_perturbation": err(a, c),
+            "fp64_1e-9_relative_perturbation": err(a, e)}
 
 
 def one_at_a_time(members, precision, steps, sample=16, seed=0):
@@ -191,7 +242,7 @@ def main():
     ap.add_argument("--steps", type=int, default=300)
     ap.add_argument("--warmup", type=int, default=30)
     ap.add_argument("--workload", default="pour",
-                    choices=["pour", "cube18k", "cube900", "batch256", "serve"])
+                    choices=["pour", "pour_packed", "cube18k", "cube900", "batch256", "serve"])
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--epoch", type=int, default=1000)
@@ -199,7 +250,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--compare-precision", action="store_true",
                     help="also time the other precision and report both")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=0,
+                    help="oracle steps of the CPU baseline (default: 2 for the pour, 20 for the "
+                         "18k cube, all 10000 steps of the 900-grain cube, 5 per batch scene)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step eagerly (no hipGraph replay)")
@@ -294,7 +347,11 @@ def main():
                         graph=not args.no_graph)
         out[f"value_{other}"] = round(n * args.steps / r2["elapsed"], 1)
         out[f"ms_per_step_{other}"] = round(1e3 * r2["elapsed"] / args.steps, 5)
+        out["position_error_after_1000_steps"] = precision_error(cfg, g, w, bonded)
     if not args.no_cpu_baseline:
+        if args.cpu_steps <= 0:
+            args.cpu_steps = {"pour": 2, "pour_packed": 2, "cube18k": 20, "cube900": 10000,
+                              "batch256": 5}[args.workload]
         v, dt = cpu_baseline(cfg, g, w, bonded, args.cpu_steps)
         sample = (f"same workload ({n} grains), {args.cpu_steps} steps after 1 warm-up step, "
                   f"serial FP64 oracle, {dt:.1f} s") if g is not None else (

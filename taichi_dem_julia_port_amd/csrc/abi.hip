@@ -237,11 +237,12 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
         S.beta_pp = (float)pp.beta;
         S.mu_pp = (float)pp.mu;
     }
-    // Dense sort/broad-phase grid: cells of at least cs/2 and of the contact reach, covering the
-    // domain with at most max(4 n, 2^20) cells (batches widen the cells to at most 8 n + 4096).
+    // Dense sort/broad-phase grids covering the domain with at most max(4 n, 2^20) cells
+    // (batches widen the cells to at most 8 n + 4096): the per-step grid has cells of at least
+    // the touching reach 2 max r, the bond pass's of at least the bond reach 2 max r0.
     S.dense.cells = 0;
-    if (want_dense) {
-        double cell = std::max(0.5 * der.cell_size, reach * (1.0 + 1e-6));
+    S.dense_init.cells = 0;
+    auto make_grid = [&](double cell, DenseGrid& G) -> bool {
         const double budget = force_dense ? 8.0 * (double)n + 4096.0
                                           : std::max(4.0 * (double)n, (double)(1 << 20));
         for (int it = 0; it < 64; ++it) {
@@ -254,16 +255,26 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
                 cells *= dd;
             }
             if (cells <= budget && cells < 2.0e9) {
-                for (int a = 0; a < 3; ++a) { S.dense.origin[a] = cfg->domain_min[a]; S.dense.dims[a] = dims[a]; }
-                S.dense.inv_cell = 1.0 / cell;
-                S.dense.cells = (uint32_t)cells;
-                break;
+                for (int a = 0; a < 3; ++a) { G.origin[a] = cfg->domain_min[a]; G.dims[a] = dims[a]; }
+                G.inv_cell = 1.0 / cell;
+                G.inv_dx = 1.0 / (double)dims[0];
+                G.inv_dxy = 1.0 / ((double)dims[0] * (double)dims[1]);
+                G.cells = (uint32_t)cells;
+                return true;
             }
-            if (!force_dense) break;
+            if (!force_dense) return false;
             cell *= std::max(1.05, cbrt(cells / budget));
         }
-        if (force_dense && S.dense.cells == 0)
-            return fail(nullptr, DEM_ERR_INVALID, "scene domain cannot be gridded");
+        return false;
+    };
+    if (want_dense) {
+        const double touch = 2.0 * rmax_r * (1.0 + 1e-6);
+        const bool ok = make_grid(touch, S.dense) && make_grid(reach * (1.0 + 1e-6), S.dense_init);
+        if (!ok) {
+            S.dense.cells = 0;
+            S.dense_init.cells = 0;
+            if (force_dense) return fail(nullptr, DEM_ERR_INVALID, "scene domain cannot be gridded");
+        }
     }
     return DEM_OK;
 }
@@ -294,7 +305,7 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     d.sidx = d.sidxb[0];
     ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
     ALLOC(d.ctr, CTR_WORDS);
-    const int64_t max_tiles = scan_tiles(std::max<int64_t>(nkeys + 1, n + 1));
+    const int64_t max_tiles = scan_tiles_max(std::max<int64_t>(nkeys + 1, n + 1));
     ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2);
     d.scan.max_tiles = max_tiles;
     ALLOC(d.scratch_u32, n + 1); ALLOC(d.scratch2_u32, n + 1); ALLOC(d.total, 1);
@@ -404,7 +415,9 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     P.nseg = 1;
     s->dense = P.sc.dense.cells > 0;
     P.sort_keys = s->dense ? P.sc.dense.cells : (uint32_t)P.sc.grid.H;
-    const int64_t nkeys = std::max<int64_t>((int64_t)P.sc.grid.H, (int64_t)P.sc.dense.cells);
+    P.sort_keys_init = s->dense ? P.sc.dense_init.cells : (uint32_t)P.sc.grid.H;
+    const int64_t nkeys = std::max<int64_t>(
+        (int64_t)P.sc.grid.H, std::max<int64_t>((int64_t)P.sc.dense.cells, (int64_t)P.sc.dense_init.cells));
     if ((rc = allocate_scene(s, g, nwall, nkeys))) {
         free_scene(s);
         return rc;
@@ -457,7 +470,7 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
     s->seg_host.resize(nscenes);
     alloc_host_fields(s, total);
     int64_t off = 0;
-    uint64_t keys = 0;
+    uint64_t keys = 0, keys_init = 0;
     woff = 0;
     for (int k = 0; k < nscenes; ++k) {
         dem_derived_t der{};
@@ -470,13 +483,15 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
         }
         S.cell_base = (uint32_t)keys;
         keys += S.dense.cells;
+        S.cell_base_init = (uint32_t)keys_init;
+        keys_init += S.dense_init.cells;
         if (k == 0) s->der = der;
         s->seg_off[k] = off;
         off += counts[k];
         woff += nwalls[k];
     }
     s->seg_off[nscenes] = off;
-    if (keys >= (1ull << 31)) {
+    if (keys >= (1ull << 31) || keys_init >= (1ull << 31)) {
         free_scene(s);
         return fail(nullptr, DEM_ERR_INVALID, "batch sort grid too large");
     }
@@ -488,7 +503,8 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
     P.sc = s->seg_host[0];
     s->dense = true;
     P.sort_keys = (uint32_t)keys;
-    if ((rc = allocate_scene(s, grains, nwall_max, (int64_t)keys))) {
+    P.sort_keys_init = (uint32_t)keys_init;
+    if ((rc = allocate_scene(s, grains, nwall_max, (int64_t)std::max(keys, keys_init)))) {
         free_scene(s);
         return rc;
     }

@@ -189,9 +189,29 @@ __device__ __forceinline__ bool ref_candidate(const HashGrid& g, d3 lo, d3 hi) {
 struct DenseGrid {
     double origin[3];
     double inv_cell;
+    double inv_dx, inv_dxy;  // 1 / dims[0], 1 / (dims[0] dims[1]) for decoding a cell index
     int32_t dims[3];
     uint32_t cells;
 };
+
+// a / d for a < 2^31 from a correctly rounded reciprocal: the product is within 2^-20 of a / d,
+// so the truncated quotient is off by at most one and the remainder fixes it.
+__device__ __forceinline__ uint32_t div_by(uint32_t a, uint32_t d, double inv) {
+    uint32_t q = (uint32_t)((double)a * inv);
+    const int64_t r = (int64_t)a - (int64_t)q * (int64_t)d;
+    return r < 0 ? q - 1u : (r >= (int64_t)d ? q + 1u : q);
+}
+
+// Cell coordinates of a linear dense-cell index.
+__device__ __forceinline__ void dense_coords(const DenseGrid& g, uint32_t cell, int32_t c[3]) {
+    const uint32_t dx = (uint32_t)g.dims[0], dxy = dx * (uint32_t)g.dims[1];
+    const uint32_t z = div_by(cell, dxy, g.inv_dxy);
+    const uint32_t rem = cell - z * dxy;
+    const uint32_t y = div_by(rem, dx, g.inv_dx);
+    c[0] = (int32_t)(rem - y * dx);
+    c[1] = (int32_t)y;
+    c[2] = (int32_t)z;
+}
 
 __device__ __forceinline__ uint32_t dense_cell(const DenseGrid& g, d3 p, int32_t c[3]) {
     const double r[3] = {p.x, p.y, p.z};

@@ -60,8 +60,11 @@ struct WallP { double n[3]; double d; int32_t mid; int32_t pad; };
 // Everything that may differ between the scenes of a batch.
 struct SceneP {
     HashGrid grid;      // the reference's hashed cell grid (utils.jl:1-16)
-    DenseGrid dense;    // fine dense grid for sorting/broad phase (dense.cells == 0: unused)
-    uint32_t cell_base; // first global sort key of this scene's dense cells
+    DenseGrid dense;    // dense grid of the per-step sort/broad phase, cells >= 2 max r
+                        // (dense.cells == 0: unused)
+    DenseGrid dense_init;    // dense grid of the bond pass, cells >= the bond reach
+    uint32_t cell_base;      // first global sort key of this scene's dense cells
+    uint32_t cell_base_init; // the same for dense_init
     int32_t nwall;
     int32_t bond_init;  // scene takes part in dem_initialize
     int32_t ref_check;  // touching pairs must be checked against the reference window rule
@@ -82,7 +85,7 @@ struct Params {
     int32_t K;            // contact slots per particle
     int32_t nseg;         // scenes (1 for a single scene)
     uint32_t sort_keys;   // keys of the per-step sort (dense cells of all scenes, or H)
-    uint32_t pad;
+    uint32_t sort_keys_init;  // keys of the bond-pass sort
     SceneP sc;
     const SceneP* seg;
     const uint16_t* sid;
@@ -102,6 +105,7 @@ struct ScanWorkspace {
 };
 
 int64_t scan_tiles(int64_t n);
+int64_t scan_tiles_max(int64_t n);  // the most tiles any scan of at most n values takes
 hipError_t launch_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n, bool zero_input,
                        uint64_t* total, hipStream_t stream);
 

@@ -41,7 +41,7 @@ template <bool DENSE, bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx_prev,
        uint32_t* __restrict__ hid, uint32_t* __restrict__ rank, float4* __restrict__ x32,
-       uint32_t* __restrict__ bcnt, unsigned long long* ctr, int reset_counters) {
+       uint32_t* __restrict__ bcnt, unsigned long long* ctr, int reset_counters, int init) {
     if (reset_counters && blockIdx.x == 0)
         for (int i = threadIdx.x; i < CTR_COPIES * CT_NSTEP; i += TPB)
             ctr[(i / CT_NSTEP) * CTR_STRIDE + i % CT_NSTEP] = 0ull;
@@ -54,7 +54,8 @@ k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restri
         x = pos[p];
         const SceneP& S = scene_of<SEG>(P, p);
         int32_t c[3];
-        b = DENSE ? S.cell_base + dense_cell(S.dense, mk3(x.x, x.y, x.z), c)
+        b = DENSE ? (init ? S.cell_base_init + dense_cell(S.dense_init, mk3(x.x, x.y, x.z), c)
+                          : S.cell_base + dense_cell(S.dense, mk3(x.x, x.y, x.z), c))
                   : ref_bucket(S.grid, mk3(x.x, x.y, x.z));
     }
     // runs of equal buckets among adjacent live lanes (live lanes are a prefix of the wave)
@@ -251,20 +252,20 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // x-neighbours of a cell row form one contiguous slot range.  Pairs are owned by the lower sorted
 // slot, and the four stencil rows below the particle's own (dz = -1, or dz = 0 and dy = -1) hold
 // only lower slots, so each thread walks five slot ranges, (dy, dz) = (0,0), (1,0), (-1,1),
-// (0,1), (1,1), flattened into one candidate stream.
+// (0,1), (1,1).
 //
 // A block's 256 consecutive slots lie in cells [cf, cl], so the j-th range of each of its threads
 // lies inside the slots of cells [cf + off_j - 1, cl + off_j + 2) (off_j the linear offset of the
 // row).  The block stages the FP32 copies of those five unions in LDS once and every thread then
-// pre-tests its candidates from LDS, eight per batch: |d|^2 against (r_i + r_j)^2 with a margin
+// pre-tests its candidates from LDS, four per batch: |d|^2 against (r_i + r_j)^2 with a margin
 // that covers the FP32 rounding of coordinates and radii, so no touching pair is rejected.  The
 // few survivors get the FP64 squared-distance test, the reference's exact test
 // (norm(k_i - k_j) < r_i + r_j, kernels.jl:103; r0 at t = 0, kernels.jl:72) and the reference's
 // candidate rule (hi's bucket in lo's window, kernels.jl:24-58), so the contact set is the
 // reference's in either precision.  Blocks whose unions exceed the stage, or that straddle two
 // scenes of a batch, read the FP32 copies from global memory instead.
-constexpr int BROAD_BATCH = 8;
-constexpr int STAGE_CAP = 2048;  // float4 per block (32 KiB)
+constexpr int BROAD_BATCH = 4;
+constexpr int STAGE_CAP = 2048;  // float4 per block (32 KiB); a power of two
 
 template <bool INIT, bool SEG>
 __global__ void __launch_bounds__(TPB)
@@ -291,12 +292,11 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
         key = skey[t];
     }
     const SceneP& S = scene_of<SEG>(P, p);
-    const DenseGrid& G = S.dense;
-    const uint32_t mycell = live ? key - S.cell_base : 0u;
+    const DenseGrid& G = INIT ? S.dense_init : S.dense;
+    const uint32_t cbase = INIT ? S.cell_base_init : S.cell_base;
+    const uint32_t mycell = live ? key - cbase : 0u;
     int32_t c[3];
-    c[0] = (int32_t)(mycell % (uint32_t)G.dims[0]);
-    c[1] = (int32_t)((mycell / (uint32_t)G.dims[0]) % (uint32_t)G.dims[1]);
-    c[2] = (int32_t)(mycell / ((uint32_t)G.dims[0] * (uint32_t)G.dims[1]));
+    dense_coords(G, mycell, c);
     double4 me = make_double4(0.0, 0.0, 0.0, 0.0);
     bool have_me = false;
     if (threadIdx.x == 0) { b_cell[0] = mycell; b_scene[0] = SEG ? (uint32_t)P.sid[p] : 0u; }
@@ -314,7 +314,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
             const int32_t y = c[1] + dy, z = c[2] + dz;
             lo[j] = 0u; len[j] = 0u;
             if (work && y >= 0 && y < G.dims[1] && z < G.dims[2]) {
-                const uint32_t row = S.cell_base +
+                const uint32_t row = cbase +
                                      (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
                 uint32_t a = bstart[row + (uint32_t)x0];
                 const uint32_t e = bstart[row + (uint32_t)x1 + 1];
@@ -332,7 +332,8 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
         uint32_t ulo = 0u, uhi = 0u;
         if (one_scene) {
             const SceneP& Sb = SEG ? P.seg[b_scene[0]] : P.sc;
-            const DenseGrid& Gb = Sb.dense;
+            const DenseGrid& Gb = INIT ? Sb.dense_init : Sb.dense;
+            const uint32_t bbase = INIT ? Sb.cell_base_init : Sb.cell_base;
             const int dy = j == 0 ? 0 : (j == 1 ? 1 : j - 3);
             const int dz = j < 2 ? 0 : 1;
             const int64_t off = (int64_t)Gb.dims[0] * ((int64_t)dy + (int64_t)Gb.dims[1] * dz);
@@ -341,8 +342,8 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
             a = a < 0 ? 0 : (a > cells ? cells : a);
             b = b < 0 ? 0 : (b > cells ? cells : b);
             if (b > a) {
-                ulo = bstart[Sb.cell_base + (uint32_t)a];
-                uhi = bstart[Sb.cell_base + (uint32_t)b];
+                ulo = bstart[bbase + (uint32_t)a];
+                uhi = bstart[bbase + (uint32_t)b];
                 if (j == 0 && ulo <= (uint32_t)t0) ulo = (uint32_t)t0 + 1;
                 if (uhi < ulo) uhi = ulo;
             }
@@ -386,58 +387,60 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
                 lb[j] = u_off[j] + (lo[j] - u_lo[j]);
             }
         }
-        const uint32_t o1 = len[0], o2 = o1 + len[1], o3 = o2 + len[2], o4 = o3 + len[3];
-        const uint32_t total = o4 + len[4];
-        auto pick = [&](uint32_t i, const uint32_t* v) -> uint32_t {
-            return i < o1 ? v[0] + i
-                 : i < o2 ? v[1] + (i - o1)
-                 : i < o3 ? v[2] + (i - o2)
-                 : i < o4 ? v[3] + (i - o3)
-                          : v[4] + (i - o4);
+        // FP32 pre-test |d|^2 <= thr^2, thr = (r_i + r_j) * fmult + slack, evaluated with fused
+        // multiply-adds; the 64-ulp margin in fmult covers their rounding
+        const float c0 = fmaf(me32.w, fmult, slack);
+        auto near = [&](const float4 o) -> bool {
+            const float ex = me32.x - o.x, ey = me32.y - o.y, ez = me32.z - o.z;
+            const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+            const float thr = fmaf(o.w, fmult, c0);
+            return !(fmaf(-thr, thr, d2) > 0.0f);
         };
 #pragma unroll 1
-        for (uint32_t base = 0; base < total; base += BROAD_BATCH) {
-            float4 o[BROAD_BATCH];
-            if (in_stage) {
-#pragma unroll
-                for (int u = 0; u < BROAD_BATCH; ++u)
-                    o[u] = base + u < total ? stage[pick(base + u, lb)] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
-            } else {
-#pragma unroll
-                for (int u = 0; u < BROAD_BATCH; ++u)
-                    o[u] = base + u < total ? spos32[pick(base + u, lo)] : make_float4(1e30f, 1e30f, 1e30f, 0.f);
-            }
-            uint32_t hits = 0u;
-#pragma unroll
-            for (int u = 0; u < BROAD_BATCH; ++u) {
-                const float ex = me32.x - o[u].x, ey = me32.y - o[u].y, ez = me32.z - o[u].z;
-                const float d2f = ex * ex + ey * ey + ez * ez;
-                const float thr = (me32.w + o[u].w) * fmult + slack;
-                if (base + u < total && !(d2f > thr * thr)) hits |= 1u << u;
-            }
+        for (int j = 0; j < 5; ++j) {
+            // range j without indexing the arrays by a runtime value (which would spill them)
+            const uint32_t rlo = j == 0 ? lo[0] : j == 1 ? lo[1] : j == 2 ? lo[2] : j == 3 ? lo[3] : lo[4];
+            const uint32_t rlen = j == 0 ? len[0] : j == 1 ? len[1] : j == 2 ? len[2] : j == 3 ? len[3] : len[4];
+            const uint32_t rlb = j == 0 ? lb[0] : j == 1 ? lb[1] : j == 2 ? lb[2] : j == 3 ? lb[3] : lb[4];
 #pragma unroll 1
-            while (hits) {
-                const int u = __builtin_ctz(hits);
-                hits &= hits - 1u;
-                const uint32_t k = pick(base + (uint32_t)u, lo);
-                const uint32_t q = sidx[k];
-                if (!have_me) { me = pos[p]; have_me = true; }
-                const double4 ov = pos[q];
-                const double ex = me.x - ov.x, ey = me.y - ov.y, ez = me.z - ov.z;
-                const double rs = (me.w + ov.w) * mult;
-                if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) continue;  // sqrt(d2) >= rs
-                const bool plo = p < q;
-                const uint32_t il = plo ? p : q, ih = plo ? q : p;
-                const double4 xl = plo ? me : ov, xh = plo ? ov : me;
-                const double rl = INIT ? r0[il] : xl.w, rh = INIT ? r0[ih] : xh.w;
-                if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
-                // when the contact reach is below cs/2 every touching pair passes this test
-                if (S.ref_check &&
-                    !ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
-                    continue;
-                ++found;
-                row_append(tcnt, touch, n, P.K, p, q, overflow);
-                row_append(tcnt, touch, n, P.K, q, p, overflow);
+            for (uint32_t i = 0; i < rlen; i += BROAD_BATCH) {
+                float4 o[BROAD_BATCH];
+                if (in_stage) {
+                    // reads past the range stay inside the stage and are masked below
+#pragma unroll
+                    for (int u = 0; u < BROAD_BATCH; ++u) o[u] = stage[(rlb + i + u) & (STAGE_CAP - 1)];
+                } else {
+#pragma unroll
+                    for (int u = 0; u < BROAD_BATCH; ++u)
+                        o[u] = i + u < rlen ? spos32[rlo + i + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+                bool any = false;
+#pragma unroll
+                for (int u = 0; u < BROAD_BATCH; ++u) any |= near(o[u]) && (u == 0 || i + u < rlen);
+                if (!any) continue;
+#pragma unroll 1
+                for (uint32_t u = 0; u < BROAD_BATCH && i + u < rlen; ++u) {
+                    const uint32_t k = rlo + i + u;
+                    if (!near(in_stage ? stage[rlb + i + u] : spos32[k])) continue;
+                    const uint32_t q = sidx[k];
+                    if (!have_me) { me = pos[p]; have_me = true; }
+                    const double4 ov = pos[q];
+                    const double ex = me.x - ov.x, ey = me.y - ov.y, ez = me.z - ov.z;
+                    const double rs = (me.w + ov.w) * mult;
+                    if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) continue;  // sqrt(d2) >= rs
+                    const bool plo = p < q;
+                    const uint32_t il = plo ? p : q, ih = plo ? q : p;
+                    const double4 xl = plo ? me : ov, xh = plo ? ov : me;
+                    const double rl = INIT ? r0[il] : xl.w, rh = INIT ? r0[ih] : xh.w;
+                    if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
+                    // when the contact reach is below cs/2 every touching pair passes this test
+                    if (S.ref_check &&
+                        !ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
+                        continue;
+                    ++found;
+                    row_append(tcnt, touch, n, P.K, p, q, overflow);
+                    row_append(tcnt, touch, n, P.K, q, p, overflow);
+                }
             }
         }
     }
@@ -457,11 +460,8 @@ k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32
     if (t < P.n) {
         const SceneP& S = scene_of<SEG>(P, SEG ? sidx[t] : 0u);
         const DenseGrid& G = S.dense;
-        const uint32_t cell = skey[t] - S.cell_base;
         int32_t cc[3];
-        cc[0] = (int32_t)(cell % (uint32_t)G.dims[0]);
-        cc[1] = (int32_t)((cell / (uint32_t)G.dims[0]) % (uint32_t)G.dims[1]);
-        cc[2] = (int32_t)(cell / ((uint32_t)G.dims[0] * (uint32_t)G.dims[1]));
+        dense_coords(G, skey[t] - S.cell_base, cc);
         const int32_t x0 = cc[0] > 0 ? cc[0] - 1 : 0;
         const int32_t x1 = cc[0] + 1 < G.dims[0] ? cc[0] + 1 : G.dims[0] - 1;
         for (int dz = -1; dz <= 1; ++dz) {
@@ -906,19 +906,27 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
                 // s of a bond stays at the zero it was created (or restored) with; kernels.jl:230-238 stores zero
                 if (plo) { ++active; ++active_b; }
-            } else if (MIXED) {
+            } else {
+              if (MIXED) {
+                // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance.
+                // Contacts it cannot decide like FP64 fall through to the FP64 law below.
                 // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance
                 const double gap = Li - ri - rj;
                 const double dn = fabs(gap);
                 const f3 d32 = tof(kj - ki);
                 const f3 a32 = fscl(d32, 1.0f / (float)Li);
+                // near a = -x the reference's frame is ill-conditioned (its rotation about the
+                // normal follows the normal's tiny transverse components): FP64 there
+                bool fp64 = a32.x < -0.99f;
                 const fm3 R32 = contact_frame_f32(a32, 1e-6f);
                 const float lever = (float)(ri - dn);
                 const f3 kip = fscl(a32, lever);
                 const f3 kjp = fsub(kip, d32);
-                const f3 vci = fadd(fcross(tof(wi), kip), tof(vi));
-                const f3 vcj = fadd(fcross(tof(wj), kjp), tof(vj));
-                const f3 vc = fmul(R32, fsub(vcj, vci));
+                // (w_j x k_j' + v_j) - (w_i x k_i' + v_i) with v_j - v_i taken in FP64: grains in
+                // contact share most of their velocity, and subtracting FP32 copies of two fast
+                // grains' velocities would lose the small relative part
+                const f3 dv = tof(vj - vi);
+                const f3 vc = fmul(R32, fadd(fsub(fcross(tof(wj), kjp), fcross(tof(wi), kip)), dv));
                 const float Rs = 1.0f / (1.0f / (float)ri + 1.0f / (float)rj);
                 const float ms = 1.0f / (1.0f / (float)mi + 1.0f / (float)mj);
                 const float srd = sqrtf(Rs * (float)dn);
@@ -932,8 +940,15 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const float F1 = -kn * (float)gap - gn * vc.x;
                 const f3 Ft = fscl(tof(sv), -kt);
                 const float nFt = fnorm(Ft);
+                // a slip decision within FP32 rounding of the threshold is taken in FP64: the
+                // reference's slip rule reverses the stored spring, so a flipped decision is an
+                // O(1) change of the tangential force
+                const float thr = S.mu_pp * F1;
+                fp64 = fp64 || fabsf(nFt - thr) <= 1e-4f * fmaxf(fabsf(nFt), fabsf(thr));
                 float F2, F3;
-                if (nFt >= S.mu_pp * F1) {
+                if (fp64) {
+                    F2 = F3 = 0.0f;
+                } else if (nFt >= thr) {
                     const float ratio = S.mu_pp * F1 / nFt;
                     F2 = ratio * Ft.y;
                     F3 = ratio * Ft.z;
@@ -942,6 +957,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                     F2 = Ft.y - gt * vc.y;
                     F3 = Ft.z - gt * vc.z;
                 }
+                if (!fp64) {
                 const f3 Fl = mkf(F1, F2, F3);
                 const f3 Fg = fmul_t(R32, mkf(-F1, -F2, -F3));
                 if (plo) { FH.add(Fg); TH.add(fcross(kip, Fg)); }
@@ -954,7 +970,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                     ++active;
                     if (gap > 0 || gn < 0.0f || gt < 0.0f) ++fatal;
                 }
-            } else {
+                continue;
+                }
+              }
+              {
                 // Hertz-Mindlin (kernels.jl:240-313)
                 const m3 R = contact_frame(normalize(kj - ki), S.tol);
                 const double gap = Li - ri - rj;
@@ -981,6 +1000,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                     ++active;
                     if (gap > 0 || h.fatal) ++fatal;
                 }
+              }
             }
         }
         if (MIXED) {
@@ -1260,7 +1280,7 @@ void timing_end(dem_scene* s) {
     } while (0)
 
 template <bool SEG>
-static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense) {
+static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool init = false) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
@@ -1270,13 +1290,13 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense) {
     if (dense) {
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<true, SEG>), grid_for(P.n, TPB), dim3(TPB), 0,
                                              s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
-                                             d.bcnt, d.ctr, reset_counters ? 1 : 0));
+                                             d.bcnt, d.ctr, reset_counters ? 1 : 0, init ? 1 : 0));
     } else {  // the reference's hashed grid: single scenes only
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<false, false>), grid_for(P.n, TPB), dim3(TPB), 0,
                                              s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
-                                             d.bcnt, d.ctr, reset_counters ? 1 : 0));
+                                             d.bcnt, d.ctr, reset_counters ? 1 : 0, 0));
     }
-    const int64_t nkeys = dense ? (int64_t)P.sort_keys : (int64_t)P.sc.grid.H;
+    const int64_t nkeys = dense ? (int64_t)(init ? P.sort_keys_init : P.sort_keys) : (int64_t)P.sc.grid.H;
     timing_begin(s, K_SCAN);
     hipError_t e = launch_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, true, nullptr, s->stream);
     timing_end(s);
@@ -1301,7 +1321,7 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     const int c = s->cur;
     const bool dense = s->dense;
     if (SEG && !dense) return hipErrorInvalidValue;
-    hipError_t e = sort_impl<SEG>(s, true, dense);
+    hipError_t e = sort_impl<SEG>(s, true, dense, init);
     if (e != hipSuccess) return e;
     const bool mixed = s->precision == DEM_PRECISION_MIXED && !init;
     const dim3 g = grid_for(P.n, TPB);

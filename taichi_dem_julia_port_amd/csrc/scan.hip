@@ -2,18 +2,25 @@
 //
 // Replaces CUDA.jl accumulate!(+, ...) of the reference (src/executor.jl:50,62).  One launch per
 // scan: each workgroup claims a tile id from a counter (so a tile only ever waits on tiles that
-// were claimed earlier and are therefore running), publishes its aggregate, then one wavefront
-// looks back over up to 64 predecessors per round.  The last workgroup to finish clears the tile
-// status words and counters, so no separate reset launch (and no memset node) is needed.
+// were claimed earlier and are therefore running), publishes its aggregate, then the whole
+// workgroup looks back over up to 256 predecessors per round (every tile is resident at once on a
+// large scan, so the look-back depth, not bandwidth, sets the time).  The last workgroup to finish
+// clears the tile status words and counters, so no separate reset launch (and no memset node) is
+// needed.
 #include "dem_internal.hpp"
+
+#include <algorithm>
 
 namespace dem {
 
 namespace {
 
 constexpr int SCAN_THREADS = 256;
-constexpr int SCAN_ITEMS = 16;
-constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;  // 4096 values per tile
+// Values per thread: short tiles keep a small scan's per-tile chain short; long tiles keep a large
+// scan's look-back shallow.
+constexpr int SCAN_ITEMS_SMALL = 8;   // 2048 values per tile
+constexpr int SCAN_ITEMS_LARGE = 32;  // 8192 values per tile
+constexpr int64_t SCAN_LARGE_FROM = (int64_t)1 << 21;
 constexpr uint64_t FLAG_AGG = 1ull << 32;
 constexpr uint64_t FLAG_INC = 2ull << 32;
 
@@ -34,14 +41,15 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
     return v;
 }
 
+template <int SCAN_ITEMS>
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_scan_exclusive(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n, int zero_input,
                  uint64_t* __restrict__ status, uint32_t* __restrict__ counters,
                  uint64_t* __restrict__ total, uint32_t ntiles) {
+    constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
     __shared__ uint32_t s_vals[SCAN_TILE + SCAN_TILE / 32];
     __shared__ uint32_t s_wave[SCAN_THREADS / 64];
     __shared__ uint32_t s_tile;
-    __shared__ uint32_t s_prefix;
     __shared__ uint32_t s_last;
 
     const int tid = threadIdx.x;
@@ -52,28 +60,34 @@ k_scan_exclusive(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t 
     const uint32_t tile = s_tile;
     const int64_t base = (int64_t)tile * SCAN_TILE;
 
-    // Coalesced load into LDS (padded by one word every 32 to avoid bank conflicts).
+    // Coalesced load into LDS (padded by one word every 32 to avoid bank conflicts).  All loads
+    // are issued before the input is cleared: a store to an address waits for the load before it.
+    uint32_t v[SCAN_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t g = base + it * SCAN_THREADS + tid;
+        v[it] = g < n ? in[g] : 0u;
+    }
+    if (zero_input) {
+#pragma unroll
+        for (int it = 0; it < SCAN_ITEMS; ++it) {
+            const int64_t g = base + it * SCAN_THREADS + tid;
+            if (g < n) in[g] = 0u;
+        }
+    }
 #pragma unroll
     for (int it = 0; it < SCAN_ITEMS; ++it) {
         const int idx = it * SCAN_THREADS + tid;
-        const int64_t g = base + idx;
-        uint32_t v = 0;
-        if (g < n) {
-            v = in[g];
-            if (zero_input) in[g] = 0u;
-        }
-        s_vals[idx + (idx >> 5)] = v;
+        s_vals[idx + (idx >> 5)] = v[it];
     }
     __syncthreads();
 
-    // Each thread owns SCAN_ITEMS consecutive values.
-    uint32_t local[SCAN_ITEMS];
+    // Each thread owns SCAN_ITEMS consecutive values (re-read from LDS after the look-back).
     uint32_t sum = 0;
 #pragma unroll
     for (int it = 0; it < SCAN_ITEMS; ++it) {
         const int idx = tid * SCAN_ITEMS + it;
-        local[it] = s_vals[idx + (idx >> 5)];
-        sum += local[it];
+        sum += s_vals[idx + (idx >> 5)];
     }
     const uint32_t incl = wave_incl_scan(sum);
     if (lane == 63) s_wave[wave] = incl;
@@ -87,46 +101,50 @@ k_scan_exclusive(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t 
     }
     const uint32_t thread_excl = wave_off + incl - sum;
 
-    // Publish and look back.
-    if (wave == 0) {
-        uint32_t prefix = 0;
-        if (tile == 0) {
-            if (lane == 0) store_status(&status[0], FLAG_INC | (uint64_t)agg);
-        } else {
-            if (lane == 0) store_status(&status[tile], FLAG_AGG | (uint64_t)agg);
-            int64_t pred = (int64_t)tile - 1;
-            while (true) {
-                const int64_t t = pred - lane;
-                uint64_t st = t >= 0 ? load_status(&status[t]) : (FLAG_INC | 0ull);
-                // Wait until every predecessor in the window has published something.
-                while (__any((st >> 32) == 0)) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if ((st >> 32) == 0) st = load_status(&status[t]);
-                }
-                const uint64_t inc_mask = __ballot((st >> 32) == 2);
-                uint32_t v = (uint32_t)st;
-                if (inc_mask) {
-                    const int first = __ffsll((unsigned long long)inc_mask) - 1;  // nearest inclusive
-                    if (lane > first) v = 0;
-                    prefix += (uint32_t)wave_sum_u64(v);
-                    break;
-                }
-                prefix += (uint32_t)wave_sum_u64(v);
-                pred -= 64;
+    // Publish, then look back over 256 predecessors per round with the whole workgroup.
+    __shared__ uint32_t s_first;
+    __shared__ uint32_t s_part[SCAN_THREADS / 64];
+    uint32_t prefix = 0;
+    if (tile == 0) {
+        if (tid == 0) store_status(&status[0], FLAG_INC | (uint64_t)agg);
+    } else {
+        if (tid == 0) store_status(&status[tile], FLAG_AGG | (uint64_t)agg);
+        int64_t pred = (int64_t)tile - 1;
+        while (true) {
+            const int64_t t = pred - tid;
+            uint64_t st = t >= 0 ? load_status(&status[t]) : (FLAG_INC | 0ull);
+            // every predecessor was claimed before this tile and publishes without waiting
+            while ((st >> 32) == 0) {
+                __builtin_amdgcn_s_sleep(1);
+                st = load_status(&status[t]);
             }
-            if (lane == 0) store_status(&status[tile], FLAG_INC | (uint64_t)(uint32_t)(prefix + agg));
+            if (tid == 0) s_first = SCAN_THREADS;
+            __syncthreads();
+            if ((st >> 32) == 2) atomicMin(&s_first, (uint32_t)tid);  // nearest inclusive prefix
+            __syncthreads();
+            const uint32_t first = s_first;
+            const uint32_t v = (uint32_t)tid <= first ? (uint32_t)st : 0u;
+            const uint32_t ws = (uint32_t)wave_sum_u64(v);
+            if (lane == 0) s_part[wave] = ws;
+            __syncthreads();
+            uint32_t round = 0;
+#pragma unroll
+            for (int w = 0; w < SCAN_THREADS / 64; ++w) round += s_part[w];
+            prefix += round;
+            if (first < (uint32_t)SCAN_THREADS) break;
+            pred -= SCAN_THREADS;
+            __syncthreads();  // s_first and s_part are rewritten by the next round
         }
-        if (lane == 0) s_prefix = prefix;
+        if (tid == 0) store_status(&status[tile], FLAG_INC | (uint64_t)(uint32_t)(prefix + agg));
     }
-    __syncthreads();
-    const uint32_t prefix = s_prefix;
 
     uint32_t run = prefix + thread_excl;
 #pragma unroll
     for (int it = 0; it < SCAN_ITEMS; ++it) {
         const int idx = tid * SCAN_ITEMS + it;
+        const uint32_t x = s_vals[idx + (idx >> 5)];
         s_vals[idx + (idx >> 5)] = run;
-        run += local[it];
+        run += x;
     }
     __syncthreads();
 #pragma unroll
@@ -156,15 +174,29 @@ k_scan_exclusive(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t 
 
 }  // namespace
 
-int64_t scan_tiles(int64_t n) { return (n + SCAN_TILE - 1) / SCAN_TILE; }
+static int64_t tile_of(int64_t n) {
+    return (int64_t)SCAN_THREADS * (n >= SCAN_LARGE_FROM ? SCAN_ITEMS_LARGE : SCAN_ITEMS_SMALL);
+}
+
+int64_t scan_tiles(int64_t n) { return (n + tile_of(n) - 1) / tile_of(n); }
+
+int64_t scan_tiles_max(int64_t n) {
+    return std::max(scan_tiles(std::min(n, SCAN_LARGE_FROM - 1)), scan_tiles(n));
+}
 
 hipError_t launch_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n, bool zero_input,
                        uint64_t* total, hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const int64_t ntiles = scan_tiles(n);
     if (ntiles > ws.max_tiles) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scan_exclusive, dim3((uint32_t)ntiles), dim3(SCAN_THREADS), 0, stream, in,
-                       out, n, zero_input ? 1 : 0, ws.status, ws.counters, total, (uint32_t)ntiles);
+    if (n >= SCAN_LARGE_FROM)
+        hipLaunchKernelGGL(k_scan_exclusive<SCAN_ITEMS_LARGE>, dim3((uint32_t)ntiles), dim3(SCAN_THREADS),
+                           0, stream, in, out, n, zero_input ? 1 : 0, ws.status, ws.counters, total,
+                           (uint32_t)ntiles);
+    else
+        hipLaunchKernelGGL(k_scan_exclusive<SCAN_ITEMS_SMALL>, dim3((uint32_t)ntiles), dim3(SCAN_THREADS),
+                           0, stream, in, out, n, zero_input ? 1 : 0, ws.status, ws.counters, total,
+                           (uint32_t)ntiles);
     return hipGetLastError();
 }
 
