@@ -258,14 +258,18 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // lies inside the slots of cells [cf + off_j - 1, cl + off_j + 2) (off_j the linear offset of the
 // row).  The block stages the FP32 copies of those five unions in LDS once and every thread then
 // pre-tests its candidates from LDS, four per batch: |d|^2 against (r_i + r_j)^2 with a margin
-// that covers the FP32 rounding of coordinates and radii, so no touching pair is rejected.  The
-// few survivors get the FP64 squared-distance test, the reference's exact test
+// that covers the FP32 rounding of coordinates and radii, so no touching pair is rejected.
+// Survivors are queued per wave and, once the scan is done, confirmed by all 64 lanes together
+// (a survivor is rare per candidate but present in nearly every batch of a dense packing, so
+// confirming in place would run the FP64 path at a fraction of the lanes).  They get the FP64
+// squared-distance test, the reference's exact test
 // (norm(k_i - k_j) < r_i + r_j, kernels.jl:103; r0 at t = 0, kernels.jl:72) and the reference's
 // candidate rule (hi's bucket in lo's window, kernels.jl:24-58), so the contact set is the
 // reference's in either precision.  Blocks whose unions exceed the stage, or that straddle two
 // scenes of a batch, read the FP32 copies from global memory instead.
 constexpr int BROAD_BATCH = 4;
 constexpr int STAGE_CAP = 2048;  // float4 per block (32 KiB); a power of two
+constexpr int HIT_QUEUE = 256;   // pre-test survivors queued per wave
 
 template <bool INIT, bool SEG>
 __global__ void __launch_bounds__(TPB)
@@ -277,6 +281,12 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     __shared__ float4 stage[STAGE_CAP];
     __shared__ uint32_t u_lo[5], u_off[6];  // union slot ranges and their LDS offsets
     __shared__ uint32_t b_cell[2], b_scene[2];
+    // per-wave queue of candidates that passed the FP32 pre-test: slot k and the owner's lane
+    __shared__ uint32_t hq_k[TPB / 64][HIT_QUEUE];
+    __shared__ uint8_t hq_l[TPB / 64][HIT_QUEUE];
+    __shared__ uint32_t hq_n[TPB / 64];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) hq_n[wv] = 0u;
     const int64_t n = P.n;
     const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TPB;
     const int64_t t = t0 + threadIdx.x;
@@ -297,8 +307,6 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     const uint32_t mycell = live ? key - cbase : 0u;
     int32_t c[3];
     dense_coords(G, mycell, c);
-    double4 me = make_double4(0.0, 0.0, 0.0, 0.0);
-    bool have_me = false;
     if (threadIdx.x == 0) { b_cell[0] = mycell; b_scene[0] = SEG ? (uint32_t)P.sid[p] : 0u; }
     if (t == tl) { b_cell[1] = mycell; b_scene[1] = SEG ? (uint32_t)P.sid[p] : 0u; }
     // this thread's five slot ranges [lo_j, lo_j + len_j), all above t
@@ -370,9 +378,29 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
 
     uint64_t found = 0;
     uint32_t overflow = 0;
+    const double mult = INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0;
+    // FP64 tests of a pre-test survivor (slot k) of the particle in slot to (index po), and the
+    // row appends of a touching pair
+    auto confirm = [&](const SceneP& Sc, uint32_t po, uint32_t k) {
+        const uint32_t q = sidx[k];
+        const double4 mo = pos[po], ov = pos[q];
+        const double ex = mo.x - ov.x, ey = mo.y - ov.y, ez = mo.z - ov.z;
+        const double rs = (mo.w + ov.w) * (INIT ? Sc.bond_mult * (1.0 + 1e-9) : 1.0);
+        if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) return;  // sqrt(d2) >= rs
+        const bool plo = po < q;
+        const uint32_t il = plo ? po : q, ih = plo ? q : po;
+        const double4 xl = plo ? mo : ov, xh = plo ? ov : mo;
+        const double rl = INIT ? r0[il] : xl.w, rh = INIT ? r0[ih] : xh.w;
+        if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) return;
+        // when the contact reach is below cs/2 every touching pair passes this test
+        if (Sc.ref_check && !ref_candidate(Sc.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
+            return;
+        ++found;
+        row_append(tcnt, touch, n, P.K, po, q, overflow);
+        row_append(tcnt, touch, n, P.K, q, po, overflow);
+    };
     if (work) {
         const float slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
-        const double mult = INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0;
         const float fmult = (float)mult * (1.0f + 64.0f * FLT_EPSILON);
         // where each range starts in the stage (inside its union by construction; checked so a
         // stage read can never stray)
@@ -414,34 +442,45 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
                     for (int u = 0; u < BROAD_BATCH; ++u)
                         o[u] = i + u < rlen ? spos32[rlo + i + u] : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
+                bool hit[BROAD_BATCH];
                 bool any = false;
 #pragma unroll
-                for (int u = 0; u < BROAD_BATCH; ++u) any |= near(o[u]) && (u == 0 || i + u < rlen);
+                for (int u = 0; u < BROAD_BATCH; ++u) {
+                    hit[u] = near(o[u]) && (u == 0 || i + u < rlen);
+                    any |= hit[u];
+                }
                 if (!any) continue;
-#pragma unroll 1
-                for (uint32_t u = 0; u < BROAD_BATCH && i + u < rlen; ++u) {
-                    const uint32_t k = rlo + i + u;
-                    if (!near(in_stage ? stage[rlb + i + u] : spos32[k])) continue;
-                    const uint32_t q = sidx[k];
-                    if (!have_me) { me = pos[p]; have_me = true; }
-                    const double4 ov = pos[q];
-                    const double ex = me.x - ov.x, ey = me.y - ov.y, ez = me.z - ov.z;
-                    const double rs = (me.w + ov.w) * mult;
-                    if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) continue;  // sqrt(d2) >= rs
-                    const bool plo = p < q;
-                    const uint32_t il = plo ? p : q, ih = plo ? q : p;
-                    const double4 xl = plo ? me : ov, xh = plo ? ov : me;
-                    const double rl = INIT ? r0[il] : xl.w, rh = INIT ? r0[ih] : xh.w;
-                    if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
-                    // when the contact reach is below cs/2 every touching pair passes this test
-                    if (S.ref_check &&
-                        !ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
-                        continue;
-                    ++found;
-                    row_append(tcnt, touch, n, P.K, p, q, overflow);
-                    row_append(tcnt, touch, n, P.K, q, p, overflow);
+                // survivors go to the wave's queue, confirmed below by all lanes together; a
+                // survivor that finds the queue full is confirmed at once
+#pragma unroll
+                for (int u = 0; u < BROAD_BATCH; ++u) {
+                    const uint64_t m = __ballot(hit[u]);
+                    if (!m) continue;
+                    const uint32_t base = hq_n[wv];
+                    const uint32_t cnt = (uint32_t)__popcll(m);
+                    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                    const bool fits = base + cnt <= (uint32_t)HIT_QUEUE;
+                    if (hit[u]) {
+                        if (fits) {
+                            hq_k[wv][base + rank] = rlo + i + (uint32_t)u;
+                            hq_l[wv][base + rank] = (uint8_t)lane;
+                        } else {
+                            confirm(S, p, rlo + i + (uint32_t)u);
+                        }
+                    }
+                    if (fits && rank == 0 && hit[u]) hq_n[wv] = base + cnt;
                 }
             }
+        }
+    }
+    // every lane of the wave confirms queued survivors (the queue holds only live owners)
+    {
+        const uint32_t qn = hq_n[wv];
+        for (uint32_t e = (uint32_t)lane; e < qn; e += 64u) {
+            const uint32_t k = hq_k[wv][e];
+            const int64_t to = t0 + (int64_t)wv * 64 + hq_l[wv][e];
+            const uint32_t po = sidx[to];
+            confirm(scene_of<SEG>(P, po), po, k);
         }
     }
     if (diag) wave_add(ctr_at(ctr, CT_TOUCH), found);
