@@ -26,28 +26,30 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> float:
     """Compulsory HBM bytes of one launch of ``kernel`` in this engine's design (each array the
     kernel must touch, read or written once):
-      broad_phase     sorted mirror read once (32 B/grain, +16 B FP32 copy in mixed precision),
-                      cell starts (4 B/cell), sorted index of touching pairs (4 B), and the two
-                      row appends per touching pair (4 B counter + 4 B entry each)
+      hash            previous order 4 B + position 32 B read; bucket, rank 8 B + FP32 position
+                      16 B written per grain
+      scan            cell counts read + cleared 8 B, starts written 4 B per cell
+      scatter         bucket, rank, previous order, FP32 position 28 B + bucket start 4 B read;
+                      order, sorted bucket, sorted FP32 position 24 B written per grain
+      broad_phase     sorted FP32 position, bucket and order 24 B per grain, bucket starts 4 B per
+                      cell; per touching pair both FP64 positions 64 B, the partner's index 4 B and
+                      the two row appends (4 B counter + 4 B entry each)
+      merge           touch count 4 B + row count and selector 5 B per grain
       force_integrate grain state read 128 B + written 128 B, row counts 5 B, wall flags 4 B;
                       per Hertz contact 2 rows x (partner 4 + spring 24 r/w + k,F 48 w),
-                      per bond 2 rows x (partner 4 + F,ti,tj 72 r/w + k 24 w), per wall contact 32 B
-      hash            position read 32 B, key + rank written 8 B, one counter atomic 4 B
-      scatter         key + rank read 8 B, position read 32 B, sorted index + position written 36 B
-      scan            cell counts read + zeroed 8 B, starts written 4 B per cell
-      merge           touch count 4 B + row count and selector 5 B per grain"""
-    n, P = c["n"], c["candidate_pairs"]
+                      per bond 2 rows x (partner 4 + F,ti,tj 72 r/w + k 24 w), per wall contact 32 B"""
+    n = c["n"]
     cells = c.get("grid_cells") or c["hash_table_size"]
     Ch, Cb, W = c["hertz"], c["bonded"], c["wall_contacts"]
     touching = Ch + Cb
     if kernel == "broad_phase":
-        return n * (32.0 + (16.0 if precision == "mixed" else 0.0)) + 4.0 * cells + touching * (4 + 16)
+        return n * 24.0 + 4.0 * cells + touching * (64 + 4 + 16)
     if kernel == "force_integrate":
         return n * (128 + 128 + 5 + 4) + 2 * Ch * (4 + 48 + 48) + 2 * Cb * (4 + 144 + 24) + 32.0 * W
     if kernel == "hash":
-        return n * 44.0
+        return n * 60.0
     if kernel == "scatter":
-        return n * 76.0
+        return n * 56.0
     if kernel == "scan":
         return 12.0 * cells
     if kernel == "merge":
@@ -59,6 +61,26 @@ def _reference_equivalent_bytes(c: dict) -> float:
     """Bytes the reference design moves for its broad phase: 20 B per cp_list entry (write 8 +
     read 8 + one 4-B BitTable probe), the per-candidate-pair figure of the measurement plan."""
     return 20.0 * c["candidate_pairs"]
+
+
+PROFILE_TAG = {"pour": "pour1m", "pour_packed": "pour1m_packed", "batch256": "batch256",
+               "cube18k": "cube18k", "cube900": "cube900"}
+
+
+def measured_traffic(workload: str, precision: str, n: int):
+    """HBM bytes per launch of each kernel from the committed rocprofv3 summary of this workload
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the same counters' reading for a copy kernel
+    of known size): the line ``traffic_bytes_per_launch = {...}`` of
+    profiles/<workload>_<precision>_hbm_traffic.md.  None when no summary covers the run."""
+    if workload not in PROFILE_TAG or (workload.startswith("pour") and n != 1_000_000):
+        return None, None
+    path = os.path.join(ROOT, "profiles", f"{PROFILE_TAG[workload]}_{precision}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch = "):
+            return json.loads(line.split("=", 1)[1]), os.path.relpath(path, ROOT)
+    return None, None
 
 
 def build_workload(name: str, n: int):
@@ -293,15 +315,18 @@ def main():
     c = r["counters"]
     alg = _algorithmic_bytes(dom, c, len(w), args.precision)
     achieved = alg / (step_ms[dom] * 1e-3) / 1e9 if step_ms[dom] > 0 else 0.0
+    traffic, tsrc = measured_traffic(args.workload, args.precision, n)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "algorithmic_bytes": alg,
+                "traffic": traffic.get(dom) if traffic else None, "algorithmic_bytes": alg,
                 "avg_launch_ms": round(step_ms[dom], 5)}
+    if traffic and traffic.get(dom):
+        roofline["traffic_source"] = tsrc
+        roofline["traffic_GB_s"] = round(traffic[dom] / (step_ms[dom] * 1e-3) / 1e9, 1)
     if dom == "broad_phase":
         # candidates examined are re-read from L2/MALL, not HBM: the kernel's practical limiter
         roofline["candidates_examined"] = c["examined"]
-        roofline["cache_GB_s"] = round(c["examined"] * (16 if args.precision == "mixed" else 32)
-                                       / (step_ms[dom] * 1e-3) / 1e9, 1)
+        roofline["cache_GB_s"] = round(c["examined"] * 16 / (step_ms[dom] * 1e-3) / 1e9, 1)
         roofline["reference_equivalent_GB_s"] = round(
             _reference_equivalent_bytes(c) / (step_ms[dom] * 1e-3) / 1e9, 1)
     per_kernel = {}
