@@ -51,7 +51,7 @@ void free_scene(dem_scene* s) {
     DevState& d = s->d;
     void* ptrs[] = {d.pos[0], d.pos[1], d.vel[0], d.vel[1], d.omg[0], d.omg[1], d.quat[0], d.quat[1],
                     d.acc, d.dw, d.ftot, d.ttot, d.r0, d.wact, d.wshear, d.hid, d.rank, d.x32, d.bcnt,
-                    d.bstart, d.sidxb[0], d.sidxb[1], d.skey, d.spos, d.spos32, d.tcnt, d.touch, d.sel,
+                    d.bstart, d.sidxb[0], d.sidxb[1], d.skey, d.eid, d.spos, d.spos32, d.tcnt, d.touch, d.sel,
                     d.rcnt[0], d.rcnt[1],
                     d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, d.scan.status, d.scan.counters,
                     d.scratch_u32, d.scratch2_u32, d.total, d.seg, d.sid, d.seg_broken};
@@ -279,6 +279,43 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
     return DEM_OK;
 }
 
+// The internal order of the particles: within each scene, by the cell of the initial position in
+// the scene's per-step dense grid (or in the reference grid when the dense grid is unused), x
+// fastest, ties by file index.  Neighbours in space become neighbours in memory, so the sort,
+// the broad phase and the partner gathers of the force kernel read nearby addresses.
+void spatial_order(dem_scene* s, const dem_grain_in_t* g) {
+    const int64_t n = s->n;
+    s->perm.resize(n);
+    s->iperm.resize(n);
+    std::vector<uint64_t> key(n);
+    const int nseg = s->batched ? s->nseg : 1;
+    for (int k = 0; k < nseg; ++k) {
+        const int64_t a = s->batched ? s->seg_off[k] : 0, b = s->batched ? s->seg_off[k + 1] : n;
+        const SceneP& S = s->batched ? s->seg_host[k] : s->P.sc;
+        for (int64_t i = a; i < b; ++i) {
+            int64_t c[3];
+            for (int ax = 0; ax < 3; ++ax) {
+                double v;
+                int64_t hi;
+                if (S.dense.cells > 0) {
+                    v = (g[i].k[ax] - S.dense.origin[ax]) * S.dense.inv_cell;
+                    hi = std::min<int64_t>(S.dense.dims[ax] - 1, (1 << 21) - 1);
+                } else {
+                    v = (g[i].k[ax] - S.grid.dmin[ax]) / S.grid.cs;
+                    hi = (1 << 21) - 1;
+                }
+                int64_t q = std::isfinite(v) ? (int64_t)std::floor(std::max(-1.0, std::min(v, 4.0e6))) : 0;
+                c[ax] = q < 0 ? 0 : (q > hi ? hi : q);
+            }
+            key[i] = ((uint64_t)c[2] << 42) | ((uint64_t)c[1] << 21) | (uint64_t)c[0];
+            s->perm[i] = (uint32_t)i;
+        }
+        std::stable_sort(s->perm.begin() + a, s->perm.begin() + b,
+                         [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
+    }
+    for (int64_t u = 0; u < n; ++u) s->iperm[s->perm[u]] = (uint32_t)u;
+}
+
 // Device allocation and initial upload (executor.jl:229-242: q = 1, w = a = dw = 0) for the n
 // grains already derived into the host arrays.
 int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max, int64_t nkeys) {
@@ -301,7 +338,7 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall_max > 0 ? nwall_max : 1) * n);
     ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.x32, n); ALLOC(d.bcnt, nkeys + 1);
     ALLOC(d.bstart, nkeys + 1); ALLOC(d.sidxb[0], n); ALLOC(d.sidxb[1], n); ALLOC(d.skey, n);
-    ALLOC(d.spos, n); ALLOC(d.spos32, n);
+    ALLOC(d.spos, n); ALLOC(d.spos32, n); ALLOC(d.eid, n);
     d.sidx = d.sidxb[0];
     ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
     ALLOC(d.ctr, CTR_WORDS);
@@ -319,19 +356,24 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     memset(s->h_ctr_raw, 0, sizeof(unsigned long long) * CTR_WORDS);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, "hipStreamCreate failed");
+    spatial_order(s, g);
     std::vector<double4> hp(n), hv(n), hw(n), hq(n);
-    for (int64_t i = 0; i < n; ++i) {
-        hp[i] = make_double4(g[i].k[0], g[i].k[1], g[i].k[2], s->h_r[i]);
-        hv[i] = make_double4(g[i].v[0], g[i].v[1], g[i].v[2], g[i].m);
-        hw[i] = make_double4(0.0, 0.0, 0.0, s->h_I[i]);
-        hq[i] = make_double4(1.0, 0.0, 0.0, 0.0);
+    std::vector<double> hr0(n);
+    for (int64_t u = 0; u < n; ++u) {
+        const int64_t i = s->perm[u];
+        hp[u] = make_double4(g[i].k[0], g[i].k[1], g[i].k[2], s->h_r[i]);
+        hv[u] = make_double4(g[i].v[0], g[i].v[1], g[i].v[2], g[i].m);
+        hw[u] = make_double4(0.0, 0.0, 0.0, s->h_I[i]);
+        hq[u] = make_double4(1.0, 0.0, 0.0, 0.0);
+        hr0[u] = s->h_r0[i];
     }
     e = hipMemcpy(d.pos[0], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.vel[0], hv.data(), n * sizeof(double4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.omg[0], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.quat[0], hq.data(), n * sizeof(double4), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.r0, s->h_r0, n * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {  // the first sort starts from file order
+    if (e == hipSuccess) e = hipMemcpy(d.r0, hr0.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d.eid, s->perm.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {  // the first sort starts from the internal (spatial) order
         std::vector<uint32_t> iota(n);
         for (int64_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
         e = hipMemcpy(d.sidxb[0], iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
@@ -543,6 +585,13 @@ int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, i
     const SceneP& Ss = src->batched ? src->seg_host[src_scene] : src->P.sc;
     const SceneP& Sd = dst->batched ? dst->seg_host[dst_scene] : dst->P.sc;
     if (Ss.nwall != Sd.nwall) return fail(dst, DEM_ERR_INVALID, "scenes differ in wall count");
+    {  // state moves slot for slot, so both sides must store the scene's particles in one order
+        const int64_t so = src->seg_off[src_scene], dof = dst->seg_off[dst_scene];
+        for (int64_t k = 0; k < n; ++k)
+            if ((int64_t)src->perm[so + k] - so != (int64_t)dst->perm[dof + k] - dof)
+                return fail(dst, DEM_ERR_INVALID,
+                            "scenes store their particles in different orders (different initial grains)");
+    }
     HIPCHK(dst, hipSetDevice(dst->device));
     HIPCHK(dst, hipStreamSynchronize(src->stream));
     HIPCHK(dst, hipStreamSynchronize(dst->stream));
@@ -724,16 +773,17 @@ int32_t dem_read_grains(dem_scene_t* s, dem_grain_t* out, int64_t n) {
     HIPCHK(s, hipMemcpy(a.data(), d.acc, n * sizeof(double4), hipMemcpyDeviceToHost));
     HIPCHK(s, hipMemcpy(dw.data(), d.dw, n * sizeof(double4), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) {
+        const int64_t u = s->iperm[i];
         dem_grain_t& o = out[i];
         memset(&o, 0, sizeof(o));
         o.id = s->h_id[i]; o.gid = s->h_gid[i]; o.mid = s->h_mid[i];
         o.V = s->h_V[i]; o.m = s->h_m[i]; o.r = s->h_r[i]; o.r0 = s->h_r0[i];
-        o.k[0] = p[i].x; o.k[1] = p[i].y; o.k[2] = p[i].z;
-        o.v[0] = v[i].x; o.v[1] = v[i].y; o.v[2] = v[i].z;
-        o.a[0] = a[i].x; o.a[1] = a[i].y; o.a[2] = a[i].z;
-        o.q[0] = q[i].x; o.q[1] = q[i].y; o.q[2] = q[i].z; o.q[3] = q[i].w;
-        o.w[0] = w[i].x; o.w[1] = w[i].y; o.w[2] = w[i].z;
-        o.dw[0] = dw[i].x; o.dw[1] = dw[i].y; o.dw[2] = dw[i].z;
+        o.k[0] = p[u].x; o.k[1] = p[u].y; o.k[2] = p[u].z;
+        o.v[0] = v[u].x; o.v[1] = v[u].y; o.v[2] = v[u].z;
+        o.a[0] = a[u].x; o.a[1] = a[u].y; o.a[2] = a[u].z;
+        o.q[0] = q[u].x; o.q[1] = q[u].y; o.q[2] = q[u].z; o.q[3] = q[u].w;
+        o.w[0] = w[u].x; o.w[1] = w[u].y; o.w[2] = w[u].z;
+        o.dw[0] = dw[u].x; o.dw[1] = dw[u].y; o.dw[2] = dw[u].z;
         o.I[0] = s->h_I[i]; o.I[4] = s->h_I[i]; o.I[8] = s->h_I[i];
     }
     return DEM_OK;
@@ -744,13 +794,14 @@ int32_t dem_write_grains(dem_scene_t* s, const dem_grain_t* in, int64_t n) {
     HIPCHK(s, hipSetDevice(s->device));
     std::vector<double4> p(n), v(n), w(n), q(n), a(n), dw(n);
     for (int64_t i = 0; i < n; ++i) {
+        const int64_t u = s->iperm[i];
         const dem_grain_t& g = in[i];
-        p[i] = make_double4(g.k[0], g.k[1], g.k[2], s->h_r[i]);
-        v[i] = make_double4(g.v[0], g.v[1], g.v[2], s->h_m[i]);
-        w[i] = make_double4(g.w[0], g.w[1], g.w[2], s->h_I[i]);
-        q[i] = make_double4(g.q[0], g.q[1], g.q[2], g.q[3]);
-        a[i] = make_double4(g.a[0], g.a[1], g.a[2], 0.0);
-        dw[i] = make_double4(g.dw[0], g.dw[1], g.dw[2], 0.0);
+        p[u] = make_double4(g.k[0], g.k[1], g.k[2], s->h_r[i]);
+        v[u] = make_double4(g.v[0], g.v[1], g.v[2], s->h_m[i]);
+        w[u] = make_double4(g.w[0], g.w[1], g.w[2], s->h_I[i]);
+        q[u] = make_double4(g.q[0], g.q[1], g.q[2], g.q[3]);
+        a[u] = make_double4(g.a[0], g.a[1], g.a[2], 0.0);
+        dw[u] = make_double4(g.dw[0], g.dw[1], g.dw[2], 0.0);
     }
     DevState& d = s->d;
     const int c = s->cur;
@@ -774,8 +825,24 @@ int32_t dem_read_contacts(dem_scene_t* s, dem_contact_t* out, uint8_t* bonded, i
     HIPCHK(s, run_read_contacts(s, &dout, &dbd, &total));
     const int64_t w = std::min(cap, total);
     hipError_t e = hipSuccess;
-    if (w > 0) e = hipMemcpy(out, dout, w * sizeof(dem_contact_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && w > 0 && bonded) e = hipMemcpy(bonded, dbd, w, hipMemcpyDeviceToHost);
+    if (w > 0) {
+        // records come out in internal row order: sort by (i, j) in file indices
+        std::vector<dem_contact_t> all(total);
+        std::vector<uint8_t> bd(total);
+        e = hipMemcpy(all.data(), dout, total * sizeof(dem_contact_t), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(bd.data(), dbd, total, hipMemcpyDeviceToHost);
+        if (e == hipSuccess) {
+            std::vector<int64_t> ord(total);
+            for (int64_t k = 0; k < total; ++k) ord[k] = k;
+            std::sort(ord.begin(), ord.end(), [&](int64_t x, int64_t y) {
+                return all[x].i != all[y].i ? all[x].i < all[y].i : all[x].j < all[y].j;
+            });
+            for (int64_t k = 0; k < w; ++k) {
+                out[k] = all[ord[k]];
+                if (bonded) bonded[k] = bd[ord[k]];
+            }
+        }
+    }
     hipFree(dout);
     hipFree(dbd);
     if (e != hipSuccess) return hip_fail(s, "hipMemcpy(contacts)", e);
@@ -789,15 +856,16 @@ int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_
     HIPCHK(s, hipSetDevice(s->device));
     const int64_t n = s->n;
     const int K = s->K;
-    struct Ent { uint32_t row, partner; int64_t src; };
+    struct Ent { uint32_t row, partner; int64_t src; bool is_i; };
     std::vector<Ent> ents;
     ents.reserve(2 * m);
     for (int64_t c = 0; c < m; ++c) {
         const int64_t i = in[c].i, j = in[c].j;
         if (i < 1 || j < 1 || i > n || j > n || i >= j)
             return fail(s, DEM_ERR_INVALID, "contacts need 1 <= i < j <= n");
-        ents.push_back({(uint32_t)(i - 1), (uint32_t)(j - 1), c});
-        ents.push_back({(uint32_t)(j - 1), (uint32_t)(i - 1), c});
+        const uint32_t ui = s->iperm[i - 1], uj = s->iperm[j - 1];
+        ents.push_back({ui, uj, c, true});
+        ents.push_back({uj, ui, c, false});
     }
     std::sort(ents.begin(), ents.end(), [](const Ent& a, const Ent& b) {
         return a.row != b.row ? a.row < b.row : a.partner < b.partner;
@@ -814,7 +882,7 @@ int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_
         if ((int)slot >= K) return fail(s, DEM_ERR_CAPACITY, "contact slots per particle exceeded");
         dst[k] = (uint64_t)slot * n + en.row;
         const bool bd = bonded && bonded[en.src];
-        word[k] = en.partner | (bd ? BONDED : 0u);
+        word[k] = en.partner | (bd ? BONDED : 0u) | (en.is_i ? IS_I : 0u);
         const dem_contact_t& c = in[en.src];
         double* sv = &state[k * S_NCOMP];
         for (int a = 0; a < 3; ++a) {
@@ -856,8 +924,9 @@ int32_t dem_read_wall_shear(dem_scene_t* s, double* out, int64_t n) {
     HIPCHK(s, hipMemcpy(act.data(), s->d.wact, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i)
         for (int j = 0; j < nw; ++j) {
-            const bool on = (act[i] >> j) & 1u;
-            const double4 v = ws[(size_t)j * n + i];
+            const int64_t u = s->iperm[i];
+            const bool on = (act[u] >> j) & 1u;
+            const double4 v = ws[(size_t)j * n + u];
             double* o = &out[(i * nw + j) * 3];
             o[0] = on ? v.x : 0.0; o[1] = on ? v.y : 0.0; o[2] = on ? v.z : 0.0;
         }
@@ -873,9 +942,10 @@ int32_t dem_write_wall_shear(dem_scene_t* s, const double* in, int64_t n) {
     std::vector<uint32_t> act(n, 0u);
     for (int64_t i = 0; i < n; ++i)
         for (int j = 0; j < nw; ++j) {
+            const int64_t u = s->iperm[i];
             const double* v = &in[(i * nw + j) * 3];
-            ws[(size_t)j * n + i] = make_double4(v[0], v[1], v[2], 0.0);
-            if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) act[i] |= 1u << j;
+            ws[(size_t)j * n + u] = make_double4(v[0], v[1], v[2], 0.0);
+            if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) act[u] |= 1u << j;
         }
     HIPCHK(s, hipStreamSynchronize(s->stream));
     HIPCHK(s, hipMemcpy(s->d.wshear, ws.data(), ws.size() * sizeof(double4), hipMemcpyHostToDevice));
@@ -891,8 +961,9 @@ int32_t dem_read_forces(dem_scene_t* s, double* F, double* T, int64_t n) {
     HIPCHK(s, hipMemcpy(f.data(), s->d.ftot, n * sizeof(double4), hipMemcpyDeviceToHost));
     HIPCHK(s, hipMemcpy(t.data(), s->d.ttot, n * sizeof(double4), hipMemcpyDeviceToHost));
     for (int64_t i = 0; i < n; ++i) {
-        if (F) { F[3 * i] = f[i].x; F[3 * i + 1] = f[i].y; F[3 * i + 2] = f[i].z; }
-        if (T) { T[3 * i] = t[i].x; T[3 * i + 1] = t[i].y; T[3 * i + 2] = t[i].z; }
+        const int64_t u = s->iperm[i];
+        if (F) { F[3 * i] = f[u].x; F[3 * i + 1] = f[u].y; F[3 * i + 2] = f[u].z; }
+        if (T) { T[3 * i] = t[u].x; T[3 * i + 1] = t[u].y; T[3 * i + 2] = t[u].z; }
     }
     return DEM_OK;
 }
@@ -910,11 +981,24 @@ int32_t dem_debug_cells(dem_scene_t* s, int32_t* kid, uint32_t* hid, double* kce
     HIPCHK(s, hipMalloc(&dn, 8 * n * sizeof(uint32_t)));
     hipError_t e = run_debug_cells(s, dk, dh, dc, dn);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
-    if (e == hipSuccess && kid) e = hipMemcpy(kid, dk, 3 * n * sizeof(int32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && hid) e = hipMemcpy(hid, dh, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && kcenter) e = hipMemcpy(kcenter, dc, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess && nbr) e = hipMemcpy(nbr, dn, 8 * n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    std::vector<int32_t> k3(3 * n);
+    std::vector<uint32_t> h1(n), n8(8 * n);
+    std::vector<double> c3(3 * n);
+    if (e == hipSuccess) e = hipMemcpy(k3.data(), dk, 3 * n * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(h1.data(), dh, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(c3.data(), dc, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(n8.data(), dn, 8 * n * sizeof(uint32_t), hipMemcpyDeviceToHost);
     hipFree(dk); hipFree(dh); hipFree(dc); hipFree(dn);
+    if (e == hipSuccess)
+        for (int64_t i = 0; i < n; ++i) {  // back to file order
+            const int64_t u = s->iperm[i];
+            for (int a = 0; a < 3; ++a) {
+                if (kid) kid[3 * i + a] = k3[3 * u + a];
+                if (kcenter) kcenter[3 * i + a] = c3[3 * u + a];
+            }
+            if (hid) hid[i] = h1[u];
+            if (nbr) for (int a = 0; a < 8; ++a) nbr[8 * i + a] = n8[8 * u + a];
+        }
     if (e != hipSuccess) return hip_fail(s, "dem_debug_cells", e);
     return DEM_OK;
 }
@@ -934,7 +1018,7 @@ int32_t dem_debug_hash_table(dem_scene_t* s, uint32_t* counts, uint32_t* start, 
         if (start) start[b] = bs[b];
     }
     if (pid)
-        for (int64_t k = 0; k < n; ++k) pid[k] = si[k] + 1u;
+        for (int64_t k = 0; k < n; ++k) pid[k] = s->perm[si[k]] + 1u;
     return DEM_OK;
 }
 

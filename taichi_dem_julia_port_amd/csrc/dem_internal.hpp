@@ -12,7 +12,9 @@
 namespace dem {
 
 constexpr int MAX_WALLS = 16;
-constexpr uint32_t IDX_MASK = 0x3fffffffu;  // partner index bits of a row entry
+constexpr uint32_t IDX_MASK = 0x1fffffffu;  // partner index bits of a row entry
+constexpr uint32_t IS_I = 1u << 29;         // the row's particle is the contact's i (the lower
+                                            // file index): the canonical orientation
 constexpr uint32_t BONDED = 1u << 30;       // entry is an EBPM bond
 constexpr uint32_t BROKEN = 1u << 31;       // bond broke this step; dropped by the next merge
 
@@ -134,6 +136,7 @@ struct DevState {
     uint32_t* sidxb[2] = {nullptr, nullptr};
     uint32_t* sidx = nullptr;    // sorted slot -> particle (sidxb[cur] after a sort)
     uint32_t* skey = nullptr;    // sorted slot -> bucket
+    uint32_t* eid = nullptr;     // internal index -> file index
     double4* spos = nullptr;     // sorted copy of pos (reference-grid broad phases only)
     float4* spos32 = nullptr;    // sorted FP32 copy (broad-phase pre-test)
     // broad phase output
@@ -172,6 +175,11 @@ struct dem_scene {
     bool dense = false;      // per-step sort on the dense cs/2 grid (else the reference's grid)
     bool batched = false;    // several scenes in one set of arrays (per-particle scene ids)
     int32_t nseg = 1;
+    // Particles are stored in a spatial order (by cell of the initial position, within each
+    // scene): internal index u holds the particle of file index perm[u]; iperm inverts it.
+    // Every array on the device uses internal indices; the read, write and debug entry points
+    // translate, and orientation (i < j) always compares file indices.
+    std::vector<uint32_t> perm, iperm;
     std::vector<int64_t> seg_off;   // [nseg + 1] first particle of each scene
     std::vector<dem::SceneP> seg_host;
     int64_t steps = 0;

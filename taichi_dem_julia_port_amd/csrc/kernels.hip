@@ -112,13 +112,14 @@ template <bool INIT, bool MIXED>
 __global__ void __launch_bounds__(TPB)
 k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
         const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
-        const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
-        unsigned long long* ctr) {
+        const double* __restrict__ r0, const uint32_t* __restrict__ eid, uint32_t* __restrict__ tcnt,
+        uint32_t* __restrict__ touch, unsigned long long* ctr) {
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     uint64_t cand = 0, found = 0;
     uint32_t overflow = 0;
     if (t < P.n) {
         const uint32_t p = sidx[t];
+        const uint32_t ep = eid[p];
         const double4 me = spos[t];
         const d3 xp = mk3(me.x, me.y, me.z);
         const double rp = INIT ? r0[p] : me.w;
@@ -138,7 +139,7 @@ k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __rest
             const uint32_t s0 = bstart[b], s1 = bstart[b + 1];
             for (uint32_t k = s0; k < s1; ++k) {
                 const uint32_t q = sidx[k];
-                if (q <= p) continue;
+                if (eid[q] <= ep) continue;  // the pair (i, j), i < j in file order
                 ++cand;
                 if (dup) continue;  // an aliased window bucket repeats the same candidates
                 if (MIXED) {
@@ -154,7 +155,7 @@ k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __rest
                 // kernels.jl:103 (norm(k_i - k_j) < r_i + r_j), kernels.jl:72 with r0 at t = 0
                 if (norm(xp - mk3(o.x, o.y, o.z)) < rp + rq) {
                     ++found;
-                    row_append(tcnt, touch, P.n, P.K, p, q, overflow);
+                    row_append(tcnt, touch, P.n, P.K, p, q | IS_I, overflow);
                     row_append(tcnt, touch, P.n, P.K, q, p, overflow);
                 }
             }
@@ -175,8 +176,8 @@ template <bool INIT, bool MIXED>
 __global__ void __launch_bounds__(TPB)
 k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* __restrict__ spos,
              const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
-             const double* __restrict__ r0, uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
-             unsigned long long* ctr) {
+             const double* __restrict__ r0, const uint32_t* __restrict__ eid,
+             uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch, unsigned long long* ctr) {
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     uint64_t found = 0;
     uint32_t overflow = 0;
@@ -214,12 +215,13 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
                 const double4 o = spos[k];
                 if (p == 0xffffffffu) p = sidx[t];
                 const uint32_t q = sidx[k];
-                const uint32_t lo = p < q ? p : q, hi = p < q ? q : p;
-                const double4 xl = p < q ? me : o, xh = p < q ? o : me;
+                const bool pi = eid[p] < eid[q];  // p is the pair's i (file order)
+                const uint32_t lo = pi ? p : q, hi = pi ? q : p;
+                const double4 xl = pi ? me : o, xh = pi ? o : me;
                 const double rl = INIT ? r0[lo] : xl.w, rh = INIT ? r0[hi] : xh.w;
                 // kernels.jl:103 (norm(k_i - k_j) < r_i + r_j), kernels.jl:72 with r0 at t = 0
                 if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) continue;
-                if (p > q) {  // owner is hi: is hi's bucket in lo's window?
+                if (!pi) {  // owner is hi: is hi's bucket in lo's window?
                     uint32_t wl[8];
                     window_of(P.sc.grid, mk3(o.x, o.y, o.z), wl);
                     bool in = false;
@@ -228,8 +230,8 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
                     if (!in) continue;
                 }
                 ++found;
-                row_append(tcnt, touch, P.n, P.K, p, q, overflow);
-                row_append(tcnt, touch, P.n, P.K, q, p, overflow);
+                row_append(tcnt, touch, P.n, P.K, p, q | (pi ? IS_I : 0u), overflow);
+                row_append(tcnt, touch, P.n, P.K, q, p | (pi ? 0u : IS_I), overflow);
             }
         }
     }
@@ -276,8 +278,8 @@ __global__ void __launch_bounds__(TPB)
 k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
               const double4* __restrict__ pos, const float4* __restrict__ spos32,
               const uint32_t* __restrict__ bstart, const double* __restrict__ r0,
-              uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch, unsigned long long* ctr,
-              int diag) {
+              const uint32_t* __restrict__ eid, uint32_t* __restrict__ tcnt,
+              uint32_t* __restrict__ touch, unsigned long long* ctr, int diag) {
     __shared__ float4 stage[STAGE_CAP];
     __shared__ uint32_t u_lo[5], u_off[6];  // union slot ranges and their LDS offsets
     __shared__ uint32_t b_cell[2], b_scene[2];
@@ -387,17 +389,17 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
         const double ex = mo.x - ov.x, ey = mo.y - ov.y, ez = mo.z - ov.z;
         const double rs = (mo.w + ov.w) * (INIT ? Sc.bond_mult * (1.0 + 1e-9) : 1.0);
         if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) return;  // sqrt(d2) >= rs
-        const bool plo = po < q;
-        const uint32_t il = plo ? po : q, ih = plo ? q : po;
-        const double4 xl = plo ? mo : ov, xh = plo ? ov : mo;
+        const bool pi = eid[po] < eid[q];  // po is the pair's i (file order)
+        const uint32_t il = pi ? po : q, ih = pi ? q : po;
+        const double4 xl = pi ? mo : ov, xh = pi ? ov : mo;
         const double rl = INIT ? r0[il] : xl.w, rh = INIT ? r0[ih] : xh.w;
         if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < rl + rh)) return;
         // when the contact reach is below cs/2 every touching pair passes this test
         if (Sc.ref_check && !ref_candidate(Sc.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z)))
             return;
         ++found;
-        row_append(tcnt, touch, n, P.K, po, q, overflow);
-        row_append(tcnt, touch, n, P.K, q, po, overflow);
+        row_append(tcnt, touch, n, P.K, po, q | (pi ? IS_I : 0u), overflow);
+        row_append(tcnt, touch, n, P.K, q, po | (pi ? 0u : IS_I), overflow);
     };
     if (work) {
         const float slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
@@ -525,15 +527,17 @@ k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32
 // the last step of each dem_step call.
 __global__ void __launch_bounds__(TPB)
 k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx,
-             const uint32_t* __restrict__ bstart, unsigned long long* ctr) {
+             const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ eid,
+             unsigned long long* ctr) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     uint64_t c = 0;
     if (i < P.n) {
         const double4 x = pos[i];
+        const uint32_t ei = eid[i];
         uint32_t w[8];
         window_of(P.sc.grid, mk3(x.x, x.y, x.z), w);
         for (int wi = 0; wi < 8; ++wi)
-            for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += (uint32_t)i < sidx[k];
+            for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += ei < eid[sidx[k]];
     }
     wave_add(ctr_at(ctr, CT_CAND), c);
 }
@@ -567,26 +571,27 @@ __device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __res
     while (io < nold || in < nu) {
         const uint32_t eo = io < nold ? old_at(io) : INF;
         const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
-        const uint32_t qn = in < nu ? touch_at(in) : INF;
+        const uint32_t tw = in < nu ? touch_at(in) : INF;
+        const uint32_t qn = in < nu ? (tw & IDX_MASK) : INF;
         if (qo < qn) {  // only in the old row
             ++io;
             if (eo & BROKEN) { changed = true; continue; }
             if ((eo & BONDED) || INIT) { ++nout; continue; }
-            const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
+            const uint32_t lo = (eo & IS_I) ? p : qo, hi = (eo & IS_I) ? qo : p;
             if (in_window(S, pos, lo, hi)) {  // in cp_list and not touching: deactivate
                 changed = true;
-                if (p < qo) ++deact;
+                if (eo & IS_I) ++deact;
             } else {
                 ++nout;  // not a candidate this step: untouched (kernels.jl:97-119)
             }
         } else if (qn < qo) {  // only in the new set: append a zeroed contact
             ++in; ++nout; changed = true;
-            if (p < qn) ++created;
+            if (tw & IS_I) ++created;
         } else {  // in both
             ++io; ++in; ++nout;
             if (eo & BROKEN) {  // bond removed at the end of last step, re-created as Hertz
                 changed = true;
-                if (p < qn) ++created;
+                if (tw & IS_I) ++created;
             } else if (INIT && !(eo & BONDED)) {
                 changed = true;
             }
@@ -606,14 +611,15 @@ __device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __res
     while (io < nold || in < nu) {
         const uint32_t eo = io < nold ? old_at(io) : INF;
         const uint32_t qo = io < nold ? (eo & IDX_MASK) : INF;
-        const uint32_t qn = in < nu ? touch_at(in) : INF;
+        const uint32_t tw = in < nu ? touch_at(in) : INF;
+        const uint32_t qn = in < nu ? (tw & IDX_MASK) : INF;
         const int64_t dst = (int64_t)o * n + p;
         if (qo < qn) {
             const int64_t src = (int64_t)io * n + p;
             ++io;
             if (eo & BROKEN) continue;
             if (!(eo & BONDED) && !INIT) {
-                const uint32_t lo = p < qo ? p : qo, hi = p < qo ? qo : p;
+                const uint32_t lo = (eo & IS_I) ? p : qo, hi = (eo & IS_I) ? qo : p;
                 if (in_window(S, pos, lo, hi)) continue;
             }
             pqn[dst] = eo;
@@ -622,7 +628,7 @@ __device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __res
             ++o;
         } else if (qn < qo) {
             ++in;
-            pqn[dst] = qn | (INIT ? BONDED : 0u);
+            pqn[dst] = (tw & (IDX_MASK | IS_I)) | (INIT ? BONDED : 0u);
 #pragma unroll
             for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
             ++o;
@@ -630,7 +636,7 @@ __device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __res
             const int64_t src = (int64_t)io * n + p;
             ++io; ++in;
             if (eo & BROKEN) {
-                pqn[dst] = qn;
+                pqn[dst] = tw & (IDX_MASK | IS_I);
 #pragma unroll
                 for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
             } else if (INIT && !(eo & BONDED)) {
@@ -649,18 +655,19 @@ __device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __res
     sel[p] = (uint8_t)nb;
 }
 
-// Sort v[0..nt) ascending (insertion sort: rows are short) and drop repeats; returns the count.
+// Sort the touch words v[0..nt) by partner index (insertion sort: rows are short) and drop
+// repeats; returns the count.
 template <typename Get, typename Set>
 __device__ __forceinline__ uint32_t sort_unique(uint32_t nt, Get get, Set set) {
     for (uint32_t a = 1; a < nt; ++a) {
         const uint32_t v = get(a);
         int b = (int)a - 1;
-        while (b >= 0 && get(b) > v) { set(b + 1, get(b)); --b; }
+        while (b >= 0 && (get(b) & IDX_MASK) > (v & IDX_MASK)) { set(b + 1, get(b)); --b; }
         set(b + 1, v);
     }
     uint32_t nu = 0;
     for (uint32_t a = 0; a < nt; ++a)
-        if (nu == 0 || get(a) != get(nu - 1)) { set(nu, get(a)); ++nu; }
+        if (nu == 0 || (get(a) & IDX_MASK) != (get(nu - 1) & IDX_MASK)) { set(nu, get(a)); ++nu; }
     return nu;
 }
 
@@ -875,9 +882,9 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             const int64_t e = (int64_t)m * n + p;
             const uint32_t word = pq[e];
             const uint32_t q = word & IDX_MASK;
-            const bool plo = p < q;
+            const bool plo = (word & IS_I) != 0u;  // p is the contact's i (the lower file index)
             const double4 Xq = pos[q], Vq = vel[q], Wq = omg[q];
-            // canonical orientation: i = min(p, q), j = max(p, q)
+            // canonical orientation: i is the lower file index, j the higher
             const d3 ki = plo ? kp : mk3(Xq.x, Xq.y, Xq.z);
             const d3 kj = plo ? mk3(Xq.x, Xq.y, Xq.z) : kp;
             const d3 vi = plo ? vp : mk3(Vq.x, Vq.y, Vq.z);
@@ -1152,11 +1159,12 @@ k_debug_cells(const Params P, const double4* __restrict__ pos, int32_t* kid, uin
 template <bool EMIT>
 __global__ void __launch_bounds__(TPB)
 k_cp_list(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx,
-          const uint32_t* __restrict__ bstart, uint32_t* __restrict__ cnt,
-          const uint32_t* __restrict__ off, int32_t* __restrict__ pairs) {
+          const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ eid,
+          uint32_t* __restrict__ cnt, const uint32_t* __restrict__ off, int32_t* __restrict__ pairs) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (i >= P.n) return;
     const double4 x = pos[i];
+    const uint32_t ei = eid[i];
     uint32_t w[8];
     window_of(P.sc.grid, mk3(x.x, x.y, x.z), w);
     uint32_t c = 0;
@@ -1164,11 +1172,11 @@ k_cp_list(const Params P, const double4* __restrict__ pos, const uint32_t* __res
     for (int wi = 0; wi < 8; ++wi) {
         const uint32_t b = w[wi];
         for (uint32_t k = bstart[b]; k < bstart[b + 1]; ++k) {
-            const uint32_t j = sidx[k];
-            if ((uint32_t)i < j) {
+            const uint32_t ej = eid[sidx[k]];
+            if (ei < ej) {  // 1-based file indices, as the reference's cp_list holds them
                 if (EMIT) {
-                    pairs[2 * ((int64_t)base + c)] = (int32_t)i + 1;
-                    pairs[2 * ((int64_t)base + c) + 1] = (int32_t)j + 1;
+                    pairs[2 * ((int64_t)base + c)] = (int32_t)ei + 1;
+                    pairs[2 * ((int64_t)base + c) + 1] = (int32_t)ej + 1;
                 }
                 ++c;
             }
@@ -1177,13 +1185,13 @@ k_cp_list(const Params P, const double4* __restrict__ pos, const uint32_t* __res
     if (!EMIT) cnt[i] = c;
 }
 
-// Active contacts owned by the lower index, in (i, j) order.
+// Active contacts, one record per pair (from the row of its i), with 1-based file indices.
 template <bool EMIT>
 __global__ void __launch_bounds__(TPB)
 k_lo_rows(const Params P, const uint8_t* __restrict__ sel, const uint32_t* __restrict__ rcnt0,
           const uint32_t* __restrict__ rcnt1, const uint32_t* __restrict__ pq0,
           const uint32_t* __restrict__ pq1, const double* __restrict__ st0,
-          const double* __restrict__ st1, const double4* __restrict__ pos, uint32_t* __restrict__ cnt,
+          const double* __restrict__ st1, const uint32_t* __restrict__ eid, uint32_t* __restrict__ cnt,
           const uint32_t* __restrict__ off, dem_contact_t* __restrict__ out,
           uint8_t* __restrict__ bonded) {
     const int64_t p64 = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -1201,11 +1209,11 @@ k_lo_rows(const Params P, const uint8_t* __restrict__ sel, const uint32_t* __res
         const int64_t e = (int64_t)m * n + p;
         const uint32_t word = pq[e];
         const uint32_t q = word & IDX_MASK;
-        if (q <= p || (word & BROKEN)) continue;
+        if (!(word & IS_I) || (word & BROKEN)) continue;
         if (EMIT) {
             dem_contact_t& o = out[base + c];
             const bool bd = (word & BONDED) != 0;
-            o.i = (int32_t)p + 1; o.j = (int32_t)q + 1; o.midi = 1; o.midj = 1;
+            o.i = (int32_t)eid[p] + 1; o.j = (int32_t)eid[q] + 1; o.midi = 1; o.midj = 1;
             o.k[0] = st[S_KX * cs + e]; o.k[1] = st[S_KY * cs + e]; o.k[2] = st[S_KZ * cs + e];
             o.F[0] = st[S_FX * cs + e]; o.F[1] = st[S_FY * cs + e]; o.F[2] = st[S_FZ * cs + e];
             o.ti[0] = bd ? st[S_TIX * cs + e] : 0.0; o.ti[1] = bd ? st[S_TIY * cs + e] : 0.0;
@@ -1366,43 +1374,43 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     const dim3 g = grid_for(P.n, TPB);
     if (count_candidates && !dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
-                                              d.sidx, d.bstart, d.ctr));
+                                              d.sidx, d.bstart, d.eid, d.ctr));
     if (dense) {
         if (init) {
             LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, SEG>), g, dim3(TPB), 0,
                                                  s->stream, P, d.sidx, d.skey, d.pos[c], d.spos32,
-                                                 d.bstart, d.r0, d.tcnt, d.touch, d.ctr, diag));
+                                                 d.bstart, d.r0, d.eid, d.tcnt, d.touch, d.ctr, diag));
         } else {
             LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, SEG>), g, dim3(TPB), 0,
                                                   s->stream, P, d.sidx, d.skey, d.pos[c], d.spos32,
-                                                  d.bstart, d.r0, d.tcnt, d.touch, d.ctr, diag));
+                                                  d.bstart, d.r0, d.eid, d.tcnt, d.touch, d.ctr, diag));
         }
     } else if (s->slot_owned) {
         if (init) {
             LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_slot<true, false>), g, dim3(TPB), 0, s->stream,
-                                                 P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                                 d.touch, d.ctr));
+                                                 P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.eid,
+                                                 d.tcnt, d.touch, d.ctr));
         } else if (mixed) {
             LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_slot<false, true>), g, dim3(TPB), 0, s->stream,
-                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                                  d.touch, d.ctr));
+                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.eid,
+                                                  d.tcnt, d.touch, d.ctr));
         } else {
             LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_slot<false, false>), g, dim3(TPB), 0, s->stream,
-                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                                  d.touch, d.ctr));
+                                                  P, d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.eid,
+                                                  d.tcnt, d.touch, d.ctr));
         }
     } else if (init) {
         LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad<true, false>), g, dim3(TPB), 0, s->stream, P,
-                                             d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                             d.touch, d.ctr));
+                                             d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.eid,
+                                             d.tcnt, d.touch, d.ctr));
     } else if (mixed) {
         LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad<false, true>), g, dim3(TPB), 0, s->stream, P,
-                                              d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                              d.touch, d.ctr));
+                                              d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.eid,
+                                              d.tcnt, d.touch, d.ctr));
     } else {
         LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad<false, false>), g, dim3(TPB), 0, s->stream, P,
-                                              d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.tcnt,
-                                              d.touch, d.ctr));
+                                              d.sidx, d.spos, d.spos32, d.bstart, d.r0, d.eid,
+                                              d.tcnt, d.touch, d.ctr));
     }
     const size_t lds = 0;
     const dim3 gm = grid_for(P.n, MERGE_TPB);
@@ -1422,7 +1430,7 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
         // the cp_list length needs the reference's bucket sort: rebuild it after the merge
         if ((e = sort_impl<false>(s, false, false)) != hipSuccess) return e;
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
-                                              d.sidx, d.bstart, d.ctr));
+                                              d.sidx, d.bstart, d.eid, d.ctr));
     }
     return hipSuccess;
 }
@@ -1475,7 +1483,7 @@ hipError_t run_debug_cp_list(dem_scene* s, int32_t** pairs_dev, int64_t* count) 
     hipError_t e = sort_particles(s, false, false);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_cp_list<false>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P,
-                       d.pos[s->cur], d.sidx, d.bstart, d.scratch_u32, nullptr, nullptr);
+                       d.pos[s->cur], d.sidx, d.bstart, d.eid, d.scratch_u32, nullptr, nullptr);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     e = launch_scan(d.scan, d.scratch_u32, d.scratch2_u32, P.n, false, d.total, s->stream);
     if (e != hipSuccess) return e;
@@ -1486,7 +1494,7 @@ hipError_t run_debug_cp_list(dem_scene* s, int32_t** pairs_dev, int64_t* count) 
     int32_t* pairs = nullptr;
     if ((e = hipMalloc(&pairs, 2 * sizeof(int32_t) * (total ? total : 1))) != hipSuccess) return e;
     hipLaunchKernelGGL((k_cp_list<true>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P,
-                       d.pos[s->cur], d.sidx, d.bstart, nullptr, d.scratch2_u32, pairs);
+                       d.pos[s->cur], d.sidx, d.bstart, d.eid, nullptr, d.scratch2_u32, pairs);
     if ((e = hipGetLastError()) != hipSuccess) { hipFree(pairs); return e; }
     *pairs_dev = pairs;
     *count = (int64_t)total;
@@ -1498,7 +1506,7 @@ hipError_t run_read_contacts(dem_scene* s, dem_contact_t** out_dev, uint8_t** bo
     DevState& d = s->d;
     const Params& P = s->P;
     hipLaunchKernelGGL((k_lo_rows<false>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P, d.sel,
-                       d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.pos[s->cur],
+                       d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.eid,
                        d.scratch_u32, nullptr, nullptr, nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -1513,7 +1521,7 @@ hipError_t run_read_contacts(dem_scene* s, dem_contact_t** out_dev, uint8_t** bo
     if ((e = hipMalloc(&out, sizeof(dem_contact_t) * (total ? total : 1))) != hipSuccess) return e;
     if ((e = hipMalloc(&bd, total ? total : 1)) != hipSuccess) { hipFree(out); return e; }
     hipLaunchKernelGGL((k_lo_rows<true>), grid_for(P.n, TPB), dim3(TPB), 0, s->stream, P, d.sel,
-                       d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.pos[s->cur],
+                       d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.eid,
                        nullptr, d.scratch2_u32, out, bd);
     if ((e = hipGetLastError()) != hipSuccess) { hipFree(out); hipFree(bd); return e; }
     *out_dev = out;

@@ -26,8 +26,8 @@ from . import scenes
 # HBM bytes per grain, per contact slot, of one scene (device arrays of abi.hip allocate_scene):
 # double-buffered pos/vel/omg/quat, a/dw/F/T outputs, r0, wall flags; sort arrays (bucket, rank,
 # FP32 position, two orders, sorted bucket, FP64 and FP32 sorted positions, about two bucket
-# counters); touch count, row selector, row counts; scratch; scene id
-BYTES_PER_GRAIN = (2 * 4 * 32 + 4 * 32 + 8 + 4 + (4 + 4 + 16 + 2 * 4 + 4 + 32 + 16 + 8)
+# counters); file index; touch count, row selector, row counts; scratch; scene id
+BYTES_PER_GRAIN = (2 * 4 * 32 + 4 * 32 + 8 + 4 + (4 + 4 + 16 + 2 * 4 + 4 + 32 + 16 + 8) + 4
                    + (4 + 1 + 2 * 4) + 2 * 4 + 2)
 BYTES_PER_SLOT = 2 * (4 + 15 * 8) + 4
 
