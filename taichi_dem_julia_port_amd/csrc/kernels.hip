@@ -28,6 +28,16 @@ namespace {
 constexpr int TPB = 256;
 constexpr int MERGE_TPB = 128;
 
+// Blocks are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its
+// own L2.  Renumbering so that XCD x runs the x-th contiguous eighth of the logical blocks keeps
+// the neighbour cells that adjacent blocks share inside one L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    constexpr uint32_t NXCD = 8;
+    const uint32_t x = b % NXCD, local = b / NXCD;
+    const uint32_t q = nb / NXCD, r = nb % NXCD;
+    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
+}
+
 __device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
 
 // ------------------------------------------------------------------------------------------
@@ -237,16 +247,6 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
     }
     wave_add(ctr_at(ctr, CT_TOUCH), found);
     if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
-}
-
-// Blocks are dispatched to the 8 XCDs round-robin (block b runs on XCD b % 8), and each XCD has its
-// own L2.  Renumbering so that XCD x runs the x-th contiguous eighth of the logical blocks keeps
-// the neighbour cells that adjacent blocks share inside one L2.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-    constexpr uint32_t NXCD = 8;
-    const uint32_t x = b % NXCD, local = b / NXCD;
-    const uint32_t q = nb / NXCD, r = nb % NXCD;
-    return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
 }
 
 // Dense-grid broad phase.  Cells have side >= the largest contact distance, so every touching
@@ -684,7 +684,7 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
         unsigned long long* ctr, int diag) {
     __shared__ uint32_t lt[MERGE_LDS * MERGE_TPB];  // touching partners
     __shared__ uint32_t lw[MERGE_LDS * MERGE_TPB];  // old-row words
-    const int64_t p64 = (int64_t)blockIdx.x * MERGE_TPB + threadIdx.x;
+    const int64_t p64 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * MERGE_TPB + threadIdx.x;
     const int64_t n = P.n;
     const int K = P.K;
     uint64_t created = 0, deact = 0, entries = 0;
@@ -862,7 +862,8 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
         double* __restrict__ st1, uint32_t* __restrict__ wact, double4* __restrict__ wshear,
         unsigned long long* ctr, int write_forces) {
-    const int64_t p64 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    // XCD-contiguous blocks: partners are spatial neighbours, so their state is in this XCD's L2
+    const int64_t p64 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     const int64_t n = P.n;
     uint64_t broken = 0, nwallc = 0, fatal = 0, active = 0, active_b = 0;
     if (p64 < n) {
@@ -878,12 +879,28 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         double* st = b ? st1 : st0;
         const uint32_t cnt = (b ? rcnt1 : rcnt0)[p];
         const int64_t cs = (int64_t)P.K * n;
+        // software pipeline over the row: entry m+1's partner state and entry m+2's word are
+        // loaded while entry m is evaluated
+        uint32_t w_cur = 0u, w_nxt = 0u;
+        double4 Xq_c = make_double4(0.0, 0.0, 0.0, 0.0), Vq_c = Xq_c, Wq_c = Xq_c;
+        if (cnt > 0) {
+            w_cur = pq[p];
+            if (cnt > 1) w_nxt = pq[n + p];
+            const uint32_t q0 = w_cur & IDX_MASK;
+            Xq_c = pos[q0]; Vq_c = vel[q0]; Wq_c = omg[q0];
+        }
         for (uint32_t m = 0; m < cnt; ++m) {
             const int64_t e = (int64_t)m * n + p;
-            const uint32_t word = pq[e];
+            const uint32_t word = w_cur;
+            const double4 Xq = Xq_c, Vq = Vq_c, Wq = Wq_c;
+            if (m + 1 < cnt) {
+                const uint32_t qn = w_nxt & IDX_MASK;
+                Xq_c = pos[qn]; Vq_c = vel[qn]; Wq_c = omg[qn];
+                w_cur = w_nxt;
+                if (m + 2 < cnt) w_nxt = pq[(int64_t)(m + 2) * n + p];
+            }
             const uint32_t q = word & IDX_MASK;
             const bool plo = (word & IS_I) != 0u;  // p is the contact's i (the lower file index)
-            const double4 Xq = pos[q], Vq = vel[q], Wq = omg[q];
             // canonical orientation: i is the lower file index, j the higher
             const d3 ki = plo ? kp : mk3(Xq.x, Xq.y, Xq.z);
             const d3 kj = plo ? mk3(Xq.x, Xq.y, Xq.z) : kp;
