@@ -252,6 +252,7 @@ def run_serving(args):
                                 f"uniform 2k-50k, epoch {args.epoch} steps"),
                    "precision": args.precision, "particle_steps_requested": work},
         "batched": out_b, "repacks": srv.repacks,
+        "repack_s": round(srv.repack_s, 3), "stepping_s": round(srv.step_s, 3),
         "one_at_a_time_fp64": out_0,
         "speedup_scenes_per_s": round(out_b["scenes_per_s"] / out_0["scenes_per_s"], 2),
     }

@@ -287,31 +287,49 @@ void spatial_order(dem_scene* s, const dem_grain_in_t* g) {
     const int64_t n = s->n;
     s->perm.resize(n);
     s->iperm.resize(n);
-    std::vector<uint64_t> key(n);
     const int nseg = s->batched ? s->nseg : 1;
+    std::vector<uint64_t> key;
+    std::vector<uint32_t> cnt;
     for (int k = 0; k < nseg; ++k) {
         const int64_t a = s->batched ? s->seg_off[k] : 0, b = s->batched ? s->seg_off[k + 1] : n;
         const SceneP& S = s->batched ? s->seg_host[k] : s->P.sc;
-        for (int64_t i = a; i < b; ++i) {
-            int64_t c[3];
-            for (int ax = 0; ax < 3; ++ax) {
-                double v;
-                int64_t hi;
-                if (S.dense.cells > 0) {
-                    v = (g[i].k[ax] - S.dense.origin[ax]) * S.dense.inv_cell;
-                    hi = std::min<int64_t>(S.dense.dims[ax] - 1, (1 << 21) - 1);
-                } else {
-                    v = (g[i].k[ax] - S.grid.dmin[ax]) / S.grid.cs;
-                    hi = (1 << 21) - 1;
+        if (S.dense.cells > 0) {
+            // counting sort by dense cell (stable: file order within a cell)
+            const DenseGrid& G = S.dense;
+            key.assign(b - a, 0);
+            cnt.assign((size_t)G.cells + 1, 0u);
+            for (int64_t i = a; i < b; ++i) {
+                uint64_t c[3];
+                for (int ax = 0; ax < 3; ++ax) {
+                    const double v = (g[i].k[ax] - G.origin[ax]) * G.inv_cell;
+                    int64_t q = std::isfinite(v) ? (int64_t)std::floor(std::max(-1.0, std::min(v, 4.0e9))) : 0;
+                    q = q < 0 ? 0 : (q >= G.dims[ax] ? G.dims[ax] - 1 : q);
+                    c[ax] = (uint64_t)q;
                 }
-                int64_t q = std::isfinite(v) ? (int64_t)std::floor(std::max(-1.0, std::min(v, 4.0e6))) : 0;
-                c[ax] = q < 0 ? 0 : (q > hi ? hi : q);
+                key[i - a] = c[0] + (uint64_t)G.dims[0] * (c[1] + (uint64_t)G.dims[1] * c[2]);
+                ++cnt[key[i - a] + 1];
             }
-            key[i] = ((uint64_t)c[2] << 42) | ((uint64_t)c[1] << 21) | (uint64_t)c[0];
-            s->perm[i] = (uint32_t)i;
+            for (size_t c = 1; c < cnt.size(); ++c) cnt[c] += cnt[c - 1];
+            for (int64_t i = a; i < b; ++i) s->perm[a + cnt[key[i - a]]++] = (uint32_t)i;
+        } else {
+            // reference grid: sort by cell coordinates, x fastest
+            key.assign(b - a, 0);
+            std::vector<uint32_t> idx(b - a);
+            for (int64_t i = a; i < b; ++i) {
+                uint64_t c[3];
+                for (int ax = 0; ax < 3; ++ax) {
+                    const double v = (g[i].k[ax] - S.grid.dmin[ax]) / S.grid.cs;
+                    int64_t q = std::isfinite(v) ? (int64_t)std::floor(std::max(-1.0, std::min(v, 4.0e6))) : 0;
+                    q = q < 0 ? 0 : (q > (1 << 21) - 1 ? (1 << 21) - 1 : q);
+                    c[ax] = (uint64_t)q;
+                }
+                key[i - a] = (c[2] << 42) | (c[1] << 21) | c[0];
+                idx[i - a] = (uint32_t)i;
+            }
+            std::stable_sort(idx.begin(), idx.end(),
+                             [&](uint32_t x, uint32_t y) { return key[x - a] < key[y - a]; });
+            std::copy(idx.begin(), idx.end(), s->perm.begin() + a);
         }
-        std::stable_sort(s->perm.begin() + a, s->perm.begin() + b,
-                         [&](uint32_t x, uint32_t y) { return key[x] < key[y]; });
     }
     for (int64_t u = 0; u < n; ++u) s->iperm[s->perm[u]] = (uint32_t)u;
 }

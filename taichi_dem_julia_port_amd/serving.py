@@ -133,6 +133,8 @@ class Server:
         self.device = device
         self.keep = keep_results
         self.repacks = 0
+        self.repack_s = 0.0  # wall time spent re-packing (batch creation, state transfer, bond pass)
+        self.step_s = 0.0    # wall time spent stepping
 
     def _slots(self, live: List[_Live]) -> int:
         return max(min(128, max(32, 2 * l.req.build()[0].max_coordinate_number)) for l in live)
@@ -172,6 +174,7 @@ class Server:
                 time.sleep(max(pending[0].arrival - now, 0.0))
                 continue
             if new or dirty or batch is None:
+                tr = time.perf_counter()
                 members = live + new
                 fresh = {id(m) for m in new}
                 nb = Batch([m.req.build() for m in members], precision=self.precision,
@@ -189,8 +192,11 @@ class Server:
                 live = members
                 dirty = False
                 self.repacks += 1
+                self.repack_s += time.perf_counter() - tr
             k = min(self.epoch, min(m.remaining for m in live))
+            ts = time.perf_counter()
             batch.step(k)
+            self.step_s += time.perf_counter() - ts
             for m in live:
                 m.remaining -= k
             finished = [m for m in live if m.remaining == 0]
