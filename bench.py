@@ -142,10 +142,10 @@ def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
 def precision_error(cfg, g, w, bonded, steps=1000):
     """Mixed precision against FP64 after `steps` steps from the same initial state (untimed):
     the largest position difference relative to the largest |k| and to the largest radius.
-    Beside it, the same figures for FP64 against FP64 started from x coordinates moved by one ulp
-    and by 1e-9 relative: the divergence the dynamics alone produce from a small difference (a
-    contact that forms or breaks one step apart changes a grain's path by far more than the
-    difference that decided it)."""
+    Beside it, the same figures for FP64 against FP64 started from x coordinates moved by one ulp,
+    by 1e-9 and by 1e-7 relative (about one FP32 rounding): the divergence the dynamics alone
+    produce from a small difference (a contact that forms or breaks one step apart changes a
+    grain's path by far more than the difference that decided it)."""
     import numpy as np
     from taichi_dem_julia_port_amd.executor import Batch, Scene
 
@@ -169,7 +169,7 @@ def precision_error(cfg, g, w, bonded, steps=1000):
         return out
 
     a, b = run("fp64"), run("mixed")
-    c, e = run("fp64", 0.0), run("fp64", 1e-9)
+    c, e, f = run("fp64", 0.0), run("fp64", 1e-9), run("fp64", 1e-7)
     scale_k = max(float(np.abs(a["k"]).max()), 1e-300)
     scale_r = max(float(a["r"].max()), 1e-300)
 
@@ -181,7 +181,8 @@ def precision_error(cfg, g, w, bonded, steps=1000):
                 "p99_abs_m": float(np.percentile(per, 99)),
                 "grains_above_1e-6_of_max_position": int((per > 1e-6 * scale_k).sum())}
     return {"steps": steps, "mixed_vs_fp64": err(a, b), "fp64_one_ulp_perturbation": err(a, c),
-            "fp64_1e-9_relative_perturbation": err(a, e)}
+            "fp64_1e-9_relative_perturbation": err(a, e),
+            "fp64_1e-7_relative_perturbation": err(a, f)}
 
 
 def one_at_a_time(members, precision, steps, sample=16, seed=0):

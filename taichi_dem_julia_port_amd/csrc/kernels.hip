@@ -794,7 +794,13 @@ __device__ __forceinline__ f3 fscl(f3 a, float s) { return mkf(a.x * s, a.y * s,
 __device__ __forceinline__ f3 fcross(f3 a, f3 b) {
     return mkf(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
 }
-__device__ __forceinline__ float fnorm(f3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Reciprocal and square root from the hardware approximations (about 1 ulp): the mixed path's
+// accuracy is set by FP32 rounding anyway, and an IEEE-exact FP32 division or square root is a
+// sequence of about ten instructions.
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float fnorm(f3 a) { return fsqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ f3 fneg(f3 a) { return mkf(-a.x, -a.y, -a.z); }
 
 struct fm3 { float a[3][3]; };
 __device__ __forceinline__ f3 fmul(const fm3& R, f3 u) {
@@ -823,7 +829,7 @@ __device__ __forceinline__ fm3 contact_frame_f32(f3 a, float tol) {
         return R;
     }
     float K[3][3] = {{0.0f, -v.z, v.y}, {v.z, 0.0f, -v.x}, {-v.y, v.x, 0.0f}};
-    const float f = (1.0f - c) / (s * s);
+    const float f = (1.0f - c) * frcp(s * s);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -851,8 +857,11 @@ struct CSum3 {
     }
 };
 
+// The single-scene mixed kernel carries both contact laws in two precisions; left alone the
+// compiler gives it more than 256 registers and one wave per SIMD, so it is held to two waves per
+// SIMD. The other variants fit in 256 and keep the schedule the compiler picks without the bound.
 template <bool MIXED, bool SEG>
-__global__ void __launch_bounds__(TPB)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(MIXED && !SEG ? 2 : 1)))
 k_force(const Params P, const double4* __restrict__ pos, const double4* __restrict__ vel,
         const double4* __restrict__ omg, const double4* __restrict__ quat,
         double4* __restrict__ pos_n, double4* __restrict__ vel_n, double4* __restrict__ omg_n,
@@ -912,6 +921,70 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             const double mi = plo ? Vv.w : Vq.w, mj = plo ? Vq.w : Vv.w;
             const double Li = norm(kj - ki);
             if (word & BONDED) {
+              if (MIXED) {
+                // EBPM with the frame, stiffnesses, increments and stresses in FP32, added to the
+                // FP64 force and moment state.  Relative motion comes from FP64 velocity
+                // differences (bonded grains share most of their motion).  A stress within FP32
+                // rounding of its strength, or a normal near -x, takes the FP64 law below.
+                const f3 d32 = tof(kj - ki);
+                const f3 a32 = fscl(d32, frcp((float)Li));
+                if (a32.x >= -0.99f) {
+                    const fm3 R32 = contact_frame_f32(a32, 1e-6f);
+                    const float dtf = (float)dt;
+                    const f3 dd = fmul(R32, fscl(tof(vi - vj), dtf));    // d_i - d_j
+                    const f3 dth = fmul(R32, fscl(tof(wi - wj), dtf));   // theta_i - theta_j
+                    const f3 thi = fmul(R32, fscl(tof(wi), dtf)), thj = fmul(R32, fscl(tof(wj), dtf));
+                    const SurfaceP& sf = S.surf[0][0];
+                    const float rb = (float)sf.rho * (float)(ri < rj ? ri : rj);
+                    const float iL = frcp((float)Li);
+                    const float E = (float)sf.E, nu = (float)sf.nu;
+                    const float rb2 = rb * rb;
+                    const float Ib = rb2 * rb2 * (float)(M_PI / 4.0);
+                    const float phi = (20.0f / 3.0f) * rb2 * iL * iL * (1.0f + nu);
+                    const float Ab = rb2 * (float)M_PI;
+                    const float i1p = frcp(1.0f + phi);
+                    const float EI = E * Ib;
+                    const float k1 = E * Ab * iL;
+                    const float k2 = 12.0f * EI * iL * iL * iL * i1p;
+                    const float k3 = 6.0f * EI * iL * iL * i1p;
+                    const float k4 = EI * iL * frcp(1.0f + nu);
+                    const float k5 = EI * (4.0f + phi) * iL * i1p;
+                    const float k6 = EI * (2.0f - phi) * iL * i1p;
+                    const f3 dF = mkf(k1 * dd.x, k2 * dd.y + k3 * (thi.z + thj.z), k2 * dd.z - k3 * (thi.y + thj.y));
+                    const f3 dTi = mkf(k4 * dth.x, -k3 * dd.z + k5 * thi.y + k6 * thj.y, k3 * dd.y + k5 * thi.z + k6 * thj.z);
+                    const f3 dTj = mkf(-k4 * dth.x, -k3 * dd.z + k6 * thi.y + k5 * thj.y, k3 * dd.y + k6 * thi.z + k5 * thj.z);
+                    const d3 Fi = mk3(st[S_FX * cs + e], st[S_FY * cs + e], st[S_FZ * cs + e]) + tod(dF);
+                    const d3 Ti = mk3(st[S_TIX * cs + e], st[S_TIY * cs + e], st[S_TIZ * cs + e]) + tod(dTi);
+                    const d3 Tj = mk3(st[S_TJX * cs + e], st[S_TJY * cs + e], st[S_TJZ * cs + e]) + tod(dTj);
+                    const f3 F32 = tof(Fi), Ti32 = tof(Ti), Tj32 = tof(Tj);
+                    const float iAb = frcp(Ab), rbIb = rb * frcp(Ib);
+                    const float sci = -F32.x * iAb - rbIb * fsqrt(Ti32.y * Ti32.y + Ti32.z * Ti32.z);
+                    const float scj = -F32.x * iAb - rbIb * fsqrt(Tj32.y * Tj32.y + Tj32.z * Tj32.z);
+                    const float sc = -fminf(sci, scj), stt = fmaxf(sci, scj);
+                    const float ss = fabsf(Ti32.x) * rbIb * 0.5f + (4.0f / 3.0f) * iAb * fsqrt(F32.y * F32.y + F32.z * F32.z);
+                    const float SC = (float)sf.sc, ST = (float)sf.st, SS = (float)sf.ss;
+                    auto close = [](float a, float b) { return fabsf(a - b) <= 1e-4f * fmaxf(fabsf(a), fabsf(b)); };
+                    if (!(close(sc, SC) || close(stt, ST) || close(ss, SS))) {
+                        if (sc >= SC || stt >= ST || ss >= SS) {
+                            pq[e] = word | BROKEN;  // both copies take the same decision
+                            if (plo) {
+                                ++broken;
+                                if (SEG) atomicAdd(&P.seg_broken[P.sid[p]], 1ull);
+                            }
+                            continue;
+                        }
+                        if (plo) { FH.add(fmul_t(R32, fneg(F32))); TH.add(fmul_t(R32, fneg(Ti32))); }
+                        else { FH.add(fmul_t(R32, F32)); TH.add(fmul_t(R32, fneg(Tj32))); }
+                        const d3 kc = (ki + kj) * 0.5;
+                        st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                        st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
+                        st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
+                        st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
+                        if (plo) { ++active; ++active_b; }
+                        continue;
+                    }
+                }
+              }
                 // EBPM bond (kernels.jl:170-239)
                 const m3 R = contact_frame(normalize(kj - ki), S.tol);
                 const d3 kc = (ki + kj) * 0.5;
@@ -973,11 +1046,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
               if (MIXED) {
                 // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance.
                 // Contacts it cannot decide like FP64 fall through to the FP64 law below.
-                // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance
                 const double gap = Li - ri - rj;
                 const double dn = fabs(gap);
                 const f3 d32 = tof(kj - ki);
-                const f3 a32 = fscl(d32, 1.0f / (float)Li);
+                const f3 a32 = fscl(d32, frcp((float)Li));
                 // near a = -x the reference's frame is ill-conditioned (its rotation about the
                 // normal follows the normal's tiny transverse components): FP64 there
                 bool fp64 = a32.x < -0.99f;
@@ -990,13 +1062,13 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 // grains' velocities would lose the small relative part
                 const f3 dv = tof(vj - vi);
                 const f3 vc = fmul(R32, fadd(fsub(fcross(tof(wj), kjp), fcross(tof(wi), kip)), dv));
-                const float Rs = 1.0f / (1.0f / (float)ri + 1.0f / (float)rj);
-                const float ms = 1.0f / (1.0f / (float)mi + 1.0f / (float)mj);
-                const float srd = sqrtf(Rs * (float)dn);
+                const float Rs = frcp(frcp((float)ri) + frcp((float)rj));
+                const float ms = frcp(frcp((float)mi) + frcp((float)mj));
+                const float srd = fsqrt(Rs * (float)dn);
                 const float kn = (4.0f / 3.0f) * S.ys_pp * srd;
                 const float kt = 8.0f * S.gs_pp * srd;
-                const float gn = -2.0f * S.beta_pp * sqrtf((5.0f / 6.0f) * (2.0f * S.ys_pp * srd) * ms);
-                const float gt = -2.0f * S.beta_pp * sqrtf((5.0f / 6.0f) * kt * ms);
+                const float gn = -2.0f * S.beta_pp * fsqrt((5.0f / 6.0f) * (2.0f * S.ys_pp * srd) * ms);
+                const float gt = -2.0f * S.beta_pp * fsqrt((5.0f / 6.0f) * kt * ms);
                 // the tangential spring accumulates in FP64: v dt increments vanish in FP32
                 d3 sv = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
                 sv = sv + mk3((double)vc.x * 0.0, (double)vc.y * dt, (double)vc.z * dt);
@@ -1012,10 +1084,11 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 if (fp64) {
                     F2 = F3 = 0.0f;
                 } else if (nFt >= thr) {
-                    const float ratio = S.mu_pp * F1 / nFt;
+                    const float ratio = S.mu_pp * F1 * frcp(nFt);
                     F2 = ratio * Ft.y;
                     F3 = ratio * Ft.z;
-                    sv = mk3(sv.x, (double)(F2 / kt), (double)(F3 / kt));
+                    const float ikt = frcp(kt);
+                    sv = mk3(sv.x, (double)(F2 * ikt), (double)(F3 * ikt));
                 } else {
                     F2 = Ft.y - gt * vc.y;
                     F3 = Ft.z - gt * vc.z;

@@ -60,3 +60,24 @@ def test_mixed_1000_steps_position_error_vs_fp64():
     assert ca["contacts"] > 0 and cb["contacts"] > 0
     rel = np.linalg.norm(ga["k"] - gb["k"], axis=1) / np.linalg.norm(ga["k"], axis=1)
     assert rel.max() <= 1e-6, rel.max()
+
+
+def test_mixed_bonded_cube_1000_steps_vs_fp64():
+    """Bonds in the mixed path (FP32 frame, stiffness and increments on FP64 state): the
+    900-grain bonded cube (no bond breaks in 1000 steps) against the FP64 engine."""
+    print("[case] mixed vs fp64 bonded cube 1000 steps", flush=True)
+    from taichi_dem_julia_port_amd import scenes
+    cfg, g, w = scenes.cube_small()
+    with _scene(cfg, g, w, precision="fp64") as a, _scene(cfg, g, w, precision="mixed") as b:
+        a.initialize()
+        b.initialize()
+        a.step(1000)
+        b.step(1000)
+        ga, gb = a.read_grains(), b.read_grains()
+        ca, cb = a.counters(), b.counters()
+        (c1, b1), (c2, b2) = a.read_contacts(), b.read_contacts()
+    assert ca["bonded"] == cb["bonded"] > 0
+    assert np.array_equal(pair_keys(c1), pair_keys(c2)) and np.array_equal(b1, b2)
+    rel = np.linalg.norm(ga["k"] - gb["k"], axis=1) / np.linalg.norm(ga["k"], axis=1)
+    assert rel.max() <= 1e-6, rel.max()
+    assert vec_rel_err(c2["F"], c1["F"]) <= 1e-3
