@@ -53,7 +53,7 @@ void free_scene(dem_scene* s) {
                     d.acc, d.dw, d.ftot, d.ttot, d.r0, d.wact, d.wshear, d.hid, d.rank, d.x32, d.bcnt,
                     d.bstart, d.sidxb[0], d.sidxb[1], d.skey, d.eid, d.spos, d.spos32, d.tcnt, d.touch, d.sel,
                     d.rcnt[0], d.rcnt[1],
-                    d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, d.scan.status, d.scan.counters,
+                    d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, d.scan.status, d.scan.counters, d.scan.tagg,
                     d.scratch_u32, d.scratch2_u32, d.total, d.seg, d.sid, d.seg_broken};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -361,7 +361,7 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
     ALLOC(d.ctr, CTR_WORDS);
     const int64_t max_tiles = scan_tiles_max(std::max<int64_t>(nkeys + 1, n + 1));
-    ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2);
+    ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2); ALLOC(d.scan.tagg, max_tiles * TAGG_COPIES);
     d.scan.max_tiles = max_tiles;
     ALLOC(d.scratch_u32, n + 1); ALLOC(d.scratch2_u32, n + 1); ALLOC(d.total, 1);
     if (s->batched) { ALLOC(d.seg, s->nseg); ALLOC(d.sid, n); ALLOC(d.seg_broken, s->nseg); }

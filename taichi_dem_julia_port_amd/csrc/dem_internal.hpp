@@ -100,16 +100,26 @@ __device__ __forceinline__ const SceneP& scene_of(const Params& P, uint32_t p) {
     else return P.sc;
 }
 
+// Copies of each tile total of a counting sort's histogram: block b of the histogram pass adds
+// into copy b % TAGG_COPIES, which on a part that deals blocks round-robin to its 8 XCDs keeps
+// blocks of different XCDs off one address.
+constexpr int TAGG_COPIES = 8;
+
 struct ScanWorkspace {
     uint64_t* status = nullptr;
     uint32_t* counters = nullptr;
+    uint32_t* tagg = nullptr;  // [tile][TAGG_COPIES] histogram tile totals (zero between sorts)
     int64_t max_tiles = 0;
 };
 
 int64_t scan_tiles(int64_t n);
 int64_t scan_tiles_max(int64_t n);  // the most tiles any scan of at most n values takes
+int scan_tile_shift(int64_t n);     // log2 of the tile length a scan of n values uses
 hipError_t launch_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n, bool zero_input,
                        uint64_t* total, hipStream_t stream);
+// Exclusive scan of n counts whose per-tile totals are in ws.tagg; clears the counts.
+hipError_t launch_tile_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n,
+                            hipStream_t stream);
 
 // Device arrays of one scene.
 struct DevState {

@@ -51,7 +51,8 @@ template <bool DENSE, bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx_prev,
        uint32_t* __restrict__ hid, uint32_t* __restrict__ rank, float4* __restrict__ x32,
-       uint32_t* __restrict__ bcnt, unsigned long long* ctr, int reset_counters, int init) {
+       uint32_t* __restrict__ bcnt, uint32_t* __restrict__ tagg, int tile_shift,
+       unsigned long long* ctr, int reset_counters, int init) {
     if (reset_counters && blockIdx.x == 0)
         for (int i = threadIdx.x; i < CTR_COPIES * CT_NSTEP; i += TPB)
             ctr[(i / CT_NSTEP) * CTR_STRIDE + i % CT_NSTEP] = 0ull;
@@ -81,6 +82,38 @@ k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restri
     uint32_t base = 0u;
     if (head) base = atomicAdd(&bcnt[b], end - (uint32_t)lane);
     base = __shfl(base, live ? h : lane, 64);
+    // The same run aggregation over scan tiles, so the scan of bcnt needs no look-back.  A wave's
+    // first tile run is combined with the other waves' in LDS (a block's grains are adjacent in
+    // the previous order, so they nearly always share one tile); later runs add directly.
+    __shared__ uint32_t s_tile[TPB / 64], s_tcnt[TPB / 64];
+    const int wave = threadIdx.x >> 6;
+    uint32_t* const tcopy = tagg + (blockIdx.x % TAGG_COPIES);
+    const uint32_t tl = b >> tile_shift;
+    const uint32_t tlprev = __shfl_up(tl, 1, 64);
+    const bool thead = live && (lane == 0 || tlprev != tl);
+    const uint64_t tabove = __ballot(thead) & ~upto;
+    const uint32_t tend = tabove ? (uint32_t)__ffsll((unsigned long long)tabove) - 1u : (uint32_t)nlive;
+    if (lane == 0) {
+        s_tile[wave] = thead ? tl : 0xffffffffu;
+        s_tcnt[wave] = thead ? tend : 0u;
+    } else if (thead) {
+        atomicAdd(&tcopy[(size_t)tl * TAGG_COPIES], tend - (uint32_t)lane);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t0 = s_tile[0], c0 = s_tcnt[0];
+#pragma unroll
+        for (int w = 1; w < TPB / 64; ++w) {
+            if (s_tile[w] == t0) {
+                c0 += s_tcnt[w];
+            } else {
+                if (c0) atomicAdd(&tcopy[(size_t)t0 * TAGG_COPIES], c0);
+                t0 = s_tile[w];
+                c0 = s_tcnt[w];
+            }
+        }
+        if (c0) atomicAdd(&tcopy[(size_t)t0 * TAGG_COPIES], c0);
+    }
     if (live) {
         hid[t] = b;
         rank[t] = base + (uint32_t)(lane - h);
@@ -95,7 +128,10 @@ k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __res
           const uint32_t* __restrict__ hid, const uint32_t* __restrict__ rank,
           const float4* __restrict__ x32, const uint32_t* __restrict__ bstart,
           uint32_t* __restrict__ sidx, uint32_t* __restrict__ skey, double4* __restrict__ spos,
-          float4* __restrict__ spos32) {
+          float4* __restrict__ spos32, uint32_t* __restrict__ tagg, int ntiles) {
+    // the scan that read the histogram's tile totals has finished: clear them for the next sort
+    if (blockIdx.x == 0)
+        for (int i = threadIdx.x; i < ntiles * TAGG_COPIES; i += TPB) tagg[i] = 0u;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     if (t >= P.n) return;
     const uint32_t b = hid[t];
@@ -1424,23 +1460,29 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool 
     // this step's order goes to sidxb[cur], starting from the previous step's in sidxb[1 - cur]
     const uint32_t* prev = d.sidxb[1 - c];
     d.sidx = d.sidxb[c];
+    const int64_t nkeys = dense ? (int64_t)(init ? P.sort_keys_init : P.sort_keys) : (int64_t)P.sc.grid.H;
+    const int64_t ntiles = scan_tiles(nkeys + 1);
+    if (ntiles > d.scan.max_tiles) return hipErrorInvalidValue;
+    const int shift = scan_tile_shift(nkeys + 1);
     if (dense) {
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<true, SEG>), grid_for(P.n, TPB), dim3(TPB), 0,
                                              s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
-                                             d.bcnt, d.ctr, reset_counters ? 1 : 0, init ? 1 : 0));
+                                             d.bcnt, d.scan.tagg, shift, d.ctr, reset_counters ? 1 : 0,
+                                             init ? 1 : 0));
     } else {  // the reference's hashed grid: single scenes only
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<false, false>), grid_for(P.n, TPB), dim3(TPB), 0,
                                              s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
-                                             d.bcnt, d.ctr, reset_counters ? 1 : 0, 0));
+                                             d.bcnt, d.scan.tagg, shift, d.ctr, reset_counters ? 1 : 0,
+                                             0));
     }
-    const int64_t nkeys = dense ? (int64_t)(init ? P.sort_keys_init : P.sort_keys) : (int64_t)P.sc.grid.H;
     timing_begin(s, K_SCAN);
-    hipError_t e = launch_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, true, nullptr, s->stream);
+    hipError_t e = launch_tile_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, s->stream);
     timing_end(s);
     if (e != hipSuccess) return e;
     LAUNCH(s, K_SCATTER, hipLaunchKernelGGL(k_scatter, grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
                                             P, d.pos[c], prev, d.hid, d.rank, d.x32, d.bstart, d.sidx,
-                                            d.skey, dense ? nullptr : d.spos, d.spos32));
+                                            d.skey, dense ? nullptr : d.spos, d.spos32, d.scan.tagg,
+                                            (int)ntiles));
     return hipSuccess;
 }
 

@@ -7,6 +7,9 @@
 // large scan, so the look-back depth, not bandwidth, sets the time).  The last workgroup to finish
 // clears the tile status words and counters, so no separate reset launch (and no memset node) is
 // needed.
+//
+// The counting sort's scan (k_scan_tiles) is cheaper still: the histogram pass already knows each
+// tile's total, so a tile only adds up the totals before it.
 #include "dem_internal.hpp"
 
 #include <algorithm>
@@ -172,6 +175,79 @@ k_scan_exclusive(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t 
     }
 }
 
+// Exclusive scan of bucket counts whose per-tile totals are already known (the counting sort's
+// histogram pass adds each wave's counts into a copy of tagg[bucket >> tile shift]).  Every tile
+// sums the totals of the tiles before it, so there is no look-back, no tile counter and no
+// fence; the counts are cleared as they are read.  tagg is cleared by the sort's scatter pass,
+// which runs after this kernel.
+template <int SCAN_ITEMS>
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_scan_tiles(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
+             const uint32_t* __restrict__ tagg) {
+    constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+    __shared__ uint32_t s_vals[SCAN_TILE + SCAN_TILE / 32];
+    __shared__ uint32_t s_wave[SCAN_THREADS / 64];
+    __shared__ uint32_t s_pre[SCAN_THREADS / 64];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const uint32_t tile = blockIdx.x;
+    const int64_t base = (int64_t)tile * SCAN_TILE;
+
+    uint32_t v[SCAN_ITEMS];
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t g = base + it * SCAN_THREADS + tid;
+        v[it] = g < n ? in[g] : 0u;
+    }
+    uint32_t pre = 0;
+    for (uint32_t k = tid; k < tile * TAGG_COPIES; k += SCAN_THREADS) pre += tagg[k];
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int64_t g = base + it * SCAN_THREADS + tid;
+        if (g < n) in[g] = 0u;
+    }
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int idx = it * SCAN_THREADS + tid;
+        s_vals[idx + (idx >> 5)] = v[it];
+    }
+    const uint32_t pw = (uint32_t)wave_sum_u64(pre);
+    if (lane == 0) s_pre[wave] = pw;
+    __syncthreads();
+
+    uint32_t sum = 0;
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int idx = tid * SCAN_ITEMS + it;
+        sum += s_vals[idx + (idx >> 5)];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    uint32_t run = incl - sum;
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; ++w) {
+        run += s_pre[w];
+        if (w < wave) run += s_wave[w];
+    }
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int idx = tid * SCAN_ITEMS + it;
+        const uint32_t x = s_vals[idx + (idx >> 5)];
+        s_vals[idx + (idx >> 5)] = run;
+        run += x;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SCAN_ITEMS; ++it) {
+        const int idx = it * SCAN_THREADS + tid;
+        const int64_t g = base + idx;
+        if (g < n) out[g] = s_vals[idx + (idx >> 5)];
+    }
+}
+
 }  // namespace
 
 static int64_t tile_of(int64_t n) {
@@ -179,6 +255,8 @@ static int64_t tile_of(int64_t n) {
 }
 
 int64_t scan_tiles(int64_t n) { return (n + tile_of(n) - 1) / tile_of(n); }
+
+int scan_tile_shift(int64_t n) { return n >= SCAN_LARGE_FROM ? 13 : 11; }
 
 int64_t scan_tiles_max(int64_t n) {
     return std::max(scan_tiles(std::min(n, SCAN_LARGE_FROM - 1)), scan_tiles(n));
@@ -197,6 +275,20 @@ hipError_t launch_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n
         hipLaunchKernelGGL(k_scan_exclusive<SCAN_ITEMS_SMALL>, dim3((uint32_t)ntiles), dim3(SCAN_THREADS),
                            0, stream, in, out, n, zero_input ? 1 : 0, ws.status, ws.counters, total,
                            (uint32_t)ntiles);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n,
+                            hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const int64_t ntiles = scan_tiles(n);
+    if (ntiles > ws.max_tiles) return hipErrorInvalidValue;
+    if (n >= SCAN_LARGE_FROM)
+        hipLaunchKernelGGL(k_scan_tiles<SCAN_ITEMS_LARGE>, dim3((uint32_t)ntiles), dim3(SCAN_THREADS), 0,
+                           stream, in, out, n, ws.tagg);
+    else
+        hipLaunchKernelGGL(k_scan_tiles<SCAN_ITEMS_SMALL>, dim3((uint32_t)ntiles), dim3(SCAN_THREADS), 0,
+                           stream, in, out, n, ws.tagg);
     return hipGetLastError();
 }
 
