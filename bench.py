@@ -8,8 +8,10 @@ synthetic input generated in-process (seed 3).  A step is one simulate! over all
 Prints one JSON line.  ``value`` is the whole-job throughput (particle-steps/s) of the timed K
 steps (graph replays), state resident in HBM before the timed region starts.  ``roofline``
 describes the dominant kernel (largest share of the step time), its average duration measured
-with HIP events on the scene's stream over K identical eagerly-launched steps just before.  ``cpu_baseline`` times the FP64 CPU restatement (the oracle) of the same step
-on a bounded sample on this host.
+with HIP events on the scene's stream over K identical eagerly-launched steps just before.
+``cpu_baseline`` times the FP64 CPU restatement (the oracle) of the same step on a bounded sample
+on this host, serial and as its OpenMP build on every thread OpenMP is given; the faster is the
+baseline and both are reported.
 """
 import argparse
 import json
@@ -205,13 +207,14 @@ def one_at_a_time(members, precision, steps, sample=16, seed=0):
     return tot_n * steps / tot_t, [int(k) for k in idx]
 
 
-def cpu_baseline(cfg, g, w, bonded, steps):
-    """The oracle (serial FP64 C restatement of the reference step) on a bounded sample."""
+def cpu_baseline(cfg, g, w, bonded, steps, parallel=False):
+    """The oracle (FP64 C restatement of the reference step) on a bounded sample: the serial
+    build, or with ``parallel`` its OpenMP build on all threads OpenMP is given."""
     from oracle.oracle import OracleScene
     if g is None:  # a batch: time the oracle on its scenes one after another (first 8)
         tot_n, tot_t = 0, 0.0
         for (c, gg, ww) in cfg[:8]:
-            o = OracleScene(c, gg, ww)
+            o = OracleScene(c, gg, ww, parallel=parallel)
             o.initialize()
             o.step(1)
             t0 = time.perf_counter()
@@ -220,7 +223,7 @@ def cpu_baseline(cfg, g, w, bonded, steps):
             tot_n += len(gg)
             o.close()
         return tot_n * steps / tot_t, tot_t
-    o = OracleScene(cfg, g, w)
+    o = OracleScene(cfg, g, w, parallel=parallel)
     if bonded:
         o.initialize()
     o.step(1)  # warm-up: allocations and first contact pass
@@ -275,8 +278,9 @@ def main():
     ap.add_argument("--compare-precision", action="store_true",
                     help="also time the other precision and report both")
     ap.add_argument("--cpu-steps", type=int, default=0,
-                    help="oracle steps of the CPU baseline (default: 2 for the pour, 20 for the "
-                         "18k cube, all 10000 steps of the 900-grain cube, 5 per batch scene)")
+                    help="steps of the serial oracle's CPU baseline (default: 2 for the pours, 20 "
+                         "for the 18k cube, 2000 for the 900-grain cube, 5 per batch scene); the "
+                         "OpenMP build runs 10, 6, 300, all 10000 and 20")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step eagerly (no hipGraph replay)")
@@ -376,16 +380,30 @@ def main():
         out[f"ms_per_step_{other}"] = round(1e3 * r2["elapsed"] / args.steps, 5)
         out["position_error_after_1000_steps"] = precision_error(cfg, g, w, bonded)
     if not args.no_cpu_baseline:
+        from oracle.oracle import num_threads
         if args.cpu_steps <= 0:
-            args.cpu_steps = {"pour": 2, "pour_packed": 2, "cube18k": 20, "cube900": 10000,
+            args.cpu_steps = {"pour": 2, "pour_packed": 2, "cube18k": 20, "cube900": 2000,
                               "batch256": 5}[args.workload]
-        v, dt = cpu_baseline(cfg, g, w, bonded, args.cpu_steps)
-        sample = (f"same workload ({n} grains), {args.cpu_steps} steps after 1 warm-up step, "
-                  f"serial FP64 oracle, {dt:.1f} s") if g is not None else (
-                  f"first 8 scenes of the batch one after another, {args.cpu_steps} steps each "
-                  f"after initialize + 1 warm-up step, serial FP64 oracle, {dt:.1f} s")
-        out["cpu_baseline"] = {"value": round(v, 1), "unit": "particle-steps/s", "cores": 1,
-                               "kind": "port", "sample": sample}
+        omp_steps = {"pour": 10, "pour_packed": 6, "cube18k": 300, "cube900": 10000,
+                     "batch256": 20}[args.workload]
+
+        def sample(steps, dt, how):
+            return (f"same workload ({n} grains), {steps} steps after 1 warm-up step, {how}, "
+                    f"{dt:.1f} s") if g is not None else (
+                    f"first 8 scenes of the batch one after another, {steps} steps each after "
+                    f"initialize + 1 warm-up step, {how}, {dt:.1f} s")
+        v1, dt1 = cpu_baseline(cfg, g, w, bonded, args.cpu_steps)
+        threads = num_threads(True)
+        vp, dtp = cpu_baseline(cfg, g, w, bonded, omp_steps, parallel=True)
+        # the faster of the two builds is the baseline; the other is reported beside it
+        serial = {"value": round(v1, 1), "cores": 1,
+                  "sample": sample(args.cpu_steps, dt1, "serial FP64 oracle")}
+        omp = {"value": round(vp, 1), "cores": threads,
+               "sample": sample(omp_steps, dtp, f"OpenMP build of the FP64 oracle, {threads} threads")}
+        best = omp if vp >= v1 else serial
+        out["cpu_baseline"] = {"value": best["value"], "unit": "particle-steps/s",
+                               "cores": best["cores"], "kind": "port", "sample": best["sample"],
+                               "serial": serial, "openmp": omp}
     out["setup_s"] = round(t_build, 2)
     if copy_cal:
         out["copy_calibration"] = copy_cal

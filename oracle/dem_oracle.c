@@ -18,12 +18,30 @@
  * reference's own particle files; Hertz / EBPM / wall closed forms; free flight; momentum.
  * Parity unpinned (no reference run is possible here): bit-level rounding of StaticArrays
  * norm/inv, Quaternionic to_rotation_matrix/normalize, Base cbrt/log, and Float64^4.
+ *
+ * The same source also builds with -fopenmp (liboracle_omp.so, the multi-core CPU baseline):
+ * per-particle and per-pair loops run in parallel, force sums use atomic adds (so their order,
+ * and the last bits of the sums, vary from run to run, as they do under the reference's CUDA
+ * atomics), and update_contacts decides in parallel but applies its set changes in pair order,
+ * which keeps the serial outcome.  Without -fopenmp the pragmas are ignored and the code is the
+ * serial restatement.
  */
 #include "dem_oracle.h"
 
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int32_t orc_num_threads(void) {
+#ifdef _OPENMP
+    return (int32_t)omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
 
 typedef struct { double x, y, z; } v3;
 
@@ -376,6 +394,7 @@ static inline uint64_t pairkey(const orc_scene* s, int32_t i, int32_t j) {
 static void contact_pass(orc_scene* s) {
     const int64_t n = s->n;
     /* executor.jl:39-43 */
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         int32_t* kid = &s->kid[3 * i];
         orc_cell(s->g[i].k, s->dmin, s->cs, kid);
@@ -395,6 +414,7 @@ static void contact_pass(orc_scene* s) {
     }
     /* executor.jl:56-62, kernels.jl:24-37 */
     memset(s->cpr, 0, n * sizeof(uint32_t));
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int64_t i = 0; i < n; ++i) {
         for (int w = 0; w < 8; ++w) {
             uint32_t b = s->nbr[8 * i + w] - 1;
@@ -414,6 +434,7 @@ static void contact_pass(orc_scene* s) {
         s->cpl_cap = cap;
     }
     /* kernels.jl:39-58 */
+#pragma omp parallel for schedule(dynamic, 1024)
     for (int64_t i = 0; i < n; ++i) {
         for (int w = 0; w < 8; ++w) {
             uint32_t b = s->nbr[8 * i + w] - 1;
@@ -463,29 +484,85 @@ static void init_bonds(orc_scene* s) {
     }
 }
 
-/* kernels.jl:87-120 */
-static void update_contacts(orc_scene* s) {
-    for (int64_t idx = 0; idx < s->total_cps; ++idx) {
-        int32_t i = s->cpl[2 * idx], j = s->cpl[2 * idx + 1];
-        uint64_t ij = pairkey(s, i, j);
-        if (!get_bit(&s->st, ij, F_BONDED)) {
-            const orc_grain* gi = &s->g[i - 1];
-            const orc_grain* gj = &s->g[j - 1];
-            int valid = vnorm(vsub(ld3(gi->k), ld3(gj->k))) < gi->r + gj->r;
-            int exist = get_bit(&s->st, ij, F_ACTIVE);
-            if (exist && !valid) {
-                s->cptr[1] += 1u;
-                clear_bit(&s->st, ij, F_ACTIVE);
-                s->cnt.deactivated_total += 1;
-            } else if (!exist && valid) {
-                if (append_contact(s, i, j)) continue;
-                set_bit(&s->st, ij, F_ACTIVE);
-            }
-        }
+/* kernels.jl:87-120 for one pair, given whether the pair is valid (touching) and the current
+ * flags; applies the set change (serial linearisation of the atomics). */
+static void apply_update(orc_scene* s, int64_t idx, int valid) {
+    int32_t i = s->cpl[2 * idx], j = s->cpl[2 * idx + 1];
+    uint64_t ij = pairkey(s, i, j);
+    if (get_bit(&s->st, ij, F_BONDED)) return;
+    int exist = get_bit(&s->st, ij, F_ACTIVE);
+    if (exist && !valid) {
+        s->cptr[1] += 1u;
+        clear_bit(&s->st, ij, F_ACTIVE);
+        s->cnt.deactivated_total += 1;
+    } else if (!exist && valid) {
+        if (append_contact(s, i, j)) return;
+        set_bit(&s->st, ij, F_ACTIVE);
     }
 }
 
+/* kernels.jl:87-120.  The pairs that may change the set (valid and not yet active, or active and
+ * no longer valid) are found in parallel; apply_update then runs on them in pair order, which
+ * re-reads the flags and so gives the serial loop's outcome (a duplicate pair sees the earlier
+ * copy's change). */
+static void update_contacts(orc_scene* s) {
+    const int64_t P = s->total_cps;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    int64_t** lists = (int64_t**)calloc((size_t)nt, sizeof(int64_t*));
+    int64_t* lens = (int64_t*)calloc((size_t)nt, sizeof(int64_t));
+    int used = 1;
+#pragma omp parallel
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#pragma omp single
+        used = omp_get_num_threads();
+#endif
+        int64_t cap = 1024, len = 0;
+        int64_t* l = (int64_t*)malloc((size_t)cap * sizeof(int64_t));
+        /* static schedule: thread t takes the t-th contiguous block of pairs */
+#pragma omp for schedule(static)
+        for (int64_t idx = 0; idx < P; ++idx) {
+            int32_t i = s->cpl[2 * idx], j = s->cpl[2 * idx + 1];
+            uint8_t f = ps_get(&s->st, pairkey(s, i, j));
+            if (f & F_BONDED) continue;
+            const orc_grain* gi = &s->g[i - 1];
+            const orc_grain* gj = &s->g[j - 1];
+            int valid = vnorm(vsub(ld3(gi->k), ld3(gj->k))) < gi->r + gj->r;
+            int exist = (f & F_ACTIVE) != 0;
+            if (exist == valid) continue;
+            if (len == cap) { cap *= 2; l = (int64_t*)realloc(l, (size_t)cap * sizeof(int64_t)); }
+            l[len++] = valid ? idx : -1 - idx;
+        }
+        lists[t] = l;
+        lens[t] = len;
+    }
+    for (int t = 0; t < used; ++t) {
+        for (int64_t k = 0; k < lens[t]; ++k) {
+            int64_t v = lists[t][k];
+            if (v >= 0) apply_update(s, v, 1);
+            else apply_update(s, -1 - v, 0);
+        }
+        free(lists[t]);
+    }
+    free(lists);
+    free(lens);
+}
+
 static void atomic_add_vec3(double* arr, int64_t i0, v3 v) { /* kernels.jl:1-5 */
+#pragma omp atomic
+    arr[i0] += v.x;
+#pragma omp atomic
+    arr[i0 + 1] += v.y;
+#pragma omp atomic
+    arr[i0 + 2] += v.z;
+}
+/* the same sum where only one thread touches the particle */
+static void add_vec3(double* arr, int64_t i0, v3 v) {
     arr[i0] += v.x; arr[i0 + 1] += v.y; arr[i0 + 2] += v.z;
 }
 
@@ -530,6 +607,8 @@ static v3 hertz_mindlin(double gap, double dn, v3 vc, double nui, double Ei, dou
 static void resolve_collision(orc_scene* s) {
     const double dt = s->cfg.dt, tol = s->cfg.tolerance;
     int64_t total = s->cptr[0];
+    int64_t nbroken = 0, nfatal = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : nbroken, nfatal)
     for (int64_t idx = 0; idx < total; ++idx) {
         orc_contact* c = &s->contacts[idx];
         int32_t i = c->i, j = c->j;
@@ -583,10 +662,10 @@ static void resolve_collision(orc_scene* s) {
             double st = sci > scj ? sci : scj;
             double ss = fabs(Ti.x) * rj / 2.0 / Ij + 4.0 / 3.0 / Aj * sqrt(Fi.y * Fi.y + Fi.z * Fi.z);
             if (sc >= sf->sc || st >= sf->st || ss >= sf->ss) {
-                s->cptr[1] += 1u;
+                /* flags of distinct pairs are distinct bytes; the set is not resized here */
                 clear_bit(&s->st, ij, F_ACTIVE);
                 clear_bit(&s->st, ij, F_BONDED);
-                s->cnt.bonds_broken_total += 1;
+                nbroken += 1;
             } else {
                 m33 Ri = minv(&R);
                 atomic_add_vec3(s->forces, 3 * (i - 1), mv(&Ri, vneg(Fi)));
@@ -599,7 +678,7 @@ static void resolve_collision(orc_scene* s) {
         } else {
             /* Hertz-Mindlin (kernels.jl:240-313) */
             double gap = Li - gi->r - gj->r;
-            if (gap > 0) s->cnt.fatal_gap += 1;
+            if (gap > 0) nfatal += 1;
             double dn = fabs(gap);
             v3 k = vadd(ki, vscl(vnormalize(vsub(kj, ki)), gi->r - dn));
             v3 kip = vsub(k, ki), kjp = vsub(k, kj);
@@ -613,7 +692,7 @@ static void resolve_collision(orc_scene* s) {
             v3 sv = ld3(c->s);
             v3 F = hertz_mindlin(gap, dn, vc, s->mats[midi - 1].nu, s->mats[midi - 1].E,
                                  s->mats[midj - 1].nu, s->mats[midj - 1].E, Rs, ms, sf->e,
-                                 sf->mu, dt, &sv, &s->cnt.fatal_gap);
+                                 sf->mu, dt, &sv, &nfatal);
             m33 Ri = minv(&R);
             v3 Fg = mv(&Ri, vneg(F));
             atomic_add_vec3(s->forces, 3 * (i - 1), Fg);
@@ -623,12 +702,16 @@ static void resolve_collision(orc_scene* s) {
             st3(c->k, k); st3(c->F, F); st3(c->s, sv);
         }
     }
+    s->cptr[1] += (uint32_t)nbroken;
+    s->cnt.bonds_broken_total += nbroken;
+    s->cnt.fatal_gap += nfatal;
 }
 
 /* kernels.jl:317-426 */
 static void resolve_wall(orc_scene* s) {
     const double dt = s->cfg.dt, tol = s->cfg.tolerance;
-    int64_t wc = 0;
+    int64_t wc = 0, nfatal = 0;
+#pragma omp parallel for schedule(static) reduction(+ : wc, nfatal)
     for (int64_t i = 0; i < s->n; ++i) {
         const orc_grain* gi = &s->g[i];
         for (int32_t j = 0; j < s->nwall; ++j) {
@@ -651,11 +734,11 @@ static void resolve_wall(orc_scene* s) {
                 v3 sv = ld3(c->s);
                 v3 F = hertz_mindlin(gap, dn, vc, s->mats[midi - 1].nu, s->mats[midi - 1].E,
                                      s->mats[midj - 1].nu, s->mats[midj - 1].E, gi->r, gi->m,
-                                     sf->e, sf->mu, dt, &sv, &s->cnt.fatal_gap);
+                                     sf->e, sf->mu, dt, &sv, &nfatal);
                 m33 Ri = minv(&R);
                 v3 Fg = mv(&Ri, vneg(F));
-                atomic_add_vec3(s->forces, 3 * i, Fg);
-                atomic_add_vec3(s->moments, 3 * i, vcross(kip, Fg));
+                add_vec3(s->forces, 3 * i, Fg);
+                add_vec3(s->moments, 3 * i, vcross(kip, Fg));
                 c->i = (int32_t)(i + 1); c->j = 0; c->midi = midi; c->midj = midj;
                 st3(c->k, k); st3(c->F, F);
                 st3(c->ti, mk(0, 0, 0)); st3(c->tj, mk(0, 0, 0)); st3(c->s, sv);
@@ -665,6 +748,7 @@ static void resolve_wall(orc_scene* s) {
         }
     }
     s->cnt.wall_contacts = wc;
+    s->cnt.fatal_gap += nfatal;
 }
 
 static double jsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
@@ -673,17 +757,18 @@ static double jsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
 static void apply_body_force(orc_scene* s) {
     const double* gr = s->cfg.gravity;
     const double dmp = s->cfg.global_damping;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < s->n; ++i) {
         const orc_grain* g = &s->g[i];
-        atomic_add_vec3(s->forces, 3 * i, vscl(ld3(gr), g->m));
+        add_vec3(s->forces, 3 * i, vscl(ld3(gr), g->m));
         v3 F = ld3(&s->forces[3 * i]);
         v3 T = ld3(&s->moments[3 * i]);
         v3 dF = mk(-dmp * fabs(F.x) * jsign(g->v[0]), -dmp * fabs(F.y) * jsign(g->v[1]),
                    -dmp * fabs(F.z) * jsign(g->v[2]));
         v3 dT = mk(-dmp * fabs(T.x) * jsign(g->w[0]), -dmp * fabs(T.y) * jsign(g->w[1]),
                    -dmp * fabs(T.z) * jsign(g->w[2]));
-        atomic_add_vec3(s->forces, 3 * i, dF);
-        atomic_add_vec3(s->moments, 3 * i, dT);
+        add_vec3(s->forces, 3 * i, dF);
+        add_vec3(s->moments, 3 * i, dT);
     }
 }
 
@@ -707,6 +792,7 @@ static m33 quat_matrix(const double q[4]) {
 /* kernels.jl:445-485 */
 static void update(orc_scene* s) {
     const double dt = s->cfg.dt;
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < s->n; ++i) {
         orc_grain* g = &s->g[i];
         v3 F = ld3(&s->forces[3 * i]);

@@ -114,6 +114,7 @@ void orc_destroy(orc_scene* s);
 void orc_get_derived(const orc_scene* s, orc_derived* out);
 int32_t orc_initialize(orc_scene* s);              /* initialize! */
 int32_t orc_step(orc_scene* s, int64_t nsteps);    /* nsteps x simulate! */
+int32_t orc_num_threads(void);  /* threads a step uses: 1, or OpenMP's count in the -fopenmp build */
 void orc_contact_pass(orc_scene* s);               /* stages 1-3 of contact! only */
 void orc_get_counters(const orc_scene* s, orc_counters* out);
 

@@ -18,6 +18,8 @@ from taichi_dem_julia_port_amd.types import (CONTACT_DTYPE, GRAIN_DTYPE, GRAIN_I
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# the same source built with OpenMP: the multi-core CPU baseline (force sums in atomic order)
+LIB_OMP_PATH = os.path.join(HERE, "_build", "liboracle_omp.so")
 
 
 class OrcDerived(ctypes.Structure):
@@ -35,7 +37,7 @@ class OrcCounters(ctypes.Structure):
         "deactivated_total", "fatal_gap", "overflow")]
 
 
-_LIB = None
+_LIBS = {}
 
 
 def build() -> str:
@@ -43,13 +45,14 @@ def build() -> str:
     return LIB_PATH
 
 
-def load() -> ctypes.CDLL:
-    global _LIB
-    if _LIB is not None:
-        return _LIB
-    if not os.path.exists(LIB_PATH):
+def load(parallel: bool = False) -> ctypes.CDLL:
+    """The serial oracle, or with ``parallel`` its OpenMP build."""
+    if parallel in _LIBS:
+        return _LIBS[parallel]
+    path = LIB_OMP_PATH if parallel else LIB_PATH
+    if not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(LIB_PATH)
+    lib = ctypes.CDLL(path)
     P = ctypes.POINTER
     vp = ctypes.c_void_p
     sig = {
@@ -66,6 +69,7 @@ def load() -> ctypes.CDLL:
         "orc_get_derived": (None, [vp, P(OrcDerived)]),
         "orc_initialize": (ctypes.c_int32, [vp]),
         "orc_step": (ctypes.c_int32, [vp, ctypes.c_int64]),
+        "orc_num_threads": (ctypes.c_int32, []),
         "orc_contact_pass": (None, [vp]),
         "orc_get_counters": (None, [vp, P(OrcCounters)]),
         "orc_get_grains": (None, [vp, P(N.DemGrain)]),
@@ -85,8 +89,13 @@ def load() -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _LIB = lib
+    _LIBS[parallel] = lib
     return lib
+
+
+def num_threads(parallel: bool = False) -> int:
+    """Threads one oracle step uses (OpenMP's count for the parallel build)."""
+    return int(load(parallel).orc_num_threads())
 
 
 def _p(a, t):
@@ -103,10 +112,11 @@ def derive(cfg, grains) -> dict:
 
 
 class OracleScene:
-    """Serial FP64 CPU restatement of the reference step (same method names as Scene)."""
+    """Serial FP64 CPU restatement of the reference step (same method names as Scene).
+    ``parallel=True`` runs the OpenMP build of the same code."""
 
-    def __init__(self, cfg, grains, walls=None):
-        self._lib = load()
+    def __init__(self, cfg, grains, walls=None, parallel: bool = False):
+        self._lib = load(parallel)
         self.cfg = cfg
         g = np.ascontiguousarray(grains, dtype=GRAIN_IN_DTYPE)
         self.n = len(g)
