@@ -280,7 +280,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the serial oracle's CPU baseline (default: 2 for the pours, 20 "
                          "for the 18k cube, 2000 for the 900-grain cube, 5 per batch scene); the "
-                         "OpenMP build runs 10, 6, 300, all 10000 and 20")
+                         "OpenMP build runs 10, 6, 1000, all 10000 and 100")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step eagerly (no hipGraph replay)")
@@ -384,8 +384,8 @@ def main():
         if args.cpu_steps <= 0:
             args.cpu_steps = {"pour": 2, "pour_packed": 2, "cube18k": 20, "cube900": 2000,
                               "batch256": 5}[args.workload]
-        omp_steps = {"pour": 10, "pour_packed": 6, "cube18k": 300, "cube900": 10000,
-                     "batch256": 20}[args.workload]
+        omp_steps = {"pour": 10, "pour_packed": 6, "cube18k": 1000, "cube900": 10000,
+                     "batch256": 100}[args.workload]
 
         def sample(steps, dt, how):
             return (f"same workload ({n} grains), {steps} steps after 1 warm-up step, {how}, "

@@ -82,9 +82,17 @@ k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restri
     uint32_t base = 0u;
     if (head) base = atomicAdd(&bcnt[b], end - (uint32_t)lane);
     base = __shfl(base, live ? h : lane, 64);
+    if (live) {
+        hid[t] = b;
+        rank[t] = base + (uint32_t)(lane - h);
+        x32[t] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
+    }
     // The same run aggregation over scan tiles, so the scan of bcnt needs no look-back.  A wave's
     // first tile run is combined with the other waves' in LDS (a block's grains are adjacent in
-    // the previous order, so they nearly always share one tile); later runs add directly.
+    // the previous order, so they nearly always share one tile); later runs add directly.  The
+    // reference's hashed buckets scatter a block over many tiles, so that grid keeps the
+    // look-back scan instead.
+    if constexpr (!DENSE) return;
     __shared__ uint32_t s_tile[TPB / 64], s_tcnt[TPB / 64];
     const int wave = threadIdx.x >> 6;
     uint32_t* const tcopy = tagg + (blockIdx.x % TAGG_COPIES);
@@ -113,11 +121,6 @@ k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restri
             }
         }
         if (c0) atomicAdd(&tcopy[(size_t)t0 * TAGG_COPIES], c0);
-    }
-    if (live) {
-        hid[t] = b;
-        rank[t] = base + (uint32_t)(lane - h);
-        x32[t] = make_float4((float)x.x, (float)x.y, (float)x.z, (float)x.w);
     }
 }
 
@@ -1476,13 +1479,14 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool 
                                              0));
     }
     timing_begin(s, K_SCAN);
-    hipError_t e = launch_tile_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, s->stream);
+    hipError_t e = dense ? launch_tile_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, s->stream)
+                         : launch_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, true, nullptr, s->stream);
     timing_end(s);
     if (e != hipSuccess) return e;
     LAUNCH(s, K_SCATTER, hipLaunchKernelGGL(k_scatter, grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
                                             P, d.pos[c], prev, d.hid, d.rank, d.x32, d.bstart, d.sidx,
                                             d.skey, dense ? nullptr : d.spos, d.spos32, d.scan.tagg,
-                                            (int)ntiles));
+                                            dense ? (int)ntiles : 0));
     return hipSuccess;
 }
 
