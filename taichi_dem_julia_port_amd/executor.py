@@ -285,8 +285,9 @@ def solve(cfg_filename: str, no_bond: bool = False, save_snapshot: bool = False,
           max_steps: Optional[int] = None) -> Scene:
     """DEMPort.solve (executor.jl:174-394) on the MI355X engine.
 
-    ``save_snapshot`` (GLMakie pictures, utils.jl:80-124) is accepted and ignored.  ``max_steps``
-    optionally stops the loop early (the reference's commented debug switch, executor.jl:360-362).
+    ``save_snapshot`` writes ``snapshot_<step>.png`` into ``out_dir`` after every save interval
+    (executor.jl:376-378; pictures drawn by ``snapshot.snapshot``).  ``max_steps`` optionally stops
+    the loop early (the reference's commented debug switch, executor.jl:360-362).
     """
     t0 = time.perf_counter()
     cfg = from_toml(cfg_filename)
@@ -311,6 +312,9 @@ def solve(cfg_filename: str, no_bond: bool = False, save_snapshot: bool = False,
         scene.step(save_per_steps)
         step += save_per_steps
         log.info("Solving... step=%d dt=%.3fs", step, time.perf_counter() - t1)
+        if save_snapshot:
+            from .snapshot import snapshot
+            snapshot(scene.read_grains(), step, out_dir)
         if save_information:
             write_p4p_frame(p4p, scene.read_grains(), step * cfg.dt)
             c, b = scene.read_contacts()
