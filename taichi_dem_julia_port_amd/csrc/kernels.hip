@@ -319,7 +319,9 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
               const uint32_t* __restrict__ bstart, const double* __restrict__ r0,
               const uint32_t* __restrict__ eid, uint32_t* __restrict__ tcnt,
               uint32_t* __restrict__ touch, unsigned long long* ctr, int diag) {
-    __shared__ float4 stage[STAGE_CAP];
+    // BROAD_BATCH - 1 entries of padding: a batch's reads past the end of the last range stay
+    // inside the array (and are masked)
+    __shared__ float4 stage[STAGE_CAP + BROAD_BATCH];
     __shared__ uint32_t u_lo[5], u_off[6];  // union slot ranges and their LDS offsets
     __shared__ uint32_t b_cell[2], b_scene[2];
     // per-wave queue of candidates that passed the FP32 pre-test: slot k and the owner's lane
@@ -465,54 +467,70 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
             const float thr = fmaf(o.w, fmult, c0);
             return !(fmaf(-thr, thr, d2) > 0.0f);
         };
-#pragma unroll 1
-        for (int j = 0; j < 5; ++j) {
-            // range j without indexing the arrays by a runtime value (which would spill them)
-            const uint32_t rlo = j == 0 ? lo[0] : j == 1 ? lo[1] : j == 2 ? lo[2] : j == 3 ? lo[3] : lo[4];
-            const uint32_t rlen = j == 0 ? len[0] : j == 1 ? len[1] : j == 2 ? len[2] : j == 3 ? len[3] : len[4];
-            const uint32_t rlb = j == 0 ? lb[0] : j == 1 ? lb[1] : j == 2 ? lb[2] : j == 3 ? lb[3] : lb[4];
-#pragma unroll 1
-            for (uint32_t i = 0; i < rlen; i += BROAD_BATCH) {
-                float4 o[BROAD_BATCH];
-                if (in_stage) {
-                    // reads past the range stay inside the stage and are masked below
+        // one batch of candidates o[] at slots rlo + i + u of a range of rlen slots: survivors go
+        // to the wave's queue, confirmed below by all lanes together; a survivor that finds the
+        // queue full is confirmed at once
+        auto batch = [&](const float4 (&o)[BROAD_BATCH], uint32_t rlo, uint32_t rlen, uint32_t i) {
+            bool hit[BROAD_BATCH];
+            bool any = false;
 #pragma unroll
-                    for (int u = 0; u < BROAD_BATCH; ++u) o[u] = stage[(rlb + i + u) & (STAGE_CAP - 1)];
-                } else {
+            for (int u = 0; u < BROAD_BATCH; ++u) {
+                hit[u] = near(o[u]) && (u == 0 || i + u < rlen);
+                any |= hit[u];
+            }
+            if (!any) return;
+#pragma unroll
+            for (int u = 0; u < BROAD_BATCH; ++u) {
+                const uint64_t m = __ballot(hit[u]);
+                if (!m) continue;
+                const uint32_t base = hq_n[wv];
+                const uint32_t cnt = (uint32_t)__popcll(m);
+                const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+                const bool fits = base + cnt <= (uint32_t)HIT_QUEUE;
+                if (hit[u]) {
+                    if (fits) {
+                        hq_k[wv][base + rank] = rlo + i + (uint32_t)u;
+                        hq_l[wv][base + rank] = (uint8_t)lane;
+                    } else {
+                        confirm(S, p, rlo + i + (uint32_t)u);
+                    }
+                }
+                if (fits && rank == 0 && hit[u]) hq_n[wv] = base + cnt;
+            }
+        };
+        // range j without indexing the arrays by a runtime value (which would spill them)
+#define BROAD_RANGE(j)                                                                           \
+    const uint32_t rlo = j == 0 ? lo[0] : j == 1 ? lo[1] : j == 2 ? lo[2] : j == 3 ? lo[3] : lo[4]; \
+    const uint32_t rlen = j == 0 ? len[0] : j == 1 ? len[1] : j == 2 ? len[2] : j == 3 ? len[3] : len[4]
+        if (in_stage) {
+#pragma unroll 1
+            for (int j = 0; j < 5; ++j) {
+                BROAD_RANGE(j);
+                const uint32_t rlb = j == 0 ? lb[0] : j == 1 ? lb[1] : j == 2 ? lb[2] : j == 3 ? lb[3] : lb[4];
+                const float4* sp = stage + rlb;  // reads past the range stay in the padded stage
+#pragma unroll 1
+                for (uint32_t i = 0; i < rlen; i += BROAD_BATCH) {
+                    float4 o[BROAD_BATCH];
+#pragma unroll
+                    for (int u = 0; u < BROAD_BATCH; ++u) o[u] = sp[i + u];
+                    batch(o, rlo, rlen, i);
+                }
+            }
+        } else {
+#pragma unroll 1
+            for (int j = 0; j < 5; ++j) {
+                BROAD_RANGE(j);
+#pragma unroll 1
+                for (uint32_t i = 0; i < rlen; i += BROAD_BATCH) {
+                    float4 o[BROAD_BATCH];
 #pragma unroll
                     for (int u = 0; u < BROAD_BATCH; ++u)
                         o[u] = i + u < rlen ? spos32[rlo + i + u] : make_float4(0.f, 0.f, 0.f, 0.f);
-                }
-                bool hit[BROAD_BATCH];
-                bool any = false;
-#pragma unroll
-                for (int u = 0; u < BROAD_BATCH; ++u) {
-                    hit[u] = near(o[u]) && (u == 0 || i + u < rlen);
-                    any |= hit[u];
-                }
-                if (!any) continue;
-                // survivors go to the wave's queue, confirmed below by all lanes together; a
-                // survivor that finds the queue full is confirmed at once
-#pragma unroll
-                for (int u = 0; u < BROAD_BATCH; ++u) {
-                    const uint64_t m = __ballot(hit[u]);
-                    if (!m) continue;
-                    const uint32_t base = hq_n[wv];
-                    const uint32_t cnt = (uint32_t)__popcll(m);
-                    const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-                    const bool fits = base + cnt <= (uint32_t)HIT_QUEUE;
-                    if (hit[u]) {
-                        if (fits) {
-                            hq_k[wv][base + rank] = rlo + i + (uint32_t)u;
-                            hq_l[wv][base + rank] = (uint8_t)lane;
-                        } else {
-                            confirm(S, p, rlo + i + (uint32_t)u);
-                        }
-                    }
-                    if (fits && rank == 0 && hit[u]) hq_n[wv] = base + cnt;
+                    batch(o, rlo, rlen, i);
                 }
             }
         }
+#undef BROAD_RANGE
     }
     // every lane of the wave confirms queued survivors (the queue holds only live owners)
     {
