@@ -216,6 +216,11 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
     for (int a = 0; a < 3; ++a) S.g[a] = cfg->gravity[a];
     S.matE[0] = cfg->particle_elastic_modulus; S.matNu[0] = cfg->particle_poisson_ratio;
     S.matE[1] = cfg->wall_elastic_modulus;     S.matNu[1] = cfg->wall_poisson_ratio;
+    for (int m = 0; m < 2; ++m) {  // the same FP64 operations, in the same order, as per contact
+        const double nui = S.matNu[0], Ei = S.matE[0], nuj = S.matNu[m], Ej = S.matE[m];
+        S.ysm[m] = 1.0 / ((1.0 - nui * nui) / Ei + (1.0 - nuj * nuj) / Ej);
+        S.gsm[m] = 1.0 / (2.0 * (2.0 - nui) * (1.0 + nui) / Ei + 2.0 * (2.0 - nuj) * (1.0 + nuj) / Ej);
+    }
     auto beta_of = [](double e) { const double le = log(e); return le / sqrt(le * le + M_PI * M_PI); };
     SurfaceP pp{cfg->pp_coefficient_friction, cfg->pp_coefficient_restitution,
                 beta_of(cfg->pp_coefficient_restitution), cfg->bond_radius_ratio,

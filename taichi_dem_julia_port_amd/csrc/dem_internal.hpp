@@ -74,6 +74,9 @@ struct SceneP {
     double dt, tol, damping;
     double g[3];
     double matE[2], matNu[2];
+    // effective Young's and shear moduli Y*, G* of a particle against material m (kernels.jl:262-263,
+    // 371-372): per-scene constants the reference evaluates per contact, evaluated once here
+    double ysm[2], gsm[2];
     SurfaceP surf[2][2];
     WallP walls[MAX_WALLS];
     // particle-particle Hertz-Mindlin constants in FP32 (mixed precision)

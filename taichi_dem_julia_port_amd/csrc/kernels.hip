@@ -805,13 +805,12 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
 // ------------------------------------------------------------------------------------------
 struct HMOut { d3 F; d3 s; bool fatal; };
 
-// Hertz-Mindlin law (kernels.jl:261-303, 369-406).
-__device__ __forceinline__ HMOut hertz_mindlin(double gap, double dn, d3 vc, double nui, double Ei,
-                                               double nuj, double Ej, double Rs, double ms,
-                                               const SurfaceP& sf, double dt, d3 s) {
+// Hertz-Mindlin law (kernels.jl:261-303, 369-406).  Ys and Gs are the effective moduli
+// (kernels.jl:262-263), per-scene constants evaluated once on the host with the same operations.
+__device__ __forceinline__ HMOut hertz_mindlin(double gap, double dn, d3 vc, double Ys, double Gs,
+                                               double Rs, double ms, const SurfaceP& sf, double dt,
+                                               d3 s) {
     HMOut o;
-    const double Ys = 1.0 / ((1.0 - nui * nui) / Ei + (1.0 - nuj * nuj) / Ej);
-    const double Gs = 1.0 / (2.0 * (2.0 - nui) * (1.0 + nui) / Ei + 2.0 * (2.0 - nuj) * (1.0 + nuj) / Ej);
     const double beta = sf.beta;
     const double srd = sqrt(Rs * dn);
     const double Sn = 2.0 * Ys * srd;
@@ -936,7 +935,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         const uint32_t p = (uint32_t)p64;
         const SceneP& S = scene_of<SEG>(P, p);
         const double dt = S.dt;
-        const double4 X = pos[p], Vv = vel[p], W = omg[p], Q = quat[p];
+        const double4 X = pos[p], Vv = vel[p], W = omg[p];  // the orientation is read after the row
         const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
         d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
         CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
@@ -945,6 +944,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         double* st = b ? st1 : st0;
         const uint32_t cnt = (b ? rcnt1 : rcnt0)[p];
         const int64_t cs = (int64_t)P.K * n;
+        const double inv_r = cnt ? 1.0 / X.w : 0.0, inv_m = cnt ? 1.0 / Vv.w : 0.0;
         // software pipeline over the row: entry m+1's partner state and entry m+2's word are
         // loaded while entry m is evaluated
         uint32_t w_cur = 0u, w_nxt = 0u;
@@ -1177,11 +1177,12 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const d3 vcj = cross(wj, kjp) + vj;
                 const d3 vc = mul(R, vcj - vci);
                 const SurfaceP& sf = S.surf[0][0];
-                const double Rs = 1.0 / (1.0 / ri + 1.0 / rj);
-                const double ms = 1.0 / (1.0 / mi + 1.0 / mj);
+                // 1/r_i + 1/r_j with this particle's reciprocal taken once per row (IEEE addition
+                // commutes, so the sum is the canonical one bit for bit)
+                const double Rs = 1.0 / (inv_r + 1.0 / (plo ? rj : ri));
+                const double ms = 1.0 / (inv_m + 1.0 / (plo ? mj : mi));
                 const d3 s0 = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
-                const HMOut h = hertz_mindlin(gap, dn, vc, S.matNu[0], S.matE[0], S.matNu[0],
-                                              S.matE[0], Rs, ms, sf, dt, s0);
+                const HMOut h = hertz_mindlin(gap, dn, vc, S.ysm[0], S.gsm[0], Rs, ms, sf, dt, s0);
                 const m3 Ri = inverse(R);
                 const d3 Fg = mul(Ri, -h.F);
                 if (plo) { F = F + Fg; T = T + cross(kip, Fg); }
@@ -1221,8 +1222,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                     const double4 ws = wshear[(int64_t)j * n + p];
                     s0 = mk3(ws.x, ws.y, ws.z);
                 }
-                const HMOut h = hertz_mindlin(gap, dn, vc, S.matNu[0], S.matE[0], S.matNu[mj],
-                                              S.matE[mj], X.w, Vv.w, sf, dt, s0);
+                const HMOut h = hertz_mindlin(gap, dn, vc, S.ysm[mj], S.gsm[mj], X.w, Vv.w, sf, dt, s0);
                 if (h.fatal) ++fatal;
                 const m3 Ri = inverse(R);
                 const d3 Fg = mul(Ri, -h.F);
@@ -1243,6 +1243,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             ttot[p] = make_double4(T.x, T.y, T.z, 0.0);
         }
         // update! (kernels.jl:445-485)
+        const double4 Q = quat[p];
         const d3 av = F / Vv.w;
         const d3 knew = (kp + vp * dt) + (0.5 * av) * (dt * dt);
         const d3 vnew = vp + av * dt;
