@@ -1033,7 +1033,9 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                         if (plo) { FH.add(fmul_t(R32, fneg(F32))); TH.add(fmul_t(R32, fneg(Ti32))); }
                         else { FH.add(fmul_t(R32, F32)); TH.add(fmul_t(R32, fneg(Tj32))); }
                         const d3 kc = (ki + kj) * 0.5;
-                        st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                        if (write_forces) {  // the contact point is an output, not state
+                            st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                        }
                         st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
                         st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
                         st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
@@ -1093,7 +1095,9 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                     F = F + mul(Ri, -Fj);
                     T = T + mul(Ri, -Tj);
                 }
-                st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                if (write_forces) {  // the contact point is an output, not state
+                    st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                }
                 st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
                 st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
                 st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
@@ -1156,8 +1160,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 if (plo) { FH.add(Fg); TH.add(fcross(kip, Fg)); }
                 else { FH.add(mkf(-Fg.x, -Fg.y, -Fg.z)); TH.add(fcross(kjp, mkf(-Fg.x, -Fg.y, -Fg.z))); }
                 const d3 kc = ki + tod(kip);
-                st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
-                st[S_FX * cs + e] = (double)Fl.x; st[S_FY * cs + e] = (double)Fl.y; st[S_FZ * cs + e] = (double)Fl.z;
+                if (write_forces) {  // contact point and local force are outputs, not state
+                    st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                    st[S_FX * cs + e] = (double)Fl.x; st[S_FY * cs + e] = (double)Fl.y; st[S_FZ * cs + e] = (double)Fl.z;
+                }
                 st[S_SX * cs + e] = sv.x; st[S_SY * cs + e] = sv.y; st[S_SZ * cs + e] = sv.z;
                 if (plo) {
                     ++active;
@@ -1187,8 +1193,10 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 const d3 Fg = mul(Ri, -h.F);
                 if (plo) { F = F + Fg; T = T + cross(kip, Fg); }
                 else { F = F + (-Fg); T = T + cross(kjp, -Fg); }
-                st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
-                st[S_FX * cs + e] = h.F.x; st[S_FY * cs + e] = h.F.y; st[S_FZ * cs + e] = h.F.z;
+                if (write_forces) {  // contact point and local force are outputs, not state
+                    st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                    st[S_FX * cs + e] = h.F.x; st[S_FY * cs + e] = h.F.y; st[S_FZ * cs + e] = h.F.z;
+                }
                 st[S_SX * cs + e] = h.s.x; st[S_SY * cs + e] = h.s.y; st[S_SZ * cs + e] = h.s.z;
                 if (plo) {
                     ++active;
