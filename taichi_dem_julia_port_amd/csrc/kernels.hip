@@ -610,6 +610,24 @@ __device__ __forceinline__ bool in_window(const SceneP& S, const double4* __rest
     return ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z));
 }
 
+// State an entry carries from step to step: a bond's force and moments (F, ti, tj), a Hertz
+// contact's tangential spring s.  The contact point, a Hertz contact's local force and the zero
+// moments of a Hertz contact are outputs, rewritten on every output step, so a row rebuilt by the
+// merge moves only the state (the bond pass, which runs before any step, moves everything).
+template <bool INIT>
+__device__ __forceinline__ void move_state(double* __restrict__ sn, const double* __restrict__ so,
+                                           int64_t cs, int64_t dst, int64_t src, uint32_t word) {
+    const int c0 = INIT ? 0 : ((word & BONDED) ? S_FX : S_SX);
+    const int c1 = INIT ? S_NCOMP : ((word & BONDED) ? S_SX : S_NCOMP);
+    for (int c = c0; c < c1; ++c) sn[c * cs + dst] = so[c * cs + src];
+}
+// A new entry's state is zero (outside the bond pass only a Hertz contact is ever created).
+template <bool INIT>
+__device__ __forceinline__ void zero_state(double* __restrict__ sn, int64_t cs, int64_t dst) {
+#pragma unroll
+    for (int c = INIT ? 0 : S_SX; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+}
+
 // Per-particle merge of the sorted old row with the sorted new touching set.  `old_at(m)` and
 // `touch_at(m)` read the m-th old-row word and the m-th touching partner.
 template <bool INIT, typename OldAt, typename TouchAt>
@@ -680,30 +698,25 @@ __device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __res
                 if (in_window(S, pos, lo, hi)) continue;
             }
             pqn[dst] = eo;
-#pragma unroll
-            for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
+            move_state<INIT>(sn, so, cs, dst, src, eo);
             ++o;
         } else if (qn < qo) {
             ++in;
             pqn[dst] = (tw & (IDX_MASK | IS_I)) | (INIT ? BONDED : 0u);
-#pragma unroll
-            for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+            zero_state<INIT>(sn, cs, dst);
             ++o;
         } else {
             const int64_t src = (int64_t)io * n + p;
             ++io; ++in;
             if (eo & BROKEN) {
                 pqn[dst] = tw & (IDX_MASK | IS_I);
-#pragma unroll
-                for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+                zero_state<INIT>(sn, cs, dst);
             } else if (INIT && !(eo & BONDED)) {
                 pqn[dst] = eo | BONDED;
-#pragma unroll
-                for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = 0.0;
+                zero_state<INIT>(sn, cs, dst);
             } else {
                 pqn[dst] = eo;
-#pragma unroll
-                for (int c = 0; c < S_NCOMP; ++c) sn[c * cs + dst] = so[c * cs + src];
+                move_state<INIT>(sn, so, cs, dst, src, eo);
             }
             ++o;
         }
@@ -1376,7 +1389,9 @@ k_lo_rows(const Params P, const uint8_t* __restrict__ sel, const uint32_t* __res
             o.ti[2] = bd ? st[S_TIZ * cs + e] : 0.0;
             o.tj[0] = bd ? st[S_TJX * cs + e] : 0.0; o.tj[1] = bd ? st[S_TJY * cs + e] : 0.0;
             o.tj[2] = bd ? st[S_TJZ * cs + e] : 0.0;
-            o.s[0] = st[S_SX * cs + e]; o.s[1] = st[S_SY * cs + e]; o.s[2] = st[S_SZ * cs + e];
+            // a bond's tangential spring is zero in every reachable state (kernels.jl:230-238)
+            o.s[0] = bd ? 0.0 : st[S_SX * cs + e]; o.s[1] = bd ? 0.0 : st[S_SY * cs + e];
+            o.s[2] = bd ? 0.0 : st[S_SZ * cs + e];
             bonded[base + c] = bd ? 1 : 0;
         }
         ++c;
