@@ -38,8 +38,10 @@ def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> floa
                       the two row appends (4 B counter + 4 B entry each)
       merge           touch count 4 B + row count and selector 5 B per grain
       force_integrate grain state read 128 B + written 128 B, row counts 5 B, wall flags 4 B;
-                      per Hertz contact 2 rows x (partner 4 + spring 24 r/w + k,F 48 w),
-                      per bond 2 rows x (partner 4 + F,ti,tj 72 r/w + k 24 w), per wall contact 32 B"""
+                      per Hertz contact 2 rows x (partner 4 + spring 24 read + 24 written),
+                      per bond 2 rows x (partner 4 + F,ti,tj 72 read + 72 written), per wall
+                      contact 32 B (contact points and local forces are written only on a call's
+                      last step, outside the timed replays)"""
     n = c["n"]
     cells = c.get("grid_cells") or c["hash_table_size"]
     Ch, Cb, W = c["hertz"], c["bonded"], c["wall_contacts"]
@@ -47,7 +49,7 @@ def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> floa
     if kernel == "broad_phase":
         return n * 24.0 + 4.0 * cells + touching * (64 + 4 + 16)
     if kernel == "force_integrate":
-        return n * (128 + 128 + 5 + 4) + 2 * Ch * (4 + 48 + 48) + 2 * Cb * (4 + 144 + 24) + 32.0 * W
+        return n * (128 + 128 + 5 + 4) + 2 * Ch * (4 + 48) + 2 * Cb * (4 + 144) + 32.0 * W
     if kernel == "hash":
         return n * 60.0
     if kernel == "scatter":
