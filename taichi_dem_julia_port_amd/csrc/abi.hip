@@ -262,6 +262,7 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
             if (cells <= budget && cells < 2.0e9) {
                 for (int a = 0; a < 3; ++a) { G.origin[a] = cfg->domain_min[a]; G.dims[a] = dims[a]; }
                 G.inv_cell = 1.0 / cell;
+                G.cell = 1.0 / G.inv_cell;
                 G.inv_dx = 1.0 / (double)dims[0];
                 G.inv_dxy = 1.0 / ((double)dims[0] * (double)dims[1]);
                 G.cells = (uint32_t)cells;

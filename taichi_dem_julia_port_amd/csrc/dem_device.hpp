@@ -189,6 +189,7 @@ __device__ __forceinline__ bool ref_candidate(const HashGrid& g, d3 lo, d3 hi) {
 struct DenseGrid {
     double origin[3];
     double inv_cell;
+    double cell;  // 1 / inv_cell: the cell edge
     double inv_dx, inv_dxy;  // 1 / dims[0], 1 / (dims[0] dims[1]) for decoding a cell index
     int32_t dims[3];
     uint32_t cells;

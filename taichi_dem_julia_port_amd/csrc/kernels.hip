@@ -308,6 +308,56 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
 // candidate rule (hi's bucket in lo's window, kernels.jl:24-58), so the contact set is the
 // reference's in either precision.  Blocks whose unions exceed the stage, or that straddle two
 // scenes of a batch, read the FP32 copies from global memory instead.
+// The slot ranges [lo_j, lo_j + len_j) (all above slot t) of the five stencil rows a particle in
+// cell c scans for partners: the row itself from x - 1 to x + 1 and the rows (y+1, z), (y-1, z+1),
+// (y, z+1), (y+1, z+1).  A pair is owned by its lower slot, which covers the other four rows.
+// A partner can be at most R = r_i * mult + cell / 2 away (the cell edge is at least the largest
+// contact reach), so a neighbour cell whose nearest face is farther than R from this particle
+// holds no partner: every particle of cell x - 1 lies below this cell's lower x face, and so on.
+// Per row the perpendicular gap is combined with the x gap.  The gaps are FP32 with a margin over
+// their rounding; a particle clamped into a boundary cell only has larger gaps to the cells it is
+// clamped away from.
+__device__ __forceinline__ void broad_ranges(const DenseGrid& G, uint32_t cbase, const int32_t c[3],
+                                             float4 me32, double mult, int64_t t, bool work,
+                                             const uint32_t* __restrict__ bstart, uint32_t lo[5],
+                                             uint32_t len[5]) {
+    const float cell = (float)G.cell;
+    const float R = fmaf(me32.w, (float)mult, 0.5f * cell) * (1.0f + 1e-5f) +
+                    16.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z) + 3.0f * cell);
+    const float R2 = R * R;
+    float gl[3], gh[3];  // distances to the cell's lower and upper faces (0 if outside)
+    {
+        const float pc[3] = {me32.x, me32.y, me32.z};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float flo = (float)(G.origin[a] + (double)c[a] * G.cell);
+            gl[a] = fmaxf(pc[a] - flo, 0.0f);
+            gh[a] = fmaxf(flo + cell - pc[a], 0.0f);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 5; ++j) {
+        const int dy = j == 0 ? 0 : (j == 1 ? 1 : j - 3);
+        const int dz = j < 2 ? 0 : 1;
+        const int32_t y = c[1] + dy, z = c[2] + dz;
+        const float gy = dy > 0 ? gh[1] : (dy < 0 ? gl[1] : 0.0f);
+        const float gz = dz > 0 ? gh[2] : 0.0f;
+        const float g2 = gy * gy + gz * gz;  // squared gap to the row
+        const int32_t x0 = (c[0] > 0 && gl[0] * gl[0] + g2 <= R2) ? c[0] - 1 : c[0];
+        const int32_t x1 = (c[0] + 1 < G.dims[0] && gh[0] * gh[0] + g2 <= R2) ? c[0] + 1 : c[0];
+        lo[j] = 0u; len[j] = 0u;
+        if (work && y >= 0 && y < G.dims[1] && z < G.dims[2] && g2 <= R2) {
+            const uint32_t row = cbase +
+                                 (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
+            uint32_t a = bstart[row + (uint32_t)x0];
+            const uint32_t e = bstart[row + (uint32_t)x1 + 1];
+            if (a <= (uint32_t)t) a = (uint32_t)t + 1;
+            lo[j] = a;
+            len[j] = e > a ? e - a : 0u;
+        }
+    }
+}
+
 constexpr int BROAD_BATCH = 4;
 constexpr int STAGE_CAP = 2048;  // float4 per block (32 KiB); a power of two
 constexpr int HIT_QUEUE = 256;   // pre-test survivors queued per wave
@@ -355,26 +405,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     // this thread's five slot ranges [lo_j, lo_j + len_j), all above t
     const bool work = live && (!INIT || S.bond_init);
     uint32_t lo[5], len[5];
-    {
-        const int32_t x0 = c[0] > 0 ? c[0] - 1 : 0;
-        const int32_t x1 = c[0] + 1 < G.dims[0] ? c[0] + 1 : G.dims[0] - 1;
-#pragma unroll
-        for (int j = 0; j < 5; ++j) {
-            const int dy = j == 0 ? 0 : (j == 1 ? 1 : j - 3);
-            const int dz = j < 2 ? 0 : 1;
-            const int32_t y = c[1] + dy, z = c[2] + dz;
-            lo[j] = 0u; len[j] = 0u;
-            if (work && y >= 0 && y < G.dims[1] && z < G.dims[2]) {
-                const uint32_t row = cbase +
-                                     (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
-                uint32_t a = bstart[row + (uint32_t)x0];
-                const uint32_t e = bstart[row + (uint32_t)x1 + 1];
-                if (a <= (uint32_t)t) a = (uint32_t)t + 1;
-                lo[j] = a;
-                len[j] = e > a ? e - a : 0u;
-            }
-        }
-    }
+    broad_ranges(G, cbase, c, me32, INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0, t, work, bstart, lo, len);
     __syncthreads();
     // union ranges (threads 0-4), then their LDS offsets (thread 0)
     const bool one_scene = !SEG || b_scene[0] == b_scene[1];
@@ -546,13 +577,14 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
 }
 
-// Candidate pairs the dense broad phase examines (slots k > t of the 9 stencil ranges): a
-// diagnostic, evaluated on the last step of each dem_step call.  (Counting inside the broad-phase
+// Candidate pairs the dense broad phase examines (its five trimmed slot ranges): a diagnostic,
+// evaluated on the last step of each dem_step call.  (Counting inside the broad-phase
 // loop itself costs more than the loop.)
 template <bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
-                 const uint32_t* __restrict__ bstart, unsigned long long* ctr) {
+                 const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
+                 unsigned long long* ctr) {
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     uint64_t c = 0;
     if (t < P.n) {
@@ -560,22 +592,10 @@ k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32
         const DenseGrid& G = S.dense;
         int32_t cc[3];
         dense_coords(G, skey[t] - S.cell_base, cc);
-        const int32_t x0 = cc[0] > 0 ? cc[0] - 1 : 0;
-        const int32_t x1 = cc[0] + 1 < G.dims[0] ? cc[0] + 1 : G.dims[0] - 1;
-        for (int dz = -1; dz <= 1; ++dz) {
-            const int32_t z = cc[2] + dz;
-            if (z < 0 || z >= G.dims[2]) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                const int32_t y = cc[1] + dy;
-                if (y < 0 || y >= G.dims[1]) continue;
-                const uint32_t row = S.cell_base +
-                                     (uint32_t)G.dims[0] * ((uint32_t)y + (uint32_t)G.dims[1] * (uint32_t)z);
-                uint32_t k = bstart[row + (uint32_t)x0];
-                const uint32_t s1 = bstart[row + (uint32_t)x1 + 1];
-                if (k <= (uint32_t)t) k = (uint32_t)t + 1;
-                if (k < s1) c += s1 - k;
-            }
-        }
+        uint32_t lo[5], len[5];
+        broad_ranges(G, S.cell_base, cc, spos32[t], 1.0, t, true, bstart, lo, len);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) c += len[j];
     }
     wave_add(ctr_at(ctr, CT_EXAMINED), c);
 }
@@ -1603,7 +1623,7 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
-                                              d.sidx, d.skey, d.bstart, d.ctr));
+                                              d.sidx, d.skey, d.spos32, d.bstart, d.ctr));
     if (count_candidates && dense && !SEG) {
         // the cp_list length needs the reference's bucket sort: rebuild it after the merge
         if ((e = sort_impl<false>(s, false, false)) != hipSuccess) return e;
