@@ -34,11 +34,35 @@ def _cases():
         # per touching pair and the reference-grid path is index-owned
         "lattice_safety09": lambda: scenes.bonded_cube(8, 8, 6, speed=10.0, dt=1e-6, base_gap=2e-5,
                                                        neighboring_search_safety_factor=0.9),
+        # grains beyond domain_max in x and y: the dense grid clamps them into its last cells
+        "packed_outside": lambda: _shrunk_domain(packed_sample(2000)),
+        # a slab narrower than two cells in x: the dense grid has one or two cells on that axis
+        "packed_slab": lambda: _slab(packed_sample(1500, box=(0.004, 0.03, 0.02))),
     }
 
 
+def _slab(case):
+    """The domain cut to the grains' x extent (4 mm, under two contact-reach cells)."""
+    cfg, g, w = case
+    lo, hi = list(cfg.domain_min), list(cfg.domain_max)
+    r = np.cbrt(g["V"] / (4.0 / 3.0 * np.pi))
+    lo[0] = float((g["k"][:, 0] - r).min())
+    hi[0] = float((g["k"][:, 0] + r).max())
+    return cfg.replace(domain_min=lo, domain_max=hi), g, w
+
+
+def _shrunk_domain(case):
+    """The same grains in a domain whose upper x and y bounds sit at the 70th percentile of the
+    grain coordinates, so about half the grains lie outside it."""
+    cfg, g, w = case
+    hi = list(cfg.domain_max)
+    hi[0] = float(np.percentile(g["k"][:, 0], 70))
+    hi[1] = float(np.percentile(g["k"][:, 1], 70))
+    return cfg.replace(domain_max=hi), g, w
+
+
 # cases stepped without initialize! (solve's no_bond = true)
-UNBONDED = {"pour_4k", "packed_4k", "cfg2_nobond"}
+UNBONDED = {"pour_4k", "packed_4k", "cfg2_nobond", "packed_outside", "packed_slab"}
 
 
 CASES = sorted(_cases())
@@ -178,7 +202,8 @@ def test_one_step_parity(name, variant):
 
 
 @pytest.mark.parametrize("variant", ["auto", "refgrid"])
-@pytest.mark.parametrize("name", ["cfg2_cube911", "lattice_10x10x9", "packed_4k"])
+@pytest.mark.parametrize("name", ["cfg2_cube911", "lattice_10x10x9", "packed_4k", "packed_outside",
+                                  "packed_slab"])
 def test_stepwise_parity_from_identical_states(name, variant):
     """Run 60 steps; every 20 steps copy the engine state into the oracle and compare the next step."""
     Scene, _ = _engine()
