@@ -838,6 +838,12 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
 // ------------------------------------------------------------------------------------------
 struct HMOut { d3 F; d3 s; bool fatal; };
 
+typedef double dvec4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ void store_nt(double4* p, double4 v) {
+    const dvec4 t = {v.x, v.y, v.z, v.w};
+    __builtin_nontemporal_store(t, reinterpret_cast<dvec4*>(p));
+}
+
 // Hertz-Mindlin law (kernels.jl:261-303, 369-406).  Ys and Gs are the effective moduli
 // (kernels.jl:262-263), per-scene constants evaluated once on the host with the same operations.
 __device__ __forceinline__ HMOut hertz_mindlin(double gap, double dn, d3 vc, double Ys, double Gs,
@@ -1307,10 +1313,11 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         const double n0 = q0 + dq0, n1 = q1 + dq1, n2 = q2 + dq2, n3 = q3 + dq3;
         const double qn = sqrt(n0 * n0 + n1 * n1 + n2 * n2 + n3 * n3);
         const d3 wnew = wp + dw * dt;
-        pos_n[p] = make_double4(knew.x, knew.y, knew.z, X.w);
-        vel_n[p] = make_double4(vnew.x, vnew.y, vnew.z, Vv.w);
-        omg_n[p] = make_double4(wnew.x, wnew.y, wnew.z, W.w);
-        quat_n[p] = make_double4(n0 / qn, n1 / qn, n2 / qn, n3 / qn);
+        // streaming stores: the next state is not read again before this launch ends
+        store_nt(pos_n + p, make_double4(knew.x, knew.y, knew.z, X.w));
+        store_nt(vel_n + p, make_double4(vnew.x, vnew.y, vnew.z, Vv.w));
+        store_nt(omg_n + p, make_double4(wnew.x, wnew.y, wnew.z, W.w));
+        store_nt(quat_n + p, make_double4(n0 / qn, n1 / qn, n2 / qn, n3 / qn));
         if (write_forces) {  // a and dw are outputs of update!; the step never reads them back
             acc[p] = make_double4(av.x, av.y, av.z, 0.0);
             dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
