@@ -240,6 +240,18 @@ int32_t dem_batch_bonds_broken(dem_scene_t* s, int64_t* out);          /* cumula
  * bond-break count.  Both must hold the same particles and walls; used to re-pack live scenes
  * into a new batch when the serving loop admits or retires scenes. */
 int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, int32_t src_scene);
+/* A new batch of scenes keep[0..nkeep) of src (a batch or a single scene on the dense grid), in
+ * that order, followed by nnew new scenes given as for dem_create_batch (cfgs, nwalls, walls,
+ * counts, grains describe the new scenes only; they may be NULL when nnew is 0).  The kept scenes
+ * arrive as dem_batch_adopt would move them, without their input rows: their derived fields,
+ * order and parameters are copied from src and their state moves device to device.  Kept scenes
+ * take no part in dem_initialize (mask 0); new ones keep the default (1).  src is left as it was.
+ * Contact slots default to the larger of dem_create_batch's default and src's.  The serving
+ * loop's re-pack at every change of the live set. */
+int32_t dem_batch_repack(dem_scene_t* src, int32_t nkeep, const int32_t* keep, int32_t nnew,
+                         const dem_config_t* cfgs, const int32_t* nwalls, const dem_wall_t* walls,
+                         const int64_t* counts, const dem_grain_in_t* grains,
+                         const dem_options_t* opt, dem_scene_t** out);
 
 /* ---- synthetic inputs (host only; no counterpart in the reference) -------------------------
  * Loose polydisperse packing by random sequential addition: log-normal radii (median, sigma)

@@ -172,6 +172,10 @@ SIGNATURES = [
     ("dem_batch_bonds_broken", ctypes.c_int32, [ctypes.c_void_p, P(ctypes.c_int64)]),
     ("dem_batch_adopt", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
                                          ctypes.c_int32]),
+    ("dem_batch_repack", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32, P(ctypes.c_int32),
+                                          ctypes.c_int32, P(DemConfig), P(ctypes.c_int32),
+                                          P(DemWall), P(ctypes.c_int64), P(DemGrainIn),
+                                          P(DemOptions), P(ctypes.c_void_p)]),
     ("dem_generate_pour", ctypes.c_int32, [ctypes.c_int64, P(ctypes.c_double), P(ctypes.c_double), ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_uint64, P(DemGrainIn)]),

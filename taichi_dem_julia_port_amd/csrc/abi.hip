@@ -43,20 +43,55 @@ hipError_t dalloc(T** p, size_t count) {
     return e;
 }
 
+// Sub-arrays of one device allocation, each 256-byte aligned.  Laid out once with a null base to
+// size the allocation, then again over it.
+struct Carve {
+    char* base;
+    size_t off = 0;
+    explicit Carve(void* b) : base((char*)b) {}
+    template <typename T>
+    void take(T** p, int64_t count) {
+        *p = (T*)(base + off);
+        off += ((size_t)std::max<int64_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    }
+};
+
+// The contact rows of K slots per particle, zeroed.
+hipError_t alloc_rows(dem_scene* s, int32_t K) {
+    DevState& d = s->d;
+    const int64_t KN = (int64_t)K * s->n;
+    auto layout = [&](void* base) {
+        Carve c(base);
+        c.take(&d.touch, KN);
+        for (int b = 0; b < 2; ++b) { c.take(&d.pq[b], KN); c.take(&d.st[b], (int64_t)S_NCOMP * KN); }
+        return c.off;
+    };
+    const size_t bytes = layout(nullptr);
+    hipError_t e = hipMalloc(&d.rows_arena, bytes);
+    if (e != hipSuccess) { d.rows_arena = nullptr; return e; }
+    layout(d.rows_arena);
+    return hipMemset(d.rows_arena, 0, bytes);
+}
+
+// Copies between host memory and a scene's device arrays, on the scene's stream and complete on
+// return: ordered after every launch on that stream, and finished before the next one.
+hipError_t to_host(const dem_scene* s, void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, s->stream);
+    return e == hipSuccess ? hipStreamSynchronize(s->stream) : e;
+}
+hipError_t to_device(const dem_scene* s, void* dst, const void* src, size_t bytes) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s->stream);
+    return e == hipSuccess ? hipStreamSynchronize(s->stream) : e;
+}
+
 // executor.jl:223 r = cbrt(V / (4/3 pi))
 double grain_radius(double V) { return cbrt(V / (4.0 / 3.0 * M_PI)); }
 
 void free_scene(dem_scene* s) {
     if (!s) return;
     DevState& d = s->d;
-    void* ptrs[] = {d.pos[0], d.pos[1], d.vel[0], d.vel[1], d.omg[0], d.omg[1], d.quat[0], d.quat[1],
-                    d.acc, d.dw, d.ftot, d.ttot, d.r0, d.wact, d.wshear, d.hid, d.rank, d.x32, d.bcnt,
-                    d.bstart, d.sidxb[0], d.sidxb[1], d.skey, d.eid, d.spos, d.spos32, d.tcnt, d.touch, d.sel,
-                    d.rcnt[0], d.rcnt[1],
-                    d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, d.scan.status, d.scan.counters, d.scan.tagg,
-                    d.scratch_u32, d.scratch2_u32, d.total, d.seg, d.sid, d.seg_broken};
-    for (void* p : ptrs)
-        if (p) hipFree(p);
+    if (d.arena) hipFree(d.arena);
+    if (d.rows_arena) hipFree(d.rows_arena);
     if (s->h_ctr) hipHostFree(s->h_ctr);
     if (s->h_ctr_raw) hipHostFree(s->h_ctr_raw);
     for (hipEvent_t e : s->ev)
@@ -104,16 +139,11 @@ int32_t collect_timing(dem_scene* s) { return s->timing ? timing_flush(s) : DEM_
 int32_t resize_rows(dem_scene* s, int32_t k) {
     DevState& d = s->d;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    void* old[] = {d.touch, d.pq[0], d.pq[1], d.st[0], d.st[1]};
-    for (void* p : old) if (p) HIPCHK(s, hipFree(p));
+    if (d.rows_arena) HIPCHK(s, hipFree(d.rows_arena));
+    d.rows_arena = nullptr;
     d.touch = nullptr; d.pq[0] = d.pq[1] = nullptr; d.st[0] = d.st[1] = nullptr;
-    const int64_t KN = (int64_t)k * s->n;
-    HIPCHK(s, dalloc(&d.touch, KN));
-    for (int b = 0; b < 2; ++b) {
-        HIPCHK(s, dalloc(&d.pq[b], KN));
-        HIPCHK(s, dalloc(&d.st[b], (int64_t)S_NCOMP * KN));
-        HIPCHK(s, hipMemset(d.rcnt[b], 0, s->n * sizeof(uint32_t)));
-    }
+    HIPCHK(s, alloc_rows(s, k));
+    for (int b = 0; b < 2; ++b) HIPCHK(s, hipMemset(d.rcnt[b], 0, s->n * sizeof(uint32_t)));
     HIPCHK(s, hipMemset(d.sel, 0, s->n));
     HIPCHK(s, hipMemset(d.tcnt, 0, s->n * sizeof(uint32_t)));
     HIPCHK(s, hipMemset(d.ctr, 0, CTR_WORDS * sizeof(unsigned long long)));
@@ -289,14 +319,15 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
 // the scene's per-step dense grid (or in the reference grid when the dense grid is unused), x
 // fastest, ties by file index.  Neighbours in space become neighbours in memory, so the sort,
 // the broad phase and the partner gathers of the force kernel read nearby addresses.
-void spatial_order(dem_scene* s, const dem_grain_in_t* g) {
+// Orders scenes k0.. only, whose input rows (file index gbase..) grows holds.
+void spatial_order(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase, int k0) {
     const int64_t n = s->n;
     s->perm.resize(n);
     s->iperm.resize(n);
     const int nseg = s->batched ? s->nseg : 1;
     std::vector<uint64_t> key;
     std::vector<uint32_t> cnt;
-    for (int k = 0; k < nseg; ++k) {
+    for (int k = k0; k < nseg; ++k) {
         const int64_t a = s->batched ? s->seg_off[k] : 0, b = s->batched ? s->seg_off[k + 1] : n;
         const SceneP& S = s->batched ? s->seg_host[k] : s->P.sc;
         if (S.dense.cells > 0) {
@@ -307,7 +338,7 @@ void spatial_order(dem_scene* s, const dem_grain_in_t* g) {
             for (int64_t i = a; i < b; ++i) {
                 uint64_t c[3];
                 for (int ax = 0; ax < 3; ++ax) {
-                    const double v = (g[i].k[ax] - G.origin[ax]) * G.inv_cell;
+                    const double v = (grows[i - gbase].k[ax] - G.origin[ax]) * G.inv_cell;
                     int64_t q = std::isfinite(v) ? (int64_t)std::floor(std::max(-1.0, std::min(v, 4.0e9))) : 0;
                     q = q < 0 ? 0 : (q >= G.dims[ax] ? G.dims[ax] - 1 : q);
                     c[ax] = (uint64_t)q;
@@ -324,7 +355,7 @@ void spatial_order(dem_scene* s, const dem_grain_in_t* g) {
             for (int64_t i = a; i < b; ++i) {
                 uint64_t c[3];
                 for (int ax = 0; ax < 3; ++ax) {
-                    const double v = (g[i].k[ax] - S.grid.dmin[ax]) / S.grid.cs;
+                    const double v = (grows[i - gbase].k[ax] - S.grid.dmin[ax]) / S.grid.cs;
                     int64_t q = std::isfinite(v) ? (int64_t)std::floor(std::max(-1.0, std::min(v, 4.0e6))) : 0;
                     q = q < 0 ? 0 : (q > (1 << 21) - 1 ? (1 << 21) - 1 : q);
                     c[ax] = (uint64_t)q;
@@ -337,41 +368,51 @@ void spatial_order(dem_scene* s, const dem_grain_in_t* g) {
             std::copy(idx.begin(), idx.end(), s->perm.begin() + a);
         }
     }
-    for (int64_t u = 0; u < n; ++u) s->iperm[s->perm[u]] = (uint32_t)u;
+    for (int64_t u = gbase; u < n; ++u) s->iperm[s->perm[u]] = (uint32_t)u;
 }
 
-// Device allocation and initial upload (executor.jl:229-242: q = 1, w = a = dw = 0) for the n
-// grains already derived into the host arrays.
-int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max, int64_t nkeys) {
+// Device allocation and initial state (executor.jl:229-242: q = 1, w = a = dw = 0) of the grains
+// already derived into the host arrays.  grows holds the input rows of file index gbase.. (scenes
+// k0..); particles below gbase are left for the caller to move in (dem_batch_repack).
+int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase, int k0,
+                       int32_t nwall_max, int64_t nkeys) {
     DevState& d = s->d;
     const int64_t n = s->n;
     const int64_t KN = (int64_t)s->K * n;
     hipError_t e = hipSuccess;
-#define ALLOC(ptr, cnt)                                                                   \
-    if (e == hipSuccess) {                                                                \
-        e = dalloc(&(ptr), (size_t)(cnt));                                                \
-        if (e != hipSuccess)                                                              \
-            return fail(nullptr, DEM_ERR_HIP, std::string("device allocation of " #ptr    \
-                                                          " failed: ") + hipGetErrorString(e)); \
-    }
-    for (int b = 0; b < 2; ++b) {
-        ALLOC(d.pos[b], n); ALLOC(d.vel[b], n); ALLOC(d.omg[b], n); ALLOC(d.quat[b], n);
-        ALLOC(d.rcnt[b], n); ALLOC(d.pq[b], KN); ALLOC(d.st[b], (int64_t)S_NCOMP * KN);
-    }
-    ALLOC(d.acc, n); ALLOC(d.dw, n); ALLOC(d.ftot, n); ALLOC(d.ttot, n); ALLOC(d.r0, n);
-    ALLOC(d.wact, n); ALLOC(d.wshear, (int64_t)(nwall_max > 0 ? nwall_max : 1) * n);
-    ALLOC(d.hid, n); ALLOC(d.rank, n); ALLOC(d.x32, n); ALLOC(d.bcnt, nkeys + 1);
-    ALLOC(d.bstart, nkeys + 1); ALLOC(d.sidxb[0], n); ALLOC(d.sidxb[1], n); ALLOC(d.skey, n);
-    ALLOC(d.spos, n); ALLOC(d.spos32, n); ALLOC(d.eid, n);
-    d.sidx = d.sidxb[0];
-    ALLOC(d.tcnt, n); ALLOC(d.touch, KN); ALLOC(d.sel, n);
-    ALLOC(d.ctr, CTR_WORDS);
     const int64_t max_tiles = scan_tiles_max(std::max<int64_t>(nkeys + 1, n + 1));
-    ALLOC(d.scan.status, max_tiles); ALLOC(d.scan.counters, 2); ALLOC(d.scan.tagg, max_tiles * TAGG_COPIES);
+    auto layout = [&](void* base) {
+        Carve c(base);
+        for (int b = 0; b < 2; ++b) {
+            c.take(&d.pos[b], n); c.take(&d.vel[b], n); c.take(&d.omg[b], n); c.take(&d.quat[b], n);
+            c.take(&d.rcnt[b], n);
+        }
+        c.take(&d.acc, n); c.take(&d.dw, n); c.take(&d.ftot, n); c.take(&d.ttot, n); c.take(&d.r0, n);
+        c.take(&d.wact, n); c.take(&d.wshear, (int64_t)(nwall_max > 0 ? nwall_max : 1) * n);
+        c.take(&d.hid, n); c.take(&d.rank, n); c.take(&d.x32, n); c.take(&d.bcnt, nkeys + 1);
+        c.take(&d.bstart, nkeys + 1); c.take(&d.sidxb[0], n); c.take(&d.sidxb[1], n); c.take(&d.skey, n);
+        c.take(&d.spos, n); c.take(&d.spos32, n); c.take(&d.eid, n);
+        c.take(&d.tcnt, n); c.take(&d.sel, n);
+        c.take(&d.ctr, CTR_WORDS);
+        c.take(&d.scan.status, max_tiles); c.take(&d.scan.counters, 2);
+        c.take(&d.scan.tagg, max_tiles * TAGG_COPIES);
+        c.take(&d.scratch_u32, n + 1); c.take(&d.scratch2_u32, n + 1); c.take(&d.total, 1);
+        if (s->batched) { c.take(&d.seg, s->nseg); c.take(&d.sid, n); c.take(&d.seg_broken, s->nseg); }
+        return c.off;
+    };
+    const size_t bytes = layout(nullptr);
+    e = hipMalloc(&d.arena, bytes);
+    if (e != hipSuccess) {
+        d.arena = nullptr;
+        return fail(nullptr, DEM_ERR_HIP, std::string("device allocation failed: ") + hipGetErrorString(e));
+    }
+    layout(d.arena);
+    d.sidx = d.sidxb[0];
     d.scan.max_tiles = max_tiles;
-    ALLOC(d.scratch_u32, n + 1); ALLOC(d.scratch2_u32, n + 1); ALLOC(d.total, 1);
-    if (s->batched) { ALLOC(d.seg, s->nseg); ALLOC(d.sid, n); ALLOC(d.seg_broken, s->nseg); }
-#undef ALLOC
+    e = hipMemset(d.arena, 0, bytes);
+    if (e == hipSuccess) e = alloc_rows(s, s->K);
+    if (e != hipSuccess)
+        return fail(nullptr, DEM_ERR_HIP, std::string("device allocation failed: ") + hipGetErrorString(e));
     if (hipHostMalloc((void**)&s->h_ctr, sizeof(unsigned long long) * CT_NCOUNTERS) != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, "hipHostMalloc failed");
     memset(s->h_ctr, 0, sizeof(unsigned long long) * CT_NCOUNTERS);
@@ -380,38 +421,36 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* g, int32_t nwall_max,
     memset(s->h_ctr_raw, 0, sizeof(unsigned long long) * CTR_WORDS);
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, "hipStreamCreate failed");
-    spatial_order(s, g);
-    std::vector<double4> hp(n), hv(n), hw(n), hq(n);
-    std::vector<double> hr0(n);
-    for (int64_t u = 0; u < n; ++u) {
-        const int64_t i = s->perm[u];
-        hp[u] = make_double4(g[i].k[0], g[i].k[1], g[i].k[2], s->h_r[i]);
-        hv[u] = make_double4(g[i].v[0], g[i].v[1], g[i].v[2], g[i].m);
-        hw[u] = make_double4(0.0, 0.0, 0.0, s->h_I[i]);
-        hq[u] = make_double4(1.0, 0.0, 0.0, 0.0);
-        hr0[u] = s->h_r0[i];
+    spatial_order(s, grows, gbase, k0);
+    // one staging allocation: input rows, radii and inertias of the new grains, r0 of all grains
+    // (file order) and the scene offsets; the gather into internal order runs on the device
+    const int64_t nn = n - gbase;
+    const size_t b_rows = (size_t)nn * sizeof(dem_grain_in_t), b_rI = (size_t)nn * sizeof(double);
+    const size_t b_r0 = (size_t)n * sizeof(double), b_off = (size_t)(s->nseg + 1) * sizeof(int64_t);
+    char* tmp = nullptr;
+    e = hipMalloc((void**)&tmp, b_rows + 2 * b_rI + b_r0 + b_off);
+    if (e != hipSuccess) return fail(nullptr, DEM_ERR_HIP, std::string("staging allocation failed: ") + hipGetErrorString(e));
+    dem_grain_in_t* t_rows = (dem_grain_in_t*)tmp;
+    double* t_r = (double*)(tmp + b_rows);
+    double* t_I = t_r + nn;
+    double* t_r0 = t_I + nn;
+    int64_t* t_off = (int64_t*)(t_r0 + n);
+    if (nn > 0) {
+        e = hipMemcpy(t_rows, grows, b_rows, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(t_r, s->h_r + gbase, b_rI, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(t_I, s->h_I + gbase, b_rI, hipMemcpyHostToDevice);
     }
-    e = hipMemcpy(d.pos[0], hp.data(), n * sizeof(double4), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.vel[0], hv.data(), n * sizeof(double4), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.omg[0], hw.data(), n * sizeof(double4), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.quat[0], hq.data(), n * sizeof(double4), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.r0, hr0.data(), n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_r0, s->h_r0, b_r0, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t_off, s->seg_off.data(), b_off, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(d.eid, s->perm.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) {  // the first sort starts from the internal (spatial) order
-        std::vector<uint32_t> iota(n);
-        for (int64_t i = 0; i < n; ++i) iota[i] = (uint32_t)i;
-        e = hipMemcpy(d.sidxb[0], iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(d.sidxb[1], iota.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
-    }
-    if (e == hipSuccess && s->batched) {
-        std::vector<uint16_t> sid(n);
-        for (int k = 0; k < s->nseg; ++k)
-            for (int64_t i = s->seg_off[k]; i < s->seg_off[k + 1]; ++i) sid[i] = (uint16_t)k;
-        e = hipMemcpy(d.sid, sid.data(), n * sizeof(uint16_t), hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(d.seg, s->seg_host.data(), s->nseg * sizeof(SceneP), hipMemcpyHostToDevice);
-    }
+    if (e == hipSuccess && s->batched)
+        e = hipMemcpy(d.seg, s->seg_host.data(), s->nseg * sizeof(SceneP), hipMemcpyHostToDevice);
+    // the zero fills and copies above run on the null stream, which the scene's non-blocking
+    // stream does not wait for
     if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = run_init_state(s, t_rows, t_r, t_I, t_r0, t_off, gbase);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    hipFree(tmp);
     if (e != hipSuccess) return fail(nullptr, DEM_ERR_HIP, std::string("initial upload failed: ") + hipGetErrorString(e));
     s->P.seg = d.seg;
     s->P.sid = d.sid;
@@ -475,6 +514,8 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
         return rc;
     }
     s->der.contact_slots = K;
+    s->seg_cfg = {*cfg};
+    s->seg_der = {s->der};
     Params& P = s->P;
     P.n = n;
     P.K = K;
@@ -484,7 +525,7 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     P.sort_keys_init = s->dense ? P.sc.dense_init.cells : (uint32_t)P.sc.grid.H;
     const int64_t nkeys = std::max<int64_t>(
         (int64_t)P.sc.grid.H, std::max<int64_t>((int64_t)P.sc.dense.cells, (int64_t)P.sc.dense_init.cells));
-    if ((rc = allocate_scene(s, g, nwall, nkeys))) {
+    if ((rc = allocate_scene(s, g, 0, 0, nwall, nkeys))) {
         free_scene(s);
         return rc;
     }
@@ -534,6 +575,8 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
     s->nseg = nscenes;
     s->seg_off.assign(nscenes + 1, 0);
     s->seg_host.resize(nscenes);
+    s->seg_cfg.resize(nscenes);
+    s->seg_der.resize(nscenes);
     alloc_host_fields(s, total);
     int64_t off = 0;
     uint64_t keys = 0, keys_init = 0;
@@ -552,6 +595,8 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
         S.cell_base_init = (uint32_t)keys_init;
         keys_init += S.dense_init.cells;
         if (k == 0) s->der = der;
+        s->seg_cfg[k] = cfgs[k];
+        s->seg_der[k] = der;
         s->seg_off[k] = off;
         off += counts[k];
         woff += nwalls[k];
@@ -570,7 +615,7 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
     s->dense = true;
     P.sort_keys = (uint32_t)keys;
     P.sort_keys_init = (uint32_t)keys_init;
-    if ((rc = allocate_scene(s, grains, nwall_max, (int64_t)std::max(keys, keys_init)))) {
+    if ((rc = allocate_scene(s, grains, 0, 0, nwall_max, (int64_t)std::max(keys, keys_init)))) {
         free_scene(s);
         return rc;
     }
@@ -594,7 +639,7 @@ int32_t dem_batch_set_bond_init(dem_scene_t* s, const uint8_t* mask) {
     for (int k = 0; k < s->nseg; ++k) s->seg_host[k].bond_init = mask[k] ? 1 : 0;
     s->P.sc = s->seg_host[0];
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(s->d.seg, s->seg_host.data(), s->nseg * sizeof(SceneP), hipMemcpyHostToDevice));
+    HIPCHK(s, to_device(s, s->d.seg, s->seg_host.data(), s->nseg * sizeof(SceneP)));
     return DEM_OK;
 }
 
@@ -626,30 +671,211 @@ int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, i
         e = run_adopt(dst, dst->seg_off[dst_scene], src, src->seg_off[src_scene], n, Ss.nwall, ovf);
     uint32_t o = 0;
     if (e == hipSuccess) e = hipStreamSynchronize(dst->stream);
-    if (e == hipSuccess) e = hipMemcpy(&o, ovf, sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = to_host(dst, &o, ovf, sizeof(uint32_t));
     hipFree(ovf);
     if (e != hipSuccess) return hip_fail(dst, "dem_batch_adopt", e);
     if (o) return fail(dst, DEM_ERR_CAPACITY, "destination has fewer contact slots than needed");
     // cumulative bond breaks of the scene travel with it
     unsigned long long nb = 0;
     if (src->batched) {
-        HIPCHK(dst, hipMemcpy(&nb, src->d.seg_broken + src_scene, sizeof(nb), hipMemcpyDeviceToHost));
+        HIPCHK(dst, to_host(src, &nb, src->d.seg_broken + src_scene, sizeof(nb)));
     } else {
         int32_t rc = sync_counters(src);
         if (rc) return fail(dst, rc, "dem_batch_adopt: reading the source counters failed");
         nb = src->h_ctr[CT_BROKEN_TOTAL];
     }
     if (dst->batched) {
-        HIPCHK(dst, hipMemcpy(dst->d.seg_broken + dst_scene, &nb, sizeof(nb), hipMemcpyHostToDevice));
+        HIPCHK(dst, to_device(dst, dst->d.seg_broken + dst_scene, &nb, sizeof(nb)));
     } else {
         // the total goes into copy 0, the other copies restart from zero
         std::vector<unsigned long long> raw(CTR_WORDS);
-        HIPCHK(dst, hipMemcpy(raw.data(), dst->d.ctr, CTR_WORDS * sizeof(unsigned long long),
-                              hipMemcpyDeviceToHost));
+        HIPCHK(dst, to_host(dst, raw.data(), dst->d.ctr, CTR_WORDS * sizeof(unsigned long long)));
         for (int c = 0; c < CTR_COPIES; ++c) raw[c * CTR_STRIDE + CT_BROKEN_TOTAL] = c == 0 ? nb : 0ull;
-        HIPCHK(dst, hipMemcpy(dst->d.ctr, raw.data(), CTR_WORDS * sizeof(unsigned long long),
-                              hipMemcpyHostToDevice));
+        HIPCHK(dst, to_device(dst, dst->d.ctr, raw.data(), CTR_WORDS * sizeof(unsigned long long)));
     }
+    return DEM_OK;
+}
+
+int32_t dem_batch_repack(dem_scene_t* src, int32_t nkeep, const int32_t* keep, int32_t nnew,
+                         const dem_config_t* cfgs, const int32_t* nwalls, const dem_wall_t* walls,
+                         const int64_t* counts, const dem_grain_in_t* grains,
+                         const dem_options_t* opt, dem_scene_t** out) {
+    g_create_err.clear();
+    if (!out) return fail(nullptr, DEM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    if (!src) return fail(nullptr, DEM_ERR_INVALID, "src is NULL");
+    if (nkeep < 0 || nnew < 0 || nkeep > src->nseg || (nkeep > 0 && !keep) || nkeep + nnew < 1 ||
+        nkeep + nnew > 65535)
+        return fail(nullptr, DEM_ERR_INVALID, "need 1..65535 scenes: kept scenes of src plus new ones");
+    if (nnew > 0 && (!cfgs || !nwalls || !counts || !grains))
+        return fail(nullptr, DEM_ERR_INVALID, "new scenes need configs, wall counts and grains");
+    dem_options_t o;
+    int32_t rc = options_of(opt, o);
+    if (rc) return rc;
+    if (o.flags & (DEM_FLAG_REFERENCE_GRID | DEM_FLAG_INDEX_OWNED_BROAD_PHASE))
+        return fail(nullptr, DEM_ERR_INVALID, "batches always use the dense sort grid");
+    if (o.device != src->device) return fail(nullptr, DEM_ERR_INVALID, "src is on another device");
+    const int32_t nscenes = nkeep + nnew;
+    std::vector<char> seen(src->nseg, 0);
+    int64_t total = 0, woff = 0;
+    int32_t nwall_max = 0, maxcoord = 0;
+    for (int j = 0; j < nkeep; ++j) {
+        const int32_t k = keep[j];
+        if (k < 0 || k >= src->nseg || seen[k])
+            return fail(nullptr, DEM_ERR_INVALID, "keep lists a scene index out of range or twice");
+        seen[k] = 1;
+        const SceneP& Sk = src->batched ? src->seg_host[k] : src->P.sc;
+        if (Sk.dense.cells == 0 || Sk.dense_init.cells == 0)
+            return fail(nullptr, DEM_ERR_INVALID, "a kept scene does not use the dense sort grid");
+        total += src->seg_off[k + 1] - src->seg_off[k];
+        nwall_max = std::max(nwall_max, Sk.nwall);
+        maxcoord = std::max(maxcoord, src->seg_cfg[k].max_coordinate_number);
+    }
+    const int64_t kept_total = total;
+    int64_t goff = 0;
+    for (int t = 0; t < nnew; ++t) {
+        if ((rc = check_scene(&cfgs[t], walls ? walls + woff : nullptr, nwalls[t], grains + goff, counts[t])))
+            return rc;
+        goff += counts[t];
+        woff += nwalls[t];
+        nwall_max = std::max(nwall_max, nwalls[t]);
+        maxcoord = std::max(maxcoord, cfgs[t].max_coordinate_number);
+    }
+    total += goff;
+    if (total > (int64_t)IDX_MASK) return fail(nullptr, DEM_ERR_INVALID, "too many particles");
+    int32_t K = slots_of(o, maxcoord);
+    // rows that dem_initialize widened stay wide: the kept scenes' bonds must fit
+    if (o.contact_slots <= 0 && nkeep > 0) K = std::max(K, src->K);
+    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
+    if (hipSetDevice(o.device) != hipSuccess) return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
+
+    dem_scene* s = new dem_scene();
+    s->n = total;
+    s->nwall = nwall_max;
+    s->precision = o.precision;
+    s->K = K;
+    s->device = o.device;
+    s->use_graph = !(o.flags & DEM_FLAG_NO_GRAPH);
+    s->batched = true;
+    s->nseg = nscenes;
+    s->seg_off.assign(nscenes + 1, 0);
+    s->seg_host.resize(nscenes);
+    s->seg_cfg.resize(nscenes);
+    s->seg_der.resize(nscenes);
+    s->perm.resize(total);
+    s->iperm.resize(total);
+    alloc_host_fields(s, total);
+    int64_t off = 0;
+    uint64_t keys = 0, keys_init = 0;
+    auto place = [&](int j, int64_t m) {
+        SceneP& S = s->seg_host[j];
+        S.cell_base = (uint32_t)keys;
+        keys += S.dense.cells;
+        S.cell_base_init = (uint32_t)keys_init;
+        keys_init += S.dense_init.cells;
+        s->seg_off[j] = off;
+        off += m;
+    };
+    // kept scenes: their derived host fields, order and parameters move over unchanged
+    for (int j = 0; j < nkeep; ++j) {
+        const int32_t k = keep[j];
+        const int64_t a = src->seg_off[k], m = src->seg_off[k + 1] - a;
+        s->seg_host[j] = src->batched ? src->seg_host[k] : src->P.sc;
+        s->seg_host[j].bond_init = 0;
+        s->seg_cfg[j] = src->seg_cfg[k];
+        s->seg_der[j] = src->seg_der[k];
+        std::copy(src->h_id + a, src->h_id + a + m, s->h_id + off);
+        std::copy(src->h_gid + a, src->h_gid + a + m, s->h_gid + off);
+        std::copy(src->h_mid + a, src->h_mid + a + m, s->h_mid + off);
+        std::copy(src->h_V + a, src->h_V + a + m, s->h_V + off);
+        std::copy(src->h_m + a, src->h_m + a + m, s->h_m + off);
+        std::copy(src->h_r + a, src->h_r + a + m, s->h_r + off);
+        std::copy(src->h_r0 + a, src->h_r0 + a + m, s->h_r0 + off);
+        std::copy(src->h_I + a, src->h_I + a + m, s->h_I + off);
+        for (int64_t u = 0; u < m; ++u) {
+            s->perm[off + u] = (uint32_t)(src->perm[a + u] - a + off);
+            s->iperm[s->perm[off + u]] = (uint32_t)(off + u);
+        }
+        place(j, m);
+    }
+    // new scenes: derived from their input rows as dem_create_batch does
+    woff = 0;
+    goff = 0;
+    for (int t = 0; t < nnew; ++t) {
+        const int j = nkeep + t;
+        bool slot_owned = false;
+        if ((rc = derive_scene(s, off, &cfgs[t], walls ? walls + woff : nullptr, nwalls[t], grains + goff,
+                               counts[t], true, true, o.flags, s->seg_host[j], s->seg_der[j], slot_owned))) {
+            free_scene(s);
+            return rc;
+        }
+        s->seg_cfg[j] = cfgs[t];
+        place(j, counts[t]);
+        goff += counts[t];
+        woff += nwalls[t];
+    }
+    s->seg_off[nscenes] = off;
+    if (keys >= (1ull << 31) || keys_init >= (1ull << 31)) {
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_INVALID, "batch sort grid too large");
+    }
+    s->cfg = s->seg_cfg[0];
+    s->der = s->seg_der[0];
+    s->der.contact_slots = K;
+    Params& P = s->P;
+    P.n = total;
+    P.K = K;
+    P.nseg = nscenes;
+    P.sc = s->seg_host[0];
+    s->dense = true;
+    P.sort_keys = (uint32_t)keys;
+    P.sort_keys_init = (uint32_t)keys_init;
+    if ((rc = allocate_scene(s, grains, kept_total, nkeep, nwall_max, (int64_t)std::max(keys, keys_init)))) {
+        free_scene(s);
+        return rc;
+    }
+    // the kept scenes' state, contact rows and bond-break totals, all on the new batch's stream
+    std::vector<unsigned long long> nb(nscenes, 0ull);
+    hipError_t e = hipStreamSynchronize(src->stream);
+    if (e == hipSuccess && nkeep > 0) {
+        if (src->batched) {
+            std::vector<unsigned long long> v(src->nseg);
+            e = to_host(src, v.data(), src->d.seg_broken, src->nseg * sizeof(unsigned long long));
+            for (int j = 0; j < nkeep; ++j) nb[j] = v[keep[j]];
+        } else {
+            rc = sync_counters(src);
+            if (rc) {
+                free_scene(s);
+                return fail(nullptr, rc, "dem_batch_repack: reading the source counters failed");
+            }
+            nb[0] = src->h_ctr[CT_BROKEN_TOTAL];
+        }
+    }
+    uint32_t* ovf = nullptr;
+    if (e == hipSuccess) e = hipMalloc(&ovf, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemsetAsync(ovf, 0, sizeof(uint32_t), s->stream);
+    for (int j = 0; j < nkeep && e == hipSuccess; ++j) {
+        const int32_t k = keep[j];
+        const SceneP& Sk = s->seg_host[j];
+        e = run_adopt(s, s->seg_off[j], src, src->seg_off[k], src->seg_off[k + 1] - src->seg_off[k],
+                      Sk.nwall, ovf);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d.seg_broken, nb.data(), nscenes * sizeof(unsigned long long),
+                           hipMemcpyHostToDevice, s->stream);
+    uint32_t o_flag = 0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&o_flag, ovf, sizeof(uint32_t), hipMemcpyDeviceToHost, s->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (ovf) hipFree(ovf);
+    if (e != hipSuccess) {
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_HIP, std::string("dem_batch_repack: ") + hipGetErrorString(e));
+    }
+    if (o_flag) {
+        free_scene(s);
+        return fail(nullptr, DEM_ERR_CAPACITY, "dem_batch_repack: a kept scene has more contacts than the new rows hold");
+    }
+    *out = s;
     return DEM_OK;
 }
 
@@ -664,7 +890,7 @@ int32_t dem_batch_bonds_broken(dem_scene_t* s, int64_t* out) {
     }
     std::vector<unsigned long long> v(s->nseg);
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(v.data(), s->d.seg_broken, s->nseg * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    HIPCHK(s, to_host(s, v.data(), s->d.seg_broken, s->nseg * sizeof(unsigned long long)));
     for (int k = 0; k < s->nseg; ++k) out[k] = (int64_t)v[k];
     return DEM_OK;
 }
@@ -790,12 +1016,12 @@ int32_t dem_read_grains(dem_scene_t* s, dem_grain_t* out, int64_t n) {
     const DevState& d = s->d;
     const int c = s->cur;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(p.data(), d.pos[c], n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(v.data(), d.vel[c], n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(w.data(), d.omg[c], n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(q.data(), d.quat[c], n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(a.data(), d.acc, n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(dw.data(), d.dw, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, to_host(s, p.data(), d.pos[c], n * sizeof(double4)));
+    HIPCHK(s, to_host(s, v.data(), d.vel[c], n * sizeof(double4)));
+    HIPCHK(s, to_host(s, w.data(), d.omg[c], n * sizeof(double4)));
+    HIPCHK(s, to_host(s, q.data(), d.quat[c], n * sizeof(double4)));
+    HIPCHK(s, to_host(s, a.data(), d.acc, n * sizeof(double4)));
+    HIPCHK(s, to_host(s, dw.data(), d.dw, n * sizeof(double4)));
     for (int64_t i = 0; i < n; ++i) {
         const int64_t u = s->iperm[i];
         dem_grain_t& o = out[i];
@@ -830,12 +1056,12 @@ int32_t dem_write_grains(dem_scene_t* s, const dem_grain_t* in, int64_t n) {
     DevState& d = s->d;
     const int c = s->cur;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(d.pos[c], p.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(d.vel[c], v.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(d.omg[c], w.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(d.quat[c], q.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(d.acc, a.data(), n * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(d.dw, dw.data(), n * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(s, to_device(s, d.pos[c], p.data(), n * sizeof(double4)));
+    HIPCHK(s, to_device(s, d.vel[c], v.data(), n * sizeof(double4)));
+    HIPCHK(s, to_device(s, d.omg[c], w.data(), n * sizeof(double4)));
+    HIPCHK(s, to_device(s, d.quat[c], q.data(), n * sizeof(double4)));
+    HIPCHK(s, to_device(s, d.acc, a.data(), n * sizeof(double4)));
+    HIPCHK(s, to_device(s, d.dw, dw.data(), n * sizeof(double4)));
     return DEM_OK;
 }
 
@@ -847,14 +1073,15 @@ int32_t dem_read_contacts(dem_scene_t* s, dem_contact_t* out, uint8_t* bonded, i
     uint8_t* dbd = nullptr;
     int64_t total = 0;
     HIPCHK(s, run_read_contacts(s, &dout, &dbd, &total));
+    HIPCHK(s, hipStreamSynchronize(s->stream));  // the records are written on the scene's stream
     const int64_t w = std::min(cap, total);
     hipError_t e = hipSuccess;
     if (w > 0) {
         // records come out in internal row order: sort by (i, j) in file indices
         std::vector<dem_contact_t> all(total);
         std::vector<uint8_t> bd(total);
-        e = hipMemcpy(all.data(), dout, total * sizeof(dem_contact_t), hipMemcpyDeviceToHost);
-        if (e == hipSuccess) e = hipMemcpy(bd.data(), dbd, total, hipMemcpyDeviceToHost);
+        e = to_host(s, all.data(), dout, total * sizeof(dem_contact_t));
+        if (e == hipSuccess) e = to_host(s, bd.data(), dbd, total);
         if (e == hipSuccess) {
             std::vector<int64_t> ord(total);
             for (int64_t k = 0; k < total; ++k) ord[k] = k;
@@ -923,10 +1150,10 @@ int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_
     HIPCHK(s, hipMalloc(&ddst, me * sizeof(uint64_t)));
     HIPCHK(s, hipMalloc(&dword, me * sizeof(uint32_t)));
     HIPCHK(s, hipMalloc(&dstate, me * S_NCOMP * sizeof(double)));
-    hipError_t e = hipMemcpy(ddst, dst.data(), ents.size() * sizeof(uint64_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dword, word.data(), ents.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(dstate, state.data(), ents.size() * S_NCOMP * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(d.rcnt[0], rc.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+    hipError_t e = to_device(s, ddst, dst.data(), ents.size() * sizeof(uint64_t));
+    if (e == hipSuccess) e = to_device(s, dword, word.data(), ents.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = to_device(s, dstate, state.data(), ents.size() * S_NCOMP * sizeof(double));
+    if (e == hipSuccess) e = to_device(s, d.rcnt[0], rc.data(), n * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(d.rcnt[1], 0, n * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(d.sel, 0, n);
     if (e == hipSuccess) e = run_write_contacts(s, ddst, dword, dstate, (int64_t)ents.size());
@@ -944,8 +1171,8 @@ int32_t dem_read_wall_shear(dem_scene_t* s, double* out, int64_t n) {
     std::vector<double4> ws((size_t)nw * n);
     std::vector<uint32_t> act(n);
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(ws.data(), s->d.wshear, ws.size() * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(act.data(), s->d.wact, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(s, to_host(s, ws.data(), s->d.wshear, ws.size() * sizeof(double4)));
+    HIPCHK(s, to_host(s, act.data(), s->d.wact, n * sizeof(uint32_t)));
     for (int64_t i = 0; i < n; ++i)
         for (int j = 0; j < nw; ++j) {
             const int64_t u = s->iperm[i];
@@ -972,8 +1199,8 @@ int32_t dem_write_wall_shear(dem_scene_t* s, const double* in, int64_t n) {
             if (v[0] != 0.0 || v[1] != 0.0 || v[2] != 0.0) act[u] |= 1u << j;
         }
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(s->d.wshear, ws.data(), ws.size() * sizeof(double4), hipMemcpyHostToDevice));
-    HIPCHK(s, hipMemcpy(s->d.wact, act.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    HIPCHK(s, to_device(s, s->d.wshear, ws.data(), ws.size() * sizeof(double4)));
+    HIPCHK(s, to_device(s, s->d.wact, act.data(), n * sizeof(uint32_t)));
     return DEM_OK;
 }
 
@@ -982,8 +1209,8 @@ int32_t dem_read_forces(dem_scene_t* s, double* F, double* T, int64_t n) {
     HIPCHK(s, hipSetDevice(s->device));
     std::vector<double4> f(n), t(n);
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    HIPCHK(s, hipMemcpy(f.data(), s->d.ftot, n * sizeof(double4), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(t.data(), s->d.ttot, n * sizeof(double4), hipMemcpyDeviceToHost));
+    HIPCHK(s, to_host(s, f.data(), s->d.ftot, n * sizeof(double4)));
+    HIPCHK(s, to_host(s, t.data(), s->d.ttot, n * sizeof(double4)));
     for (int64_t i = 0; i < n; ++i) {
         const int64_t u = s->iperm[i];
         if (F) { F[3 * i] = f[u].x; F[3 * i + 1] = f[u].y; F[3 * i + 2] = f[u].z; }
@@ -1008,10 +1235,10 @@ int32_t dem_debug_cells(dem_scene_t* s, int32_t* kid, uint32_t* hid, double* kce
     std::vector<int32_t> k3(3 * n);
     std::vector<uint32_t> h1(n), n8(8 * n);
     std::vector<double> c3(3 * n);
-    if (e == hipSuccess) e = hipMemcpy(k3.data(), dk, 3 * n * sizeof(int32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(h1.data(), dh, n * sizeof(uint32_t), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(c3.data(), dc, 3 * n * sizeof(double), hipMemcpyDeviceToHost);
-    if (e == hipSuccess) e = hipMemcpy(n8.data(), dn, 8 * n * sizeof(uint32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = to_host(s, k3.data(), dk, 3 * n * sizeof(int32_t));
+    if (e == hipSuccess) e = to_host(s, h1.data(), dh, n * sizeof(uint32_t));
+    if (e == hipSuccess) e = to_host(s, c3.data(), dc, 3 * n * sizeof(double));
+    if (e == hipSuccess) e = to_host(s, n8.data(), dn, 8 * n * sizeof(uint32_t));
     hipFree(dk); hipFree(dh); hipFree(dc); hipFree(dn);
     if (e == hipSuccess)
         for (int64_t i = 0; i < n; ++i) {  // back to file order
@@ -1035,8 +1262,8 @@ int32_t dem_debug_hash_table(dem_scene_t* s, uint32_t* counts, uint32_t* start, 
     HIPCHK(s, hipStreamSynchronize(s->stream));
     const int64_t H = s->der.hash_table_size, n = s->n;
     std::vector<uint32_t> bs(H + 1), si(n);
-    HIPCHK(s, hipMemcpy(bs.data(), s->d.bstart, (H + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    HIPCHK(s, hipMemcpy(si.data(), s->d.sidx, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    HIPCHK(s, to_host(s, bs.data(), s->d.bstart, (H + 1) * sizeof(uint32_t)));
+    HIPCHK(s, to_host(s, si.data(), s->d.sidx, n * sizeof(uint32_t)));
     for (int64_t b = 0; b < H; ++b) {
         if (counts) counts[b] = bs[b + 1] - bs[b];
         if (start) start[b] = bs[b];
@@ -1055,7 +1282,7 @@ int32_t dem_debug_cp_list(dem_scene_t* s, int32_t* pairs, int64_t cap, int64_t* 
     HIPCHK(s, run_debug_cp_list(s, &dp, &total));
     const int64_t w = std::min(cap, total);
     hipError_t e = hipStreamSynchronize(s->stream);
-    if (e == hipSuccess && w > 0) e = hipMemcpy(pairs, dp, 2 * w * sizeof(int32_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess && w > 0) e = to_host(s, pairs, dp, 2 * w * sizeof(int32_t));
     hipFree(dp);
     if (e != hipSuccess) return hip_fail(s, "dem_debug_cp_list", e);
     *count = total;

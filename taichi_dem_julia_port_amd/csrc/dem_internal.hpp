@@ -171,6 +171,10 @@ struct DevState {
     SceneP* seg = nullptr;
     uint16_t* sid = nullptr;
     unsigned long long* seg_broken = nullptr;
+    // every array above lives in one of two allocations: the contact rows (touch, pq, st), which
+    // change size when rows are widened, and everything else
+    void* arena = nullptr;
+    void* rows_arena = nullptr;
 };
 
 }  // namespace dem
@@ -195,6 +199,8 @@ struct dem_scene {
     std::vector<uint32_t> perm, iperm;
     std::vector<int64_t> seg_off;   // [nseg + 1] first particle of each scene
     std::vector<dem::SceneP> seg_host;
+    std::vector<dem_config_t> seg_cfg;   // [nseg] each scene's configuration
+    std::vector<dem_derived_t> seg_der;  // [nseg] and its derived quantities
     int64_t steps = 0;
     uint64_t broken_total_host = 0;
     dem_config_t cfg{};
@@ -242,8 +248,12 @@ hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint3
                               const double* state_dev, int64_t m);
 void timing_begin(dem_scene* s, int kernel);
 void timing_end(dem_scene* s);
-int32_t timing_flush(dem_scene* s);
+int32_t timing_flush(dem_scene* s);  // abi.hip: collect elapsed times of recorded pairs
 hipError_t run_copy(const void* src, void* dst, int64_t nbytes, hipStream_t stream);
 hipError_t run_adopt(dem_scene* dst, int64_t dst_off, const dem_scene* src, int64_t src_off,
-                     int64_t n, int32_t nwall, uint32_t* overflow_dev);  // abi.hip: collect elapsed times of recorded pairs
+                     int64_t n, int32_t nwall, uint32_t* overflow_dev);
+// initial state of the particles of file index >= gbase from device copies of their input rows,
+// radii and inertias; r0, sort order and scene ids of all particles
+hipError_t run_init_state(dem_scene* s, const dem_grain_in_t* rows, const double* r,
+                          const double* I, const double* r0f, const int64_t* seg_off, int64_t gbase);
 }  // namespace dem

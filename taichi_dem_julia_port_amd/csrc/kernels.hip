@@ -1496,6 +1496,50 @@ __global__ void __launch_bounds__(TPB) k_adopt(const AdoptArgs a) {
     a.dsel[pd] = 0;
 }
 
+// Initial particle state (executor.jl:229-242: q = 1, w = a = dw = 0) from the file-order input
+// rows: internal index u holds file index eid[u].  The rows, radii and inertias cover file indices
+// gbase..n-1; particles below gbase already hold state (moved in by k_adopt) and get only their
+// r0, their starting sort order and their scene id.
+struct InitArgs {
+    int64_t n, gbase;
+    int32_t nseg;
+    const dem_grain_in_t* rows;
+    const double *r, *I;   // [n - gbase]
+    const double* r0f;     // [n], file order
+    const int64_t* seg_off;  // [nseg + 1] (batches), else null
+    const uint32_t* eid;
+    double4 *pos, *vel, *omg, *quat;
+    double* r0;
+    uint32_t *sidx0, *sidx1;
+    uint16_t* sid;
+};
+
+__global__ void __launch_bounds__(TPB) k_init_state(const InitArgs a) {
+    const int64_t u = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (u >= a.n) return;
+    const int64_t i = a.eid[u];
+    a.r0[u] = a.r0f[i];
+    a.sidx0[u] = (uint32_t)u;  // the first sort starts from the internal (spatial) order
+    a.sidx1[u] = (uint32_t)u;
+    if (a.seg_off) {
+        // the last scene whose first particle is at or before u (scenes are never empty)
+        int lo = 0, hi = a.nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (a.seg_off[mid] <= u) lo = mid;
+            else hi = mid - 1;
+        }
+        a.sid[u] = (uint16_t)lo;
+    }
+    if (i < a.gbase) return;
+    const int64_t k = i - a.gbase;
+    const dem_grain_in_t& g = a.rows[k];
+    a.pos[u] = make_double4(g.k[0], g.k[1], g.k[2], a.r[k]);
+    a.vel[u] = make_double4(g.v[0], g.v[1], g.v[2], g.m);
+    a.omg[u] = make_double4(0.0, 0.0, 0.0, a.I[k]);
+    a.quat[u] = make_double4(1.0, 0.0, 0.0, 0.0);
+}
+
 inline dim3 grid_for(int64_t n, int tpb) { return dim3((uint32_t)((n + tpb - 1) / tpb)); }
 
 }  // namespace
@@ -1760,6 +1804,21 @@ hipError_t run_adopt(dem_scene* dst, int64_t dst_off, const dem_scene* src, int6
     a.sst0 = S.st[0]; a.sst1 = S.st[1]; a.dst0 = D.st[0];
     a.overflow = overflow_dev;
     hipLaunchKernelGGL(k_adopt, grid_for(n, TPB), dim3(TPB), 0, dst->stream, a);
+    return hipGetLastError();
+}
+
+hipError_t run_init_state(dem_scene* s, const dem_grain_in_t* rows, const double* r,
+                          const double* I, const double* r0f, const int64_t* seg_off, int64_t gbase) {
+    DevState& d = s->d;
+    InitArgs a;
+    a.n = s->n; a.gbase = gbase; a.nseg = s->nseg;
+    a.rows = rows; a.r = r; a.I = I; a.r0f = r0f;
+    a.seg_off = s->batched ? seg_off : nullptr;
+    a.eid = d.eid;
+    a.pos = d.pos[s->cur]; a.vel = d.vel[s->cur]; a.omg = d.omg[s->cur]; a.quat = d.quat[s->cur];
+    a.r0 = d.r0; a.sidx0 = d.sidxb[0]; a.sidx1 = d.sidxb[1];
+    a.sid = s->batched ? d.sid : nullptr;
+    hipLaunchKernelGGL(k_init_state, grid_for(s->n, TPB), dim3(TPB), 0, s->stream, a);
     return hipGetLastError();
 }
 

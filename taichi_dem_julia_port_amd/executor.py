@@ -205,6 +205,53 @@ class Batch(Scene):
                  graph: bool = True):
         self._lib = N.load_library()
         members = list(members)
+        cfgs, nwalls, walls, counts, allw, allg = self._pack(members)
+        opt = self._options(precision, contact_slots, device, graph)
+        h = ctypes.c_void_p()
+        rc = self._lib.dem_create_batch(len(members), cfgs, _ptr(nwalls, ctypes.c_int32),
+                                        _ptr(allw, N.DemWall), _ptr(counts, ctypes.c_int64),
+                                        _ptr(allg, N.DemGrainIn), ctypes.byref(opt), ctypes.byref(h))
+        if rc != 0:
+            raise DEMError(rc, self._lib.dem_last_error(None).decode())
+        self._attach(h, [m[0] for m in members], walls, precision)
+
+    @classmethod
+    def repack(cls, src: "Scene", keep, members=(), precision: Optional[str] = None,
+               contact_slots: int = 0, device: int = 0, graph: bool = True) -> "Batch":
+        """dem_batch_repack: a new batch of scenes ``keep`` of ``src`` (a Batch or a Scene), their
+        state moved over on the device, followed by the new ``members`` (cfg, grains, walls).
+        ``src`` is left unchanged; the caller closes it."""
+        self = cls.__new__(cls)
+        self._lib = N.load_library()
+        keep = np.ascontiguousarray(np.asarray(keep, dtype=np.int32).reshape(-1))
+        members = list(members)
+        precision = precision or src.precision
+        opt = self._options(precision, contact_slots, device, graph)
+        h = ctypes.c_void_p()
+        if members:
+            cfgs, nwalls, walls, counts, allw, allg = self._pack(members)
+            args = (cfgs, _ptr(nwalls, ctypes.c_int32), _ptr(allw, N.DemWall),
+                    _ptr(counts, ctypes.c_int64), _ptr(allg, N.DemGrainIn))
+        else:
+            walls, args = [], (None, None, None, None, None)
+        rc = self._lib.dem_batch_repack(src._h, len(keep), _ptr(keep, ctypes.c_int32),
+                                        len(members), *args, ctypes.byref(opt), ctypes.byref(h))
+        if rc != 0:
+            raise DEMError(rc, self._lib.dem_last_error(None).decode())
+        src_cfgs = getattr(src, "scene_cfgs", [src.cfg])
+        src_walls = getattr(src, "scene_walls", [src.walls])
+        self._attach(h, [src_cfgs[k] for k in keep] + [m[0] for m in members],
+                     [src_walls[k] for k in keep] + walls, precision)
+        return self
+
+    @staticmethod
+    def _options(precision, contact_slots, device, graph):
+        flags = 0 if graph else N.DEM_FLAG_NO_GRAPH
+        return N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), flags)
+
+    @staticmethod
+    def _pack(members):
+        """The concatenated C arrays of dem_create_batch for scenes (cfg, grains, walls)."""
         S = len(members)
         cfgs = (N.DemConfig * S)()
         nwalls = np.zeros(S, dtype=np.int32)
@@ -223,24 +270,22 @@ class Batch(Scene):
         allw = np.ascontiguousarray(np.concatenate(walls)) if sum(map(len, walls)) else \
             np.zeros(1, dtype=WALL_DTYPE)
         allg = np.ascontiguousarray(np.concatenate(grains))
-        flags = 0 if graph else N.DEM_FLAG_NO_GRAPH
-        opt = N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), flags)
-        h = ctypes.c_void_p()
-        rc = self._lib.dem_create_batch(S, cfgs, _ptr(nwalls, ctypes.c_int32), _ptr(allw, N.DemWall),
-                                        _ptr(counts, ctypes.c_int64), _ptr(allg, N.DemGrainIn),
-                                        ctypes.byref(opt), ctypes.byref(h))
-        if rc != 0:
-            raise DEMError(rc, self._lib.dem_last_error(None).decode())
+        return cfgs, nwalls, walls, counts, allw, allg
+
+    def _attach(self, h, cfgs, walls, precision) -> None:
         self._h = h
-        self.cfg = members[0][0]
-        self.n = int(counts.sum())
-        self.nscenes = S
-        self.nwall = int(nwalls.max()) if S else 0
-        self.walls = allw
+        self.scene_cfgs = list(cfgs)
+        self.scene_walls = list(walls)
+        self.cfg = self.scene_cfgs[0]
+        self.nscenes = len(self.scene_cfgs)
+        self.nwall = max(len(w) for w in self.scene_walls)
+        self.walls = np.ascontiguousarray(np.concatenate(self.scene_walls)) \
+            if sum(map(len, self.scene_walls)) else np.zeros(0, dtype=WALL_DTYPE)
         self.precision = precision
-        off = np.zeros(S + 1, dtype=np.int64)
+        off = np.zeros(self.nscenes + 1, dtype=np.int64)
         self._check(self._lib.dem_batch_offsets(self._h, _ptr(off, ctypes.c_int64)))
         self.offsets = off
+        self.n = int(off[-1])
 
     def set_bond_init(self, mask) -> None:
         m = np.ascontiguousarray(np.asarray(mask, dtype=np.uint8))

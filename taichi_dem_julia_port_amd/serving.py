@@ -3,8 +3,8 @@
 Requests (a scene and a number of steps) arrive over time.  At every scheduling epoch the server
 admits the requests that have arrived -- first come, first served, as long as the estimated HBM
 footprint of all live scenes stays within the budget -- and re-packs the live set into one batch
-(dem_create_batch + dem_batch_adopt for the scenes already running, dem_initialize for the newly
-admitted bonded ones).  It then advances every live scene together, one segmented launch per
+(dem_batch_repack: the scenes already running move over device to device, only the arrivals are
+uploaded; dem_initialize for the newly admitted bonded ones).  It then advances every live scene together, one segmented launch per
 stage and one graph replay per two steps, by the smaller of the epoch length and the fewest steps
 any live scene still needs, so no scene runs past its request.  Finished scenes are read back and
 retired at the next epoch.  State stays on the device between epochs.
@@ -135,6 +135,9 @@ class Server:
         self.repacks = 0
         self.repack_s = 0.0  # wall time spent re-packing (batch creation, state transfer, bond pass)
         self.step_s = 0.0    # wall time spent stepping
+        # one record per epoch: start time, live scenes and grains, steps, and the seconds spent
+        # packing the batch, initialising bonds, releasing the previous batch and stepping
+        self.epochs: List[dict] = []
 
     def _slots(self, live: List[_Live]) -> int:
         return max(min(128, max(32, 2 * l.req.build()[0].max_coordinate_number)) for l in live)
@@ -173,19 +176,24 @@ class Server:
                     batch = None
                 time.sleep(max(pending[0].arrival - now, 0.0))
                 continue
+            rec = {"t": round(now, 4), "new": len(new)}
             if new or dirty or batch is None:
                 tr = time.perf_counter()
                 members = live + new
-                fresh = {id(m) for m in new}
-                nb = Batch([m.req.build() for m in members], precision=self.precision,
-                           device=self.device)
-                nb.set_bond_init([1 if (id(m) in fresh and m.req.bonded) else 0 for m in members])
+                fresh = [m.req.build() for m in new]
+                if batch is None or not live:
+                    nb = Batch(fresh, precision=self.precision, device=self.device)
+                else:
+                    # the live scenes move over on the device; only arrivals are uploaded
+                    nb = Batch.repack(batch, [m.slot for m in live], fresh,
+                                      precision=self.precision, device=self.device)
+                nb.set_bond_init([0] * len(live) + [1 if m.req.bonded else 0 for m in new])
                 for k, m in enumerate(members):
-                    if m.slot >= 0:
-                        nb._check(nb._lib.dem_batch_adopt(nb._h, k, batch._h, m.slot))
                     m.slot = k
+                ti = time.perf_counter()
                 if any(m.req.bonded for m in new):
                     nb.initialize()
+                te = time.perf_counter()
                 if batch is not None:
                     batch.close()
                 batch = nb
@@ -193,10 +201,16 @@ class Server:
                 dirty = False
                 self.repacks += 1
                 self.repack_s += time.perf_counter() - tr
+                rec.update(pack_s=round(ti - tr, 4), init_s=round(te - ti, 4),
+                           close_s=round(time.perf_counter() - te, 4),
+                           slots=int(batch.derived()["contact_slots"]))
             k = min(self.epoch, min(m.remaining for m in live))
             ts = time.perf_counter()
             batch.step(k)
             self.step_s += time.perf_counter() - ts
+            rec.update(scenes=len(live), grains=int(batch.n), steps=int(k),
+                       step_s=round(time.perf_counter() - ts, 4))
+            self.epochs.append(rec)
             for m in live:
                 m.remaining -= k
             finished = [m for m in live if m.remaining == 0]

@@ -85,3 +85,102 @@ def test_batch_against_oracle_one_step():
         go = o.read_grains()
         err = np.abs(gb[k]["k"] - go["k"]).max() / max(np.abs(go["k"]).max(), 1e-30)
         assert err < 1e-12, (k, err)
+
+
+def _state(b):
+    g = b.split_grains(b.read_grains())
+    c = b.split_contacts(*b.read_contacts())
+    return g, c, b.bonds_broken()
+
+
+def _assert_same(x, y, label):
+    gx, cx, bx = x
+    gy, cy, by = y
+    assert len(gx) == len(gy), label
+    for k in range(len(gx)):
+        for f in ("id", "gid", "V", "m", "r", "r0", "I", "k", "v", "w", "q", "a", "dw"):
+            assert np.array_equal(np.ascontiguousarray(gx[k][f]).view(np.uint8),
+                                  np.ascontiguousarray(gy[k][f]).view(np.uint8)), (label, k, f)
+        (c1, b1), (c2, b2) = cx[k], cy[k]
+        assert np.array_equal(pair_keys(c1), pair_keys(c2)), (label, k)
+        assert np.array_equal(b1, b2), (label, k)
+        for f in ("F", "s", "ti", "tj", "k"):
+            assert np.array_equal(_bits(c1[f]), _bits(c2[f])), (label, k, f)
+    assert np.array_equal(bx, by), label
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_repack_equals_create_and_adopt(precision):
+    """dem_batch_repack (kept scenes moved device to device, new ones uploaded) gives the batch
+    that dem_create_batch + dem_batch_adopt gives, bit for bit, before and after more steps."""
+    Batch, _ = _engine()
+    old = _members(4)            # 4 cubes, a packed sample, a reference case
+    extra = scenes.batched_cubes(2, seed=31, nmin=300, nmax=1500, base_gap=2e-5)
+    print(f"[case] repack {precision}: {len(old)} scenes, keep 3, add 2", flush=True)
+    with Batch(old, precision=precision) as src:
+        src.set_bond_init([1, 1, 1, 1, 0, 1])
+        src.initialize()
+        src.step(23)
+        keep = [5, 1, 3]
+        with Batch.repack(src, keep, extra, precision=precision) as a, \
+                Batch([old[k] for k in keep] + extra, precision=precision) as b:
+            for j, k in enumerate(keep):
+                b._check(b._lib.dem_batch_adopt(b._h, j, src._h, k))
+            mask = [0, 0, 0, 1, 1]
+            a.set_bond_init(mask)
+            b.set_bond_init(mask)
+            assert np.array_equal(a.offsets, b.offsets)
+            _assert_same(_state(a), _state(b), "after repack")
+            a.initialize()
+            b.initialize()
+            a.step(31)
+            b.step(31)
+            _assert_same(_state(a), _state(b), "after stepping")
+        # src is untouched by the repack
+        src.step(3)
+
+
+def test_repack_keeps_only_and_from_single_scene():
+    """No new scenes; and a single scene (not a batch) as the source."""
+    Batch, Scene = _engine()
+    m = _members(3)
+    print("[case] repack without arrivals; repack of a single scene", flush=True)
+    with Batch(m) as src:
+        src.set_bond_init([1, 1, 1, 0, 1])
+        src.initialize()
+        src.step(11)
+        before = _state(src)
+        with Batch.repack(src, [4, 0]) as a:
+            g, c, br = _state(a)
+            _assert_same((g, c, br), ([before[0][4], before[0][0]], [before[1][4], before[1][0]],
+                                      before[2][[4, 0]]), "kept only")
+            a.step(5)
+            src.step(5)
+            after = _state(src)
+            _assert_same(_state(a), ([after[0][4], after[0][0]], [after[1][4], after[1][0]],
+                                     after[2][[4, 0]]), "kept only, stepped")
+    cfg, g, w = m[0]
+    with Scene(cfg, g, w) as s:
+        s.initialize()
+        s.step(9)
+        with Batch.repack(s, [0], [m[1]]) as a, Batch([m[0], m[1]]) as b:
+            b._check(b._lib.dem_batch_adopt(b._h, 0, s._h, 0))
+            a.set_bond_init([0, 1])
+            b.set_bond_init([0, 1])
+            a.initialize()
+            b.initialize()
+            a.step(7)
+            b.step(7)
+            _assert_same(_state(a), _state(b), "single-scene source")
+
+
+def test_repack_rejects_bad_arguments():
+    Batch, _ = _engine()
+    from taichi_dem_julia_port_amd.executor import DEMError
+    with Batch(_members(2)) as src:
+        for keep in ([0, 0], [7], [-1]):
+            with pytest.raises(DEMError) as ei:
+                Batch.repack(src, keep)
+            assert ei.value.code == -1, keep
+        with pytest.raises(DEMError):
+            Batch.repack(src, [])   # no scenes at all
