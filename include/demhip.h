@@ -221,6 +221,12 @@ const char* dem_kernel_name(int32_t k);
 /* Plain device-to-device copy kernel over nbytes (rounded down to 32-byte units), run repeats
  * times after one warm-up; reports the mean duration.  Used to calibrate HBM byte counters. */
 int32_t dem_debug_copy(int64_t nbytes, int32_t repeats, double* ms_per_copy);
+/* Device memory of destroyed scenes kept for reuse by later ones, up to max_bytes in total
+ * (0, the default: memory goes back to the runtime at once; lowering the limit frees the
+ * excess).  No counterpart in the reference: for callers that create and destroy batches of
+ * similar sizes in a loop (the serving re-pack), where releasing and mapping tens of gigabytes
+ * costs up to hundreds of milliseconds per batch. */
+int32_t dem_set_memory_cache(int64_t max_bytes);
 
 /* ---- segmented batches (no counterpart in the reference) -----------------------------------
  * nscenes independent scenes share one set of device arrays and are advanced together by one

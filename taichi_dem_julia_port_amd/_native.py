@@ -163,6 +163,7 @@ SIGNATURES = [
     ("dem_set_timing", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("dem_get_kernel_times", ctypes.c_int32, [ctypes.c_void_p, P(DemKernelTimes)]),
     ("dem_kernel_name", ctypes.c_char_p, [ctypes.c_int32]),
+    ("dem_set_memory_cache", ctypes.c_int32, [ctypes.c_int64]),
     ("dem_debug_copy", ctypes.c_int32, [ctypes.c_int64, ctypes.c_int32, P(ctypes.c_double)]),
     ("dem_create_batch", ctypes.c_int32, [ctypes.c_int32, P(DemConfig), P(ctypes.c_int32), P(DemWall),
                                           P(ctypes.c_int64), P(DemGrainIn), P(DemOptions),

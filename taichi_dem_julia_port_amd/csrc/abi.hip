@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -56,6 +57,79 @@ struct Carve {
     }
 };
 
+// Scene allocations released by dem_destroy, kept for reuse when a cache is enabled
+// (dem_set_memory_cache): a serving loop releases and creates batches of similar sizes at every
+// change of its live set, and handing tens of gigabytes back to the driver and mapping them again
+// costs up to hundreds of milliseconds per batch.  A request takes the smallest cached block of
+// its device that holds it and wastes at most half of it.
+struct CachedBlock {
+    int device;
+    void* ptr;
+    size_t bytes;
+};
+std::mutex g_cache_mu;
+std::vector<CachedBlock> g_cache;  // oldest first
+size_t g_cache_cap = 0, g_cache_bytes = 0;
+
+// Frees cached blocks, oldest first, until at most keep bytes remain.  Needs g_cache_mu.
+void cache_trim(size_t keep) {
+    while (g_cache_bytes > keep && !g_cache.empty()) {
+        const CachedBlock b = g_cache.front();
+        g_cache.erase(g_cache.begin());
+        g_cache_bytes -= b.bytes;
+        int cur = 0;
+        hipGetDevice(&cur);
+        hipSetDevice(b.device);
+        hipFree(b.ptr);
+        hipSetDevice(cur);
+    }
+}
+
+// A device allocation of at least bytes on the current device (*got: its size).
+hipError_t block_alloc(int device, void** p, size_t bytes, size_t* got) {
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        int best = -1;
+        for (int i = 0; i < (int)g_cache.size(); ++i) {
+            const CachedBlock& b = g_cache[i];
+            if (b.device == device && b.bytes >= bytes && b.bytes <= 2 * bytes &&
+                (best < 0 || b.bytes < g_cache[best].bytes))
+                best = i;
+        }
+        if (best >= 0) {
+            *p = g_cache[best].ptr;
+            *got = g_cache[best].bytes;
+            g_cache_bytes -= *got;
+            g_cache.erase(g_cache.begin() + best);
+            return hipSuccess;
+        }
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {  // cached blocks may be what fills the device
+        (void)hipGetLastError();
+        {
+            std::lock_guard<std::mutex> lk(g_cache_mu);
+            cache_trim(0);
+        }
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) *p = nullptr;
+    *got = bytes;
+    return e;
+}
+
+void block_free(int device, void* p, size_t bytes) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    if (bytes > g_cache_cap) {
+        hipFree(p);
+        return;
+    }
+    g_cache.push_back({device, p, bytes});
+    g_cache_bytes += bytes;
+    cache_trim(g_cache_cap);
+}
+
 // The contact rows of K slots per particle, zeroed.
 hipError_t alloc_rows(dem_scene* s, int32_t K) {
     DevState& d = s->d;
@@ -67,8 +141,8 @@ hipError_t alloc_rows(dem_scene* s, int32_t K) {
         return c.off;
     };
     const size_t bytes = layout(nullptr);
-    hipError_t e = hipMalloc(&d.rows_arena, bytes);
-    if (e != hipSuccess) { d.rows_arena = nullptr; return e; }
+    hipError_t e = block_alloc(s->device, &d.rows_arena, bytes, &d.rows_arena_bytes);
+    if (e != hipSuccess) return e;
     layout(d.rows_arena);
     return hipMemset(d.rows_arena, 0, bytes);
 }
@@ -90,8 +164,8 @@ double grain_radius(double V) { return cbrt(V / (4.0 / 3.0 * M_PI)); }
 void free_scene(dem_scene* s) {
     if (!s) return;
     DevState& d = s->d;
-    if (d.arena) hipFree(d.arena);
-    if (d.rows_arena) hipFree(d.rows_arena);
+    block_free(s->device, d.arena, d.arena_bytes);
+    block_free(s->device, d.rows_arena, d.rows_arena_bytes);
     if (s->h_ctr) hipHostFree(s->h_ctr);
     if (s->h_ctr_raw) hipHostFree(s->h_ctr_raw);
     for (hipEvent_t e : s->ev)
@@ -139,7 +213,7 @@ int32_t collect_timing(dem_scene* s) { return s->timing ? timing_flush(s) : DEM_
 int32_t resize_rows(dem_scene* s, int32_t k) {
     DevState& d = s->d;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    if (d.rows_arena) HIPCHK(s, hipFree(d.rows_arena));
+    block_free(s->device, d.rows_arena, d.rows_arena_bytes);
     d.rows_arena = nullptr;
     d.touch = nullptr; d.pq[0] = d.pq[1] = nullptr; d.st[0] = d.st[1] = nullptr;
     HIPCHK(s, alloc_rows(s, k));
@@ -147,6 +221,8 @@ int32_t resize_rows(dem_scene* s, int32_t k) {
     HIPCHK(s, hipMemset(d.sel, 0, s->n));
     HIPCHK(s, hipMemset(d.tcnt, 0, s->n * sizeof(uint32_t)));
     HIPCHK(s, hipMemset(d.ctr, 0, CTR_WORDS * sizeof(unsigned long long)));
+    // the fills run on the null stream, which the scene's non-blocking stream does not wait for
+    HIPCHK(s, hipDeviceSynchronize());
     s->K = k;
     s->P.K = k;
     s->der.contact_slots = k;
@@ -401,11 +477,9 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase,
         return c.off;
     };
     const size_t bytes = layout(nullptr);
-    e = hipMalloc(&d.arena, bytes);
-    if (e != hipSuccess) {
-        d.arena = nullptr;
+    e = block_alloc(s->device, &d.arena, bytes, &d.arena_bytes);
+    if (e != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, std::string("device allocation failed: ") + hipGetErrorString(e));
-    }
     layout(d.arena);
     d.sidx = d.sidxb[0];
     d.scan.max_tiles = max_tiles;
@@ -1310,6 +1384,14 @@ int32_t dem_debug_scan(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* t
     hipFree(ws.status); hipFree(ws.counters); hipFree(din); hipFree(dout); hipFree(dtot);
     if (e != hipSuccess) { g_create_err = std::string("dem_debug_scan: ") + hipGetErrorString(e); return DEM_ERR_HIP; }
     if (total) *total = t;
+    return DEM_OK;
+}
+
+int32_t dem_set_memory_cache(int64_t max_bytes) {
+    if (max_bytes < 0) return DEM_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    g_cache_cap = (size_t)max_bytes;
+    cache_trim(g_cache_cap);
     return DEM_OK;
 }
 

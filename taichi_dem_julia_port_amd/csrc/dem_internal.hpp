@@ -175,6 +175,7 @@ struct DevState {
     // change size when rows are widened, and everything else
     void* arena = nullptr;
     void* rows_arena = nullptr;
+    size_t arena_bytes = 0, rows_arena_bytes = 0;
 };
 
 }  // namespace dem

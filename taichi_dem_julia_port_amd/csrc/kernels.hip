@@ -38,6 +38,24 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
     return x < r ? x * (q + 1) + local : r * (q + 1) + (x - r) * q + local;
 }
 
+// The same for batches, whose scenes differ in cost per particle: XCD x runs chunks of
+// XCD_CHUNK contiguous logical blocks, every 8th chunk, so each XCD gets a share of every part of
+// the batch.  Blocks past the last whole round of 8 chunks keep their own number.
+constexpr uint32_t XCD_CHUNK = 32;
+__device__ __forceinline__ uint32_t xcd_block_chunked(uint32_t b, uint32_t nb) {
+    constexpr uint32_t NXCD = 8, ROUND = NXCD * XCD_CHUNK;
+    const uint32_t nfull = nb / ROUND * ROUND;
+    if (b >= nfull) return b;
+    const uint32_t x = b % NXCD, local = b / NXCD;
+    return local / XCD_CHUNK * ROUND + x * XCD_CHUNK + local % XCD_CHUNK;
+}
+
+template <bool SEG>
+__device__ __forceinline__ uint32_t xcd_map(uint32_t b, uint32_t nb) {
+    if constexpr (SEG) return xcd_block_chunked(b, nb);
+    else return xcd_block(b, nb);
+}
+
 __device__ __forceinline__ double jsign(double x) { return x > 0 ? 1.0 : (x < 0 ? -1.0 : x); }
 
 // ------------------------------------------------------------------------------------------
@@ -381,7 +399,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (lane == 0) hq_n[wv] = 0u;
     const int64_t n = P.n;
-    const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TPB;
+    const int64_t t0 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * TPB;
     const int64_t t = t0 + threadIdx.x;
     const int64_t tl = (t0 + TPB < n ? t0 + TPB : n) - 1;
     const bool live = t < n;
@@ -774,7 +792,7 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
         unsigned long long* ctr, int diag) {
     __shared__ uint32_t lt[MERGE_LDS * MERGE_TPB];  // touching partners
     __shared__ uint32_t lw[MERGE_LDS * MERGE_TPB];  // old-row words
-    const int64_t p64 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * MERGE_TPB + threadIdx.x;
+    const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * MERGE_TPB + threadIdx.x;
     const int64_t n = P.n;
     const int K = P.K;
     uint64_t created = 0, deact = 0, entries = 0;
@@ -967,7 +985,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         double* __restrict__ st1, uint32_t* __restrict__ wact, double4* __restrict__ wshear,
         unsigned long long* ctr, int write_forces) {
     // XCD-contiguous blocks: partners are spatial neighbours, so their state is in this XCD's L2
-    const int64_t p64 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
+    const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
     const int64_t n = P.n;
     uint64_t broken = 0, nwallc = 0, fatal = 0, active = 0, active_b = 0;
     if (p64 < n) {

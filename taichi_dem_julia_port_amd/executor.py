@@ -312,6 +312,15 @@ class Batch(Scene):
         return out
 
 
+def set_memory_cache(max_bytes: int) -> None:
+    """dem_set_memory_cache: keep up to max_bytes of destroyed scenes' device memory for reuse
+    by later scenes (0 frees it and turns the cache off)."""
+    lib = N.load_library()
+    rc = lib.dem_set_memory_cache(int(max_bytes))
+    if rc != 0:
+        raise DEMError(rc, "dem_set_memory_cache: max_bytes must be >= 0")
+
+
 def device_scan(values: np.ndarray):
     """Exclusive prefix sum with the engine's decoupled-lookback scan."""
     lib = N.load_library()

@@ -126,9 +126,12 @@ def summarize(results: List[Result], makespan: float) -> dict:
 
 class Server:
     def __init__(self, precision: str = "mixed", hbm_budget_gb: float = 200.0,
-                 epoch_steps: int = 1000, device: int = 0, keep_results: bool = False):
+                 epoch_steps: int = 1000, device: int = 0, keep_results: bool = False,
+                 memory_cache_gb: float = 64.0):
         self.precision = precision
         self.budget = hbm_budget_gb * 1e9
+        # released batches' device memory kept for the next re-pack while the server runs
+        self.cache = int(memory_cache_gb * 1e9)
         self.epoch = epoch_steps
         self.device = device
         self.keep = keep_results
@@ -144,6 +147,14 @@ class Server:
 
     def run(self, requests: List[Request]) -> tuple:
         """Serve the requests in real time (arrival times are seconds after the call)."""
+        from .executor import set_memory_cache
+        set_memory_cache(self.cache)
+        try:
+            return self._run(requests)
+        finally:
+            set_memory_cache(0)
+
+    def _run(self, requests: List[Request]) -> tuple:
         from .executor import Batch
         pending = sorted(requests, key=lambda r: r.arrival)
         for r in pending:

@@ -184,3 +184,31 @@ def test_repack_rejects_bad_arguments():
             assert ei.value.code == -1, keep
         with pytest.raises(DEMError):
             Batch.repack(src, [])   # no scenes at all
+
+
+def test_memory_cache_reuse_gives_same_results():
+    """With dem_set_memory_cache on, a batch built in a destroyed batch's memory (stale contents,
+    cleared on reuse) steps exactly as one built in fresh memory."""
+    Batch, _ = _engine()
+    from taichi_dem_julia_port_amd.executor import set_memory_cache
+    m = _members(3)
+    small = m[:2] + m[3:4]
+
+    def run(members):
+        with Batch(members) as b:
+            b.set_bond_init([1] * (len(members) - 1) + [0])
+            b.initialize()
+            b.step(17)
+            return _state(b)
+
+    print("[case] memory cache off / on", flush=True)
+    fresh = run(small)
+    set_memory_cache(8 << 30)
+    try:
+        run(m)                # leaves its (larger) blocks in the cache
+        reused = run(small)   # built in them
+        again = run(small)
+    finally:
+        set_memory_cache(0)
+    _assert_same(reused, fresh, "cached blocks")
+    _assert_same(again, fresh, "cached blocks, second reuse")
