@@ -1,4 +1,4 @@
-"""Compile libdemhip.so (gfx950) in-tree with hipcc."""
+"""Compile libdemhip.so (gfx950) in-tree with hipcc, and the native demhip_solve driver over it."""
 import os
 import subprocess
 
@@ -36,5 +36,25 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+CLI = os.path.join(HERE, "demhip_solve")
+CLI_SOURCE = os.path.join(HERE, "host", "demhip_solve.cpp")
+
+
+def build_cli(verbose: bool = False) -> str:
+    """The native solve() driver, linked against the in-tree libdemhip.so (found next to it)."""
+    cxx = os.environ.get("CXX", "g++")
+    tmp = CLI + ".tmp"
+    cmd = [cxx, "-O2", "-std=c++17", "-Wall", "-Wextra", "-o", tmp, CLI_SOURCE, "-L" + HERE,
+           "-ldemhip", "-lz", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError("building demhip_solve failed:\n" + res.stdout + res.stderr)
+    os.replace(tmp, CLI)
+    return CLI
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_cli(verbose=True))
