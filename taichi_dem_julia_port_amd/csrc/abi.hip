@@ -190,6 +190,12 @@ int32_t sync_counters(dem_scene* s) {
     return DEM_OK;
 }
 
+// The force kernel's block size for the next steps, from the row entries counted on the last
+// counted step (sync_counters must have run).
+void choose_force_blocks(dem_scene* s) {
+    s->force_tpb = 4 * s->h_ctr[CT_ENTRIES] >= (unsigned long long)s->n ? 64 : 256;
+}
+
 }  // namespace
 
 int32_t dem::timing_flush(dem_scene* s) {
@@ -999,6 +1005,7 @@ int32_t dem_initialize(dem_scene_t* s) {
         if ((rc = resize_rows(s, std::min(128, 2 * s->K)))) return rc;
     }
     s->initialized = true;
+    choose_force_blocks(s);
     return DEM_OK;
 }
 
@@ -1021,6 +1028,7 @@ static int32_t capture_graph(dem_scene* s) {
     s->graph = g;
     HIPCHK(s, hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
     s->graph_parity = start;
+    s->graph_force_tpb = s->force_tpb;
     return DEM_OK;
 }
 
@@ -1030,7 +1038,7 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
     int64_t t = 0;
     // body: pairs of steps replayed from a captured graph (not while timing individual kernels)
     if (s->use_graph && !s->timing && nsteps >= 3) {
-        if (!s->graph_exec || s->graph_parity != s->cur) {
+        if (!s->graph_exec || s->graph_parity != s->cur || s->graph_force_tpb != s->force_tpb) {
             int32_t rc = capture_graph(s);
             if (rc) return rc;
         }
@@ -1049,6 +1057,7 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
     if (s->h_ctr[CT_OVERFLOW])
         return fail(s, DEM_ERR_CAPACITY,
                     "contact slots per particle exceeded (raise contact_slots / max_coordinate_number)");
+    if (nsteps > 0) choose_force_blocks(s);
     return DEM_OK;
 }
 

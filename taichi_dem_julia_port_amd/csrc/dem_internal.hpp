@@ -225,6 +225,10 @@ struct dem_scene {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_parity = -1;   // value of cur the graph was captured from
+    // threads per block of the force kernel: 64 when the last counted step had at least one row
+    // entry per four particles, else 256; the graph holds the value it was captured with
+    int force_tpb = 256;
+    int graph_force_tpb = 0;
     bool use_graph = true;
     // timing: an event pair around every launch, collected when the pool fills or on request
     bool timing = false;
