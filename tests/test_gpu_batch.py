@@ -212,3 +212,19 @@ def test_memory_cache_reuse_gives_same_results():
         set_memory_cache(0)
     _assert_same(reused, fresh, "cached blocks")
     _assert_same(again, fresh, "cached blocks, second reuse")
+
+
+def test_repack_reports_rows_too_narrow_for_kept_scenes():
+    """Explicit contact_slots below what a kept scene's rows hold is a capacity error, not a
+    silent truncation; the source batch stays usable."""
+    Batch, _ = _engine()
+    from taichi_dem_julia_port_amd.executor import DEMError
+    m = _members(2)
+    with Batch(m) as src:
+        src.set_bond_init([1, 1, 0, 1])
+        src.initialize()
+        src.step(3)
+        with pytest.raises(DEMError) as ei:
+            Batch.repack(src, [0, 1], contact_slots=2)
+        assert ei.value.code == -3, ei.value
+        src.step(2)
