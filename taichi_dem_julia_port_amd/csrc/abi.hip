@@ -164,6 +164,8 @@ double grain_radius(double V) { return cbrt(V / (4.0 / 3.0 * M_PI)); }
 void free_scene(dem_scene* s) {
     if (!s) return;
     DevState& d = s->d;
+    // nothing queued may still use memory that goes back to the cache
+    if (s->stream) hipStreamSynchronize(s->stream);
     block_free(s->device, d.arena, d.arena_bytes);
     block_free(s->device, d.rows_arena, d.rows_arena_bytes);
     if (s->h_ctr) hipHostFree(s->h_ctr);
