@@ -105,3 +105,91 @@ def test_create_rejects_bad_input_without_gpu_work():
     rc = lib.dem_create(ctypes.byref(cfg), None, 0, g, 1, None, ctypes.byref(h))
     assert rc == N.DEM_ERR_INVALID
     assert b"dt" in lib.dem_last_error(None)
+
+
+def _valid_cfg():
+    from taichi_dem_julia_port_amd.config import DEMConfig
+    return DEMConfig(domain_min=[-1.0, -1.0, -1.0], domain_max=[1.0, 1.0, 1.0], dt=1e-6).to_c()
+
+
+def _grains(n=2):
+    g = (N.DemGrainIn * n)()
+    for i in range(n):
+        g[i].id, g[i].gid, g[i].V, g[i].m = i + 1, 1, 1e-6, 2.5e-3
+        g[i].k[:] = [0.1 * i, 0.0, 0.0]
+    return g
+
+
+@pytest.mark.parametrize("case, message", [
+    ("n0", b"n out of range"),
+    ("volume", b"volume and mass"),
+    ("nwall", b"nwall must be in [0, 16]"),
+    ("wall_mid", b"wall material id"),
+    ("precision", b"unknown precision"),
+    ("slots", b"contact_slots must be in [1, 128]"),
+    ("out_null", b"out is NULL"),
+])
+def test_create_validates_before_any_gpu_work(case, message):
+    """dem_create's checks all run before the device is touched, so they hold without a GPU."""
+    lib = N.load_library()
+    h = ctypes.c_void_p()
+    cfg = _valid_cfg()
+    g = _grains()
+    n, nwall = 2, 1
+    walls = (N.DemWall * 17)()
+    for w in walls:
+        w.n[:] = [0.0, 0.0, 1.0]
+        w.mid = 2
+    opt = N.DemOptions(N.DEM_PRECISION_FP64, 0, 0, 0)
+    out = ctypes.byref(h)
+    if case == "n0":
+        n = 0
+    elif case == "volume":
+        g[1].V = 0.0
+    elif case == "nwall":
+        nwall = 17
+    elif case == "wall_mid":
+        walls[0].mid = 3
+    elif case == "precision":
+        opt.precision = 7
+    elif case == "slots":
+        opt.contact_slots = 200
+    elif case == "out_null":
+        out = None
+    rc = lib.dem_create(ctypes.byref(cfg), walls, nwall, g, n, ctypes.byref(opt), out)
+    assert rc == N.DEM_ERR_INVALID
+    assert message in lib.dem_last_error(None), lib.dem_last_error(None)
+    assert not h.value
+
+
+def test_batch_entry_points_validate_before_any_gpu_work():
+    lib = N.load_library()
+    h = ctypes.c_void_p()
+    cfg = (N.DemConfig * 1)(_valid_cfg())
+    nw = (ctypes.c_int32 * 1)(0)
+    cnt = (ctypes.c_int64 * 1)(2)
+    g = _grains()
+    rc = lib.dem_create_batch(0, cfg, nw, None, cnt, g, None, ctypes.byref(h))
+    assert rc == N.DEM_ERR_INVALID and b"1..65535 scenes" in lib.dem_last_error(None)
+    opt = N.DemOptions(N.DEM_PRECISION_FP64, 0, 0, N.DEM_FLAG_REFERENCE_GRID)
+    rc = lib.dem_create_batch(1, cfg, nw, None, cnt, g, ctypes.byref(opt), ctypes.byref(h))
+    assert rc == N.DEM_ERR_INVALID and b"dense sort grid" in lib.dem_last_error(None)
+    rc = lib.dem_batch_repack(None, 0, None, 1, cfg, nw, None, cnt, g, None, ctypes.byref(h))
+    assert rc == N.DEM_ERR_INVALID and b"src is NULL" in lib.dem_last_error(None)
+    assert not h.value
+
+
+def test_null_handles_are_rejected():
+    lib = N.load_library()
+    cnt = ctypes.c_int64(0)
+    assert lib.dem_step(None, 1) == N.DEM_ERR_INVALID
+    assert lib.dem_initialize(None) == N.DEM_ERR_INVALID
+    assert lib.dem_get_derived(None, ctypes.byref(N.DemDerived())) == N.DEM_ERR_INVALID
+    assert lib.dem_get_counters(None, ctypes.byref(N.DemCounters())) == N.DEM_ERR_INVALID
+    assert lib.dem_read_grains(None, None, 0) == N.DEM_ERR_INVALID
+    assert lib.dem_read_contacts(None, None, None, 0, ctypes.byref(cnt)) == N.DEM_ERR_INVALID
+    assert lib.dem_write_contacts(None, None, None, 0) == N.DEM_ERR_INVALID
+    assert lib.dem_batch_offsets(None, None) == N.DEM_ERR_INVALID
+    assert lib.dem_batch_adopt(None, 0, None, 0) == N.DEM_ERR_INVALID
+    assert lib.dem_set_memory_cache(-1) == N.DEM_ERR_INVALID
+    lib.dem_destroy(None)  # a no-op
