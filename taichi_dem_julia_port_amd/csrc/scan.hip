@@ -201,8 +201,19 @@ k_scan_tiles(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
         const int64_t g = base + it * SCAN_THREADS + tid;
         v[it] = g < n ? in[g] : 0u;
     }
+    // the totals of the tiles before this one, eight independent loads per thread in flight (a
+    // large grid's last tiles sum thousands of them)
     uint32_t pre = 0;
-    for (uint32_t k = tid; k < tile * TAGG_COPIES; k += SCAN_THREADS) pre += tagg[k];
+    const uint32_t lim = tile * TAGG_COPIES;
+    uint32_t k = tid;
+    for (; k + 7 * SCAN_THREADS < lim; k += 8 * SCAN_THREADS) {
+        uint32_t a[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = tagg[k + u * SCAN_THREADS];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) pre += a[u];
+    }
+    for (; k < lim; k += SCAN_THREADS) pre += tagg[k];
 #pragma unroll
     for (int it = 0; it < SCAN_ITEMS; ++it) {
         const int64_t g = base + it * SCAN_THREADS + tid;
