@@ -39,3 +39,33 @@ def test_million_grain_steps_match_the_oracle(name):
         if name == "pour_packed":
             assert len(c) > 800_000
     o.close()
+
+
+def test_full_batch_scenes_equal_alone():
+    """The 256-scene ensemble (1.27M grains) at its benchmark shape: a sample of its scenes,
+    stepped in the batch, equals the same scenes stepped alone bit for bit."""
+    from taichi_dem_julia_port_amd.executor import Batch, Scene
+    members = scenes.batched_cubes(256, seed=4)
+    sizes = np.array([len(m[1]) for m in members])
+    sample = sorted({0, int(sizes.argmax()), int(sizes.argmin()), 137})
+    print(f"[case] batch of 256, n={sizes.sum()}, sample {sample}", flush=True)
+    with Batch(members) as b:
+        b.initialize()
+        b.step(30)
+        gb = b.split_grains(b.read_grains())
+        cb = b.split_contacts(*b.read_contacts())
+        broken = b.bonds_broken()
+    for k in sample:
+        cfg, g, w = members[k]
+        with Scene(cfg, g, w) as s:
+            s.initialize()
+            s.step(30)
+            ga = s.read_grains()
+            ca, ba = s.read_contacts()
+            assert broken[k] == s.counters()["bonds_broken_total"], k
+        for f in ("k", "v", "w", "q"):
+            assert np.array_equal(ga[f].view(np.uint64), gb[k][f].view(np.uint64)), (k, f)
+        c, bd = cb[k]
+        assert np.array_equal(pair_keys(c), pair_keys(ca)) and np.array_equal(bd, ba), k
+        for f in ("F", "ti", "tj", "s"):
+            assert np.array_equal(c[f].view(np.uint64), ca[f].view(np.uint64)), (k, f)
