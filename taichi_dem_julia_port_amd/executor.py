@@ -355,6 +355,7 @@ def solve(cfg_filename: str, no_bond: bool = False, save_snapshot: bool = False,
         scene.initialize()
     p4p = p4c = None
     if save_information:
+        os.makedirs(out_dir, exist_ok=True)
         p4p = open(os.path.join(out_dir, "output.p4p"), "w")
         p4c = open(os.path.join(out_dir, "output.p4c"), "w")
         write_p4p_frame(p4p, scene.read_grains(), 0.0)

@@ -24,6 +24,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <map>
 #include <set>
 #include <stdexcept>
@@ -394,6 +395,8 @@ int solve(const std::string& cfg_path, bool no_bond, bool save, int32_t precisio
         if (!no_bond) check(dem_initialize(h), h, "dem_initialize");
         const int64_t n = (int64_t)rows.size();
         if (save) {
+            std::error_code ec;
+            std::filesystem::create_directories(out_dir, ec);
             p4p = std::fopen((out_dir + "/output.p4p").c_str(), "w");
             p4c = std::fopen((out_dir + "/output.p4c").c_str(), "w");
             if (!p4p || !p4c) throw std::runtime_error("cannot write output files in " + out_dir);
