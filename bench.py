@@ -290,8 +290,8 @@ def main():
                     help="first run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs one scene per GPU on one GPU; multi-GPU "
-                                   "runs are not part of this build (use --gpus 1)"}))
+        print(json.dumps({"error": "this engine runs on one GPU; scenes are not split across GPUs "
+                                   "and multi-GPU runs are not supported (use --gpus 1)"}))
         sys.exit(2)
 
     if args.workload == "serve":

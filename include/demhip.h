@@ -10,7 +10,7 @@
  * over the device-resident GlobalData (src/executor.jl:1-29) that solve() builds
  * (src/executor.jl:174-339), plus the read-backs that save_single performs
  * (src/utils.jl:30-78).  The entry points below replace exactly that seam; a DEMPort host binds
- * them with ccall (the binding is listed in INTEGRATION.md).
+ * them with ccall.
  *
  * Conventions: C calling convention; host pointers only; every struct is POD with the layout of
  * the corresponding Julia isbits type where one exists; particle and contact indices are
