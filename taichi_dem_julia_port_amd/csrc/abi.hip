@@ -475,7 +475,7 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase,
         c.take(&d.wact, n); c.take(&d.wshear, (int64_t)(nwall_max > 0 ? nwall_max : 1) * n);
         c.take(&d.hid, n); c.take(&d.rank, n); c.take(&d.x32, n); c.take(&d.bcnt, nkeys + 1);
         c.take(&d.bstart, nkeys + 1); c.take(&d.sidxb[0], n); c.take(&d.sidxb[1], n); c.take(&d.skey, n);
-        c.take(&d.spos, n); c.take(&d.spos32, n); c.take(&d.eid, n);
+        c.take(&d.spos, n); c.take(&d.spos32, n); c.take(&d.eid, n); c.take(&d.sfid, n);
         c.take(&d.tcnt, n); c.take(&d.sel, n);
         c.take(&d.ctr, CTR_WORDS);
         c.take(&d.scan.status, max_tiles); c.take(&d.scan.counters, 2);

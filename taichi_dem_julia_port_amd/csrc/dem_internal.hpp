@@ -150,6 +150,7 @@ struct DevState {
     uint32_t* sidx = nullptr;    // sorted slot -> particle (sidxb[cur] after a sort)
     uint32_t* skey = nullptr;    // sorted slot -> bucket
     uint32_t* eid = nullptr;     // internal index -> file index
+    uint32_t* sfid = nullptr;    // sorted slot -> file index (reference-grid sorts only)
     double4* spos = nullptr;     // sorted copy of pos (reference-grid broad phases only)
     float4* spos32 = nullptr;    // sorted FP32 copy (broad-phase pre-test)
     // broad phase output

@@ -149,7 +149,8 @@ k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __res
           const uint32_t* __restrict__ hid, const uint32_t* __restrict__ rank,
           const float4* __restrict__ x32, const uint32_t* __restrict__ bstart,
           uint32_t* __restrict__ sidx, uint32_t* __restrict__ skey, double4* __restrict__ spos,
-          float4* __restrict__ spos32, uint32_t* __restrict__ tagg, int ntiles) {
+          float4* __restrict__ spos32, const uint32_t* __restrict__ eid, uint32_t* __restrict__ sfid,
+          uint32_t* __restrict__ tagg, int ntiles) {
     // the scan that read the histogram's tile totals has finished: clear them for the next sort
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < ntiles * TAGG_COPIES; i += TPB) tagg[i] = 0u;
@@ -162,6 +163,7 @@ k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __res
     skey[slot] = b;
     spos32[slot] = x32[t];
     if (spos) spos[slot] = pos[p];  // the reference-grid broad phases read FP64 sorted positions
+    if (sfid) sfid[slot] = eid[p];  // file index by slot, for the candidate count
 }
 
 // ------------------------------------------------------------------------------------------
@@ -621,7 +623,7 @@ k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32
 // Candidate-pair count of the reference's cp_list (kernels.jl:24-37): a diagnostic, evaluated on
 // the last step of each dem_step call.
 __global__ void __launch_bounds__(TPB)
-k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx,
+k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sfid,
              const uint32_t* __restrict__ bstart, const uint32_t* __restrict__ eid,
              unsigned long long* ctr) {
     const int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x;
@@ -632,7 +634,7 @@ k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __
         uint32_t w[8];
         window_of(P.sc.grid, mk3(x.x, x.y, x.z), w);
         for (int wi = 0; wi < 8; ++wi)
-            for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += ei < eid[sidx[k]];
+            for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += ei < sfid[k];
     }
     wave_add(ctr_at(ctr, CT_CAND), c);
 }
@@ -1619,7 +1621,8 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool 
     if (e != hipSuccess) return e;
     LAUNCH(s, K_SCATTER, hipLaunchKernelGGL(k_scatter, grid_for(P.n, TPB), dim3(TPB), 0, s->stream,
                                             P, d.pos[c], prev, d.hid, d.rank, d.x32, d.bstart, d.sidx,
-                                            d.skey, dense ? nullptr : d.spos, d.spos32, d.scan.tagg,
+                                            d.skey, dense ? nullptr : d.spos, d.spos32, d.eid,
+                                            dense ? nullptr : d.sfid, d.scan.tagg,
                                             dense ? (int)ntiles : 0));
     return hipSuccess;
 }
@@ -1644,7 +1647,7 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     const dim3 g = grid_for(P.n, TPB);
     if (count_candidates && !dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
-                                              d.sidx, d.bstart, d.eid, d.ctr));
+                                              d.sfid, d.bstart, d.eid, d.ctr));
     if (dense) {
         if (init) {
             LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, SEG>), g, dim3(TPB), 0,
@@ -1700,7 +1703,7 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
         // the cp_list length needs the reference's bucket sort: rebuild it after the merge
         if ((e = sort_impl<false>(s, false, false)) != hipSuccess) return e;
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_cand_count, g, dim3(TPB), 0, s->stream, P, d.pos[c],
-                                              d.sidx, d.bstart, d.eid, d.ctr));
+                                              d.sfid, d.bstart, d.eid, d.ctr));
     }
     return hipSuccess;
 }
