@@ -633,8 +633,17 @@ k_cand_count(const Params P, const double4* __restrict__ pos, const uint32_t* __
         const uint32_t ei = eid[i];
         uint32_t w[8];
         window_of(P.sc.grid, mk3(x.x, x.y, x.z), w);
-        for (int wi = 0; wi < 8; ++wi)
-            for (uint32_t k = bstart[w[wi]]; k < bstart[w[wi] + 1]; ++k) c += ei < sfid[k];
+        uint32_t lo[8], hi[8];
+#pragma unroll
+        for (int wi = 0; wi < 8; ++wi) { lo[wi] = bstart[w[wi]]; hi[wi] = bstart[w[wi] + 1]; }
+        for (int wi = 0; wi < 8; ++wi) {
+            uint32_t k = lo[wi];
+            for (; k + 3 < hi[wi]; k += 4) {  // four independent loads in flight
+                const uint32_t a = sfid[k], b = sfid[k + 1], d = sfid[k + 2], e = sfid[k + 3];
+                c += (uint64_t)(ei < a) + (ei < b) + (ei < d) + (ei < e);
+            }
+            for (; k < hi[wi]; ++k) c += ei < sfid[k];
+        }
     }
     wave_add(ctr_at(ctr, CT_CAND), c);
 }
