@@ -904,6 +904,8 @@ int32_t dem_batch_repack(dem_scene_t* src, int32_t nkeep, const int32_t* keep, i
     s->cfg = s->seg_cfg[0];
     s->der = s->seg_der[0];
     s->der.contact_slots = K;
+    // the kept scenes bring their contacts: start with the source's force-kernel block size
+    if (nkeep > 0) s->force_tpb = src->force_tpb;
     Params& P = s->P;
     P.n = total;
     P.K = K;
