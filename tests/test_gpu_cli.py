@@ -29,22 +29,21 @@ def _layout(tmp_path, k):
     return str(cfg)
 
 
-@pytest.mark.parametrize("k, max_steps, no_bond", [(1, 3000, False), (2, 3000, False),
-                                                   (2, 2000, True), (3, 20000, False)])
-def test_native_driver_writes_the_python_drivers_bytes(tmp_path, k, max_steps, no_bond):
+@pytest.mark.parametrize("k, max_steps, no_bond, precision", [
+    (1, 3000, False, "fp64"), (2, 3000, False, "fp64"), (2, 2000, True, "fp64"),
+    (3, 20000, False, "fp64"), (2, 3000, False, "mixed")])
+def test_native_driver_writes_the_python_drivers_bytes(tmp_path, k, max_steps, no_bond, precision):
     from taichi_dem_julia_port_amd.executor import solve
-    print(f"[case] demhip_solve cfg{k} max_steps={max_steps} no_bond={no_bond}", flush=True)
+    print(f"[case] demhip_solve cfg{k} max_steps={max_steps} no_bond={no_bond} {precision}", flush=True)
     cfg = _layout(tmp_path, k)
     a, b = tmp_path / "native", tmp_path / "python"
-    a.mkdir()
-    b.mkdir()
-    args = [_build.CLI, cfg, "--out", str(a), "--max-steps", str(max_steps)]
+    args = [_build.CLI, cfg, "--out", str(a), "--max-steps", str(max_steps), "--precision", precision]
     if no_bond:
         args.append("--no-bond")
     r = subprocess.run(args, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "Total time" in r.stderr
-    solve(cfg, no_bond=no_bond, out_dir=str(b), max_steps=max_steps).close()
+    solve(cfg, no_bond=no_bond, out_dir=str(b), max_steps=max_steps, precision=precision).close()
     for name in ("output.p4p", "output.p4c"):
         x, y = (a / name).read_bytes(), (b / name).read_bytes()
         assert x.count(b"TIMESTEP") >= 3, name
