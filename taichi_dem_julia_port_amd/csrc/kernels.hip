@@ -794,16 +794,16 @@ __device__ __forceinline__ uint32_t sort_unique(uint32_t nt, Get get, Set set) {
 // with batched independent loads; longer rows are merged in place in global memory.
 constexpr int MERGE_LDS = 16;
 
-template <bool INIT, bool SEG>
-__global__ void __launch_bounds__(MERGE_TPB)
+template <bool INIT, bool SEG, int BS>
+__global__ void __launch_bounds__(BS)
 k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ hid,
         uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch, uint8_t* __restrict__ sel,
         uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1, uint32_t* __restrict__ pq0,
         uint32_t* __restrict__ pq1, double* __restrict__ st0, double* __restrict__ st1,
         unsigned long long* ctr, int diag) {
-    __shared__ uint32_t lt[MERGE_LDS * MERGE_TPB];  // touching partners
-    __shared__ uint32_t lw[MERGE_LDS * MERGE_TPB];  // old-row words
-    const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * MERGE_TPB + threadIdx.x;
+    __shared__ uint32_t lt[MERGE_LDS * BS];  // touching partners
+    __shared__ uint32_t lw[MERGE_LDS * BS];  // old-row words
+    const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     const int64_t n = P.n;
     const int K = P.K;
     uint64_t created = 0, deact = 0, entries = 0;
@@ -826,21 +826,21 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
 #pragma unroll
                     for (int u = 0; u < 8; ++u) v[u] = m + u < nt ? touch[(int64_t)(m + u) * n + p] : 0u;
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) if (m + u < nt) lt[(m + u) * MERGE_TPB + tid] = v[u];
+                    for (int u = 0; u < 8; ++u) if (m + u < nt) lt[(m + u) * BS + tid] = v[u];
                 }
                 for (uint32_t m = 0; m < nold; m += 8) {
                     uint32_t v[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) v[u] = m + u < nold ? pqo[(int64_t)(m + u) * n + p] : 0u;
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) if (m + u < nold) lw[(m + u) * MERGE_TPB + tid] = v[u];
+                    for (int u = 0; u < 8; ++u) if (m + u < nold) lw[(m + u) * BS + tid] = v[u];
                 }
                 const uint32_t nu = sort_unique(
-                    nt, [&](uint32_t m) { return lt[m * MERGE_TPB + tid]; },
-                    [&](uint32_t m, uint32_t v) { lt[m * MERGE_TPB + tid] = v; });
+                    nt, [&](uint32_t m) { return lt[m * BS + tid]; },
+                    [&](uint32_t m, uint32_t v) { lt[m * BS + tid] = v; });
                 merge_rows<INIT>(S, pos, p, n, K, nold, nu, b,
-                                 [&](uint32_t m) { return lw[m * MERGE_TPB + tid]; },
-                                 [&](uint32_t m) { return lt[m * MERGE_TPB + tid]; }, sel, rcnt0,
+                                 [&](uint32_t m) { return lw[m * BS + tid]; },
+                                 [&](uint32_t m) { return lt[m * BS + tid]; }, sel, rcnt0,
                                  rcnt1, pq0, pq1, st0, st1, created, deact, entries, overflow);
             } else {
                 // long rows: the touching list is sorted in place (it is rebuilt every step)
@@ -1697,13 +1697,21 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     const size_t lds = 0;
     const dim3 gm = grid_for(P.n, MERGE_TPB);
     if (init) {
-        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG>), gm, dim3(MERGE_TPB), lds, s->stream,
-                                              P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
-                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, diag));
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
+                                              s->stream, P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel,
+                                              d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
+                                              d.ctr, diag));
+    } else if (s->force_tpb == 64) {
+        // contact-dense, as the force kernel's block choice: one-wave blocks for uneven rows
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), lds,
+                                              s->stream, P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel,
+                                              d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
+                                              d.ctr, diag));
     } else {
-        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG>), gm, dim3(MERGE_TPB), lds, s->stream,
-                                              P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel, d.rcnt[0],
-                                              d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, diag));
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
+                                              s->stream, P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel,
+                                              d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
+                                              d.ctr, diag));
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
