@@ -379,7 +379,7 @@ __device__ __forceinline__ void broad_ranges(const DenseGrid& G, uint32_t cbase,
 }
 
 constexpr int BROAD_BATCH = 4;
-constexpr int STAGE_CAP = 2048;  // float4 per block (32 KiB); a power of two
+constexpr int STAGE_CAP = 1600;  // float4 per block (25 KiB): five blocks per CU fit in the LDS
 constexpr int HIT_QUEUE = 256;   // pre-test survivors queued per wave
 
 template <bool INIT, bool SEG>
