@@ -379,11 +379,14 @@ __device__ __forceinline__ void broad_ranges(const DenseGrid& G, uint32_t cbase,
 }
 
 constexpr int BROAD_BATCH = 4;
-constexpr int STAGE_CAP = 1600;  // float4 per block (25 KiB): five blocks per CU fit in the LDS
+constexpr int STAGE_CAP = 1344;  // float4 per block (21 KiB): six blocks per CU fit in the LDS
 constexpr int HIT_QUEUE = 256;   // pre-test survivors queued per wave
 
+// A thread waits on a chain of dependent loads (key, bucket bounds, union bounds, stage), so the
+// single-scene kernel is held to 80 VGPRs for six waves per SIMD (a little spill, faster); the
+// segmented one slowed down when squeezed.
 template <bool INIT, bool SEG>
-__global__ void __launch_bounds__(TPB)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(SEG ? 1 : 6)))
 k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
               const double4* __restrict__ pos, const float4* __restrict__ spos32,
               const uint32_t* __restrict__ bstart, const double* __restrict__ r0,
