@@ -499,17 +499,15 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     if (work) {
         const float slack = 8.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z));
         const float fmult = (float)mult * (1.0f + 64.0f * FLT_EPSILON);
-        // where each range starts in the stage (inside its union by construction; checked so a
-        // stage read can never stray)
-        uint32_t lb[5];
+        // each range lies inside its union by construction; checked so a stage read can never
+        // stray.  Range j starts at stage slot u_off[j] + (lo[j] - u_lo[j]), recomputed from LDS
+        // in the loop rather than held in registers.
         bool in_stage = staged;
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
-            lb[j] = 0u;
             if (staged && len[j]) {
                 const uint32_t ulen = u_off[j + 1] - u_off[j];
                 if (lo[j] < u_lo[j] || lo[j] - u_lo[j] + len[j] > ulen) in_stage = false;
-                lb[j] = u_off[j] + (lo[j] - u_lo[j]);
             }
         }
         // FP32 pre-test |d|^2 <= thr^2, thr = (r_i + r_j) * fmult + slack, evaluated with fused
@@ -560,7 +558,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
 #pragma unroll 1
             for (int j = 0; j < 5; ++j) {
                 BROAD_RANGE(j);
-                const uint32_t rlb = j == 0 ? lb[0] : j == 1 ? lb[1] : j == 2 ? lb[2] : j == 3 ? lb[3] : lb[4];
+                const uint32_t rlb = rlen ? u_off[j] + (rlo - u_lo[j]) : 0u;
                 const float4* sp = stage + rlb;  // reads past the range stay in the padded stage
 #pragma unroll 1
                 for (uint32_t i = 0; i < rlen; i += BROAD_BATCH) {
