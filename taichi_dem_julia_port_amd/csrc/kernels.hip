@@ -383,8 +383,8 @@ constexpr int STAGE_CAP = 1344;  // float4 per block (21 KiB): six blocks per CU
 constexpr int HIT_QUEUE = 256;   // pre-test survivors queued per wave
 
 // A thread waits on a chain of dependent loads (key, bucket bounds, union bounds, stage), so the
-// single-scene kernel is held to 80 VGPRs for six waves per SIMD (a little spill, faster); the
-// segmented one slowed down when squeezed.
+// single-scene kernel is held to 80 VGPRs for six waves per SIMD (the per-step variant does not
+// spill); the segmented one slowed down when squeezed.
 template <bool INIT, bool SEG>
 __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(SEG ? 1 : 6)))
 k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
@@ -473,7 +473,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     }
     __syncthreads();
 
-    uint64_t found = 0;
+    uint32_t found = 0u;  // touching pairs this thread owns (few)
     uint32_t overflow = 0;
     const double mult = INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0;
     // FP64 tests of a pre-test survivor (slot k) of the particle in slot to (index po), and the
