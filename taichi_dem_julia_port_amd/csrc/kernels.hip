@@ -383,10 +383,10 @@ constexpr int STAGE_CAP = 1344;  // float4 per block (21 KiB): six blocks per CU
 constexpr int HIT_QUEUE = 256;   // pre-test survivors queued per wave
 
 // A thread waits on a chain of dependent loads (key, bucket bounds, union bounds, stage), so the
-// single-scene kernel is held to 80 VGPRs for six waves per SIMD (the per-step variant does not
-// spill); the segmented one slowed down when squeezed.
+// kernel is held to the register count of six waves per SIMD (80 VGPRs) for single scenes and
+// five (96) for batches; neither per-step variant spills.
 template <bool INIT, bool SEG>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(SEG ? 1 : 6)))
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(SEG ? 5 : 6)))
 k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
               const double4* __restrict__ pos, const float4* __restrict__ spos32,
               const uint32_t* __restrict__ bstart, const double* __restrict__ r0,
