@@ -206,6 +206,17 @@ int32_t dem_debug_cp_list(dem_scene_t* s, int32_t* pairs, int64_t cap, int64_t* 
 /* Exclusive prefix sum of n uint32 values with the engine's decoupled-lookback scan (the
  * replacement of CUDA.jl accumulate!, executor.jl:50,62); *total receives the sum. */
 int32_t dem_debug_scan(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* total);
+/* No counterpart in the reference.  The scan of dem_debug_scan for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): in, out and total (may be NULL) are DEVICE pointers, the scan is enqueued on
+ * stream (a hipStream_t; NULL = the null stream) and the call returns without waiting.  Its
+ * look-back scratch is a fixed array of about 1 MiB per GPU that the library holds until the
+ * process ends; n is limited to 2^30 (DEM_ERR_INVALID beyond).  Every call on one GPU shares
+ * that array, so calls whose scans overlap in time on one GPU (different streams, or threads)
+ * race; calls ordered on one stream do not.  The first call on a device resolves the array's
+ * address and should not be made under graph capture. */
+int32_t dem_scan_noalloc(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* total,
+                         void* stream);
 
 /* ---- per-kernel timing (HIP events on the scene's stream) ------------------------------ */
 #define DEM_NUM_KERNELS 8

@@ -160,6 +160,8 @@ SIGNATURES = [
                                            P(ctypes.c_int64)]),
     ("dem_debug_scan", ctypes.c_int32, [P(ctypes.c_uint32), P(ctypes.c_uint32), ctypes.c_int64,
                                         P(ctypes.c_uint64)]),
+    ("dem_scan_noalloc", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                          ctypes.c_void_p, ctypes.c_void_p]),
     ("dem_set_timing", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("dem_get_kernel_times", ctypes.c_int32, [ctypes.c_void_p, P(DemKernelTimes)]),
     ("dem_kernel_name", ctypes.c_char_p, [ctypes.c_int32]),

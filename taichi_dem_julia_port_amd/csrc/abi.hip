@@ -200,6 +200,8 @@ void choose_force_blocks(dem_scene* s) {
 
 }  // namespace
 
+void dem::set_last_error_global(const std::string& msg) { g_create_err = msg; }
+
 int32_t dem::timing_flush(dem_scene* s) {
     if (s->nev == 0) return DEM_OK;
     HIPCHK(s, hipStreamSynchronize(s->stream));

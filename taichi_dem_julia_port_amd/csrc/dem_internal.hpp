@@ -252,6 +252,8 @@ hipError_t run_read_contacts(dem_scene* s, dem_contact_t** out_dev, uint8_t** bo
                              int64_t* count);
 hipError_t run_write_contacts(dem_scene* s, const uint64_t* dst_dev, const uint32_t* word_dev,
                               const double* state_dev, int64_t m);
+// the message dem_last_error(NULL) returns (entry points without a scene handle)
+void set_last_error_global(const std::string& msg);
 void timing_begin(dem_scene* s, int kernel);
 void timing_end(dem_scene* s);
 int32_t timing_flush(dem_scene* s);  // abi.hip: collect elapsed times of recorded pairs

@@ -13,6 +13,8 @@
 #include "dem_internal.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace dem {
 
@@ -289,6 +291,48 @@ hipError_t launch_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n
     return hipGetLastError();
 }
 
+// ---- the allocation-free scan (dem_scan_noalloc) -------------------------------------------
+// Its look-back scratch lives in this file-scope array: words [0, 32) hold the two tile counters
+// (the first 8 bytes; the rest pads the status words to a 256-byte boundary), words from 32 on
+// hold one status word per tile.  Each GPU holds its own copy for the life of the process.
+constexpr int64_t NOALLOC_MAX_VALUES = (int64_t)1 << 30;
+constexpr int64_t NOALLOC_HEAD_WORDS = 32;
+constexpr int64_t NOALLOC_MAX_TILES = ((int64_t)1 << 30) / (SCAN_THREADS * SCAN_ITEMS_LARGE);
+static_assert(NOALLOC_MAX_VALUES >= SCAN_LARGE_FROM, "the largest scan uses long tiles");
+static __device__ __align__(256) uint64_t scratch[NOALLOC_HEAD_WORDS + NOALLOC_MAX_TILES];
+
+namespace {
+
+// Clears the scratch words a scan of ntiles tiles reads (counters and status words), and writes
+// a zero total for an empty scan.  A kernel rather than hipMemsetAsync, so that a caller may
+// capture the call into a graph.
+__global__ void __launch_bounds__(SCAN_THREADS)
+k_noalloc_clear(uint64_t* __restrict__ words, int64_t nwords, uint64_t* __restrict__ zero_total) {
+    const int64_t i = (int64_t)blockIdx.x * SCAN_THREADS + threadIdx.x;
+    if (i < nwords) words[i] = 0ull;
+    if (zero_total && i == 0) *zero_total = 0ull;
+}
+
+// The address of this device's copy of scratch, resolved once per device (the host-side name
+// `scratch` is not a device address).
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mu;
+uint64_t* g_noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+hipError_t noalloc_scratch(int device, uint64_t** out) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_noalloc_ptr[device] = (uint64_t*)p;
+    }
+    *out = g_noalloc_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
 hipError_t launch_tile_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n,
                             hipStream_t stream) {
     if (n <= 0) return hipSuccess;
@@ -304,3 +348,37 @@ hipError_t launch_tile_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int6
 }
 
 }  // namespace dem
+
+extern "C" int32_t dem_scan_noalloc(const uint32_t* in, uint32_t* out, int64_t n, uint64_t* total,
+                                    void* stream) {
+    using namespace dem;
+    if (n < 0 || n > NOALLOC_MAX_VALUES || (n > 0 && (!in || !out))) {
+        set_last_error_global("dem_scan_noalloc: need 0 <= n <= 2^30 and device buffers in, out");
+        return DEM_ERR_INVALID;
+    }
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e == hipSuccess && (device < 0 || device >= NOALLOC_MAX_DEVICES)) e = hipErrorInvalidDevice;
+    uint64_t* words = nullptr;
+    if (e == hipSuccess) e = noalloc_scratch(device, &words);
+    const hipStream_t st = (hipStream_t)stream;
+    const int64_t ntiles = n > 0 ? scan_tiles(n) : 0;
+    if (e == hipSuccess) {
+        const int64_t nwords = NOALLOC_HEAD_WORDS + ntiles;
+        hipLaunchKernelGGL(k_noalloc_clear, dim3((uint32_t)((nwords + SCAN_THREADS - 1) / SCAN_THREADS)),
+                           dim3(SCAN_THREADS), 0, st, words, nwords, n == 0 ? total : nullptr);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess && n > 0) {
+        ScanWorkspace ws;
+        ws.counters = (uint32_t*)words;
+        ws.status = words + NOALLOC_HEAD_WORDS;
+        ws.max_tiles = NOALLOC_MAX_TILES;
+        e = launch_scan(ws, const_cast<uint32_t*>(in), out, n, false, total, st);
+    }
+    if (e != hipSuccess) {
+        set_last_error_global(std::string("dem_scan_noalloc: ") + hipGetErrorString(e));
+        return DEM_ERR_HIP;
+    }
+    return DEM_OK;
+}
