@@ -116,13 +116,23 @@ def build_workload(name: str, n: int):
     raise SystemExit(f"unknown workload {name}")
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries the one JSON result line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
     from taichi_dem_julia_port_amd.executor import Batch, Scene
+    progress(f"engine run: {precision}, {steps} steps after {warmup} warm-up")
     s = (Batch(cfg, precision=precision, graph=graph) if g is None
          else Scene(cfg, g, w, precision=precision, graph=graph))
     if bonded:
         s.initialize()
     s.step(warmup)
+    # capture the step graph of both buffer parities (a call of 3 steps replays one two-step
+    # graph and flips the parity), so that no capture falls into a timed call
+    s.step(3)
+    s.step(3)
     # per-kernel durations: K steps with a HIP event pair around every launch on the scene's
     # stream (events keep these steps out of the captured graph)
     s.set_timing(True)
@@ -136,6 +146,10 @@ def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
     t2 = time.perf_counter()
     s.step(steps)
     t3 = time.perf_counter()
+    # one more step, untimed, with the diagnostic passes: the reference's cp_list length and the
+    # candidates the broad phase examined
+    s.set_diagnostics(True)
+    s.step(1)
     c = s.counters()
     d = s.derived()
     s.close()
@@ -172,6 +186,7 @@ def precision_error(cfg, g, w, bonded, steps=1000):
         s.close()
         return out
 
+    progress(f"position error after {steps} steps: mixed and perturbed FP64 runs")
     a, b = run("fp64"), run("mixed")
     c, e, f = run("fp64", 0.0), run("fp64", 1e-9), run("fp64", 1e-7)
     scale_k = max(float(np.abs(a["k"]).max()), 1e-300)
@@ -265,6 +280,42 @@ def run_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def summarize(r, workload, precision, nwall, n, steps):
+    """Throughput, per-kernel durations and the dominant kernel's roofline of one engine run."""
+    kt = r["kernels"]
+    step_ms = {k: (ms / max(l, 1)) for k, (ms, l) in kt.items() if l}
+    total_kernel_ms = sum(ms for ms, _ in kt.values())
+    dom = max(((k, ms) for k, (ms, l) in kt.items() if l), key=lambda kv: kv[1])[0]
+    c = r["counters"]
+    alg = _algorithmic_bytes(dom, c, nwall, precision)
+    achieved = alg / (step_ms[dom] * 1e-3) / 1e9 if step_ms[dom] > 0 else 0.0
+    traffic, tsrc = measured_traffic(workload, precision, n)
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic.get(dom) if traffic else None, "algorithmic_bytes": alg,
+                "avg_launch_ms": round(step_ms[dom], 5)}
+    if traffic and traffic.get(dom):
+        roofline["traffic_source"] = tsrc
+        roofline["traffic_GB_s"] = round(traffic[dom] / (step_ms[dom] * 1e-3) / 1e9, 1)
+    if dom == "broad_phase":
+        # candidates examined are re-read from L2/MALL, not HBM: the kernel's practical limiter
+        roofline["candidates_examined"] = c["examined"]
+        roofline["cache_GB_s"] = round(c["examined"] * 16 / (step_ms[dom] * 1e-3) / 1e9, 1)
+        roofline["reference_equivalent_GB_s"] = round(
+            _reference_equivalent_bytes(c) / (step_ms[dom] * 1e-3) / 1e9, 1)
+    per_kernel = {}
+    for k, ms in step_ms.items():
+        b = _algorithmic_bytes(k, c, nwall, precision)
+        if b > 0 and ms > 0:
+            per_kernel[k] = round(b / (ms * 1e-3) / 1e9, 1)
+    return {"value": n * steps / r["elapsed"],
+            "ms_per_step": round(1e3 * r["elapsed"] / steps, 5),
+            "kernel_ms_per_launch": {k: round(v, 5) for k, v in step_ms.items()},
+            "kernel_share": {k: round(ms / total_kernel_ms, 4) for k, (ms, l) in kt.items() if l},
+            "ms_per_step_with_kernel_events": round(1e3 * r["elapsed_events"] / steps, 5),
+            "roofline": roofline, "kernel_GB_s": per_kernel}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,8 +328,10 @@ def main():
     ap.add_argument("--epoch", type=int, default=1000)
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--compare-precision", action="store_true",
-                    help="also time the other precision and report both")
+    ap.add_argument("--no-compare-precision", action="store_true",
+                    help="skip timing the other precision and the 1000-step position error")
+    ap.add_argument("--no-contact-heavy", action="store_true",
+                    help="skip the pour's contact-heavy state (reported beside the pour)")
     ap.add_argument("--cpu-steps", type=int, default=0,
                     help="steps of the serial oracle's CPU baseline (default: 2 for the pours, 20 "
                          "for the 18k cube, 2000 for the 900-grain cube, 5 per batch scene); the "
@@ -315,41 +368,16 @@ def main():
         w = cfg[0][2]
     r = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup,
                    graph=not args.no_graph)
-    value = n * args.steps / r["elapsed"]
-    kt = r["kernels"]
-    step_ms = {k: (ms / max(l, 1)) for k, (ms, l) in kt.items() if l}
-    total_kernel_ms = sum(ms for ms, _ in kt.values())
-    dom = max(((k, ms) for k, (ms, l) in kt.items() if l), key=lambda kv: kv[1])[0]
+    m = summarize(r, args.workload, args.precision, len(w), n, args.steps)
     c = r["counters"]
-    alg = _algorithmic_bytes(dom, c, len(w), args.precision)
-    achieved = alg / (step_ms[dom] * 1e-3) / 1e9 if step_ms[dom] > 0 else 0.0
-    traffic, tsrc = measured_traffic(args.workload, args.precision, n)
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic.get(dom) if traffic else None, "algorithmic_bytes": alg,
-                "avg_launch_ms": round(step_ms[dom], 5)}
-    if traffic and traffic.get(dom):
-        roofline["traffic_source"] = tsrc
-        roofline["traffic_GB_s"] = round(traffic[dom] / (step_ms[dom] * 1e-3) / 1e9, 1)
-    if dom == "broad_phase":
-        # candidates examined are re-read from L2/MALL, not HBM: the kernel's practical limiter
-        roofline["candidates_examined"] = c["examined"]
-        roofline["cache_GB_s"] = round(c["examined"] * 16 / (step_ms[dom] * 1e-3) / 1e9, 1)
-        roofline["reference_equivalent_GB_s"] = round(
-            _reference_equivalent_bytes(c) / (step_ms[dom] * 1e-3) / 1e9, 1)
-    per_kernel = {}
-    for k, ms in step_ms.items():
-        b = _algorithmic_bytes(k, c, len(w), args.precision)
-        if b > 0 and ms > 0:
-            per_kernel[k] = round(b / (ms * 1e-3) / 1e9, 1)
     out = {
         "metric": "particle-steps/s (1M grains, FP32 vs FP64)",
-        "value": round(value, 1),
+        "value": round(m["value"], 1),
         "unit": "particle-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * r["elapsed"] / args.steps, 5),
+        "ms_per_step": m["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -361,26 +389,47 @@ def main():
         "counters": {k: c[k] for k in ("candidate_pairs", "examined", "grid_cells", "contacts",
                                        "bonded", "hertz", "wall_contacts", "bonds_broken_total",
                                        "overflow")},
-        "kernel_ms_per_launch": {k: round(v, 5) for k, v in step_ms.items()},
-        "kernel_share": {k: round(ms / total_kernel_ms, 4) for k, (ms, l) in kt.items() if l},
-        "ms_per_step_with_kernel_events": round(1e3 * r["elapsed_events"] / args.steps, 5),
+        "kernel_ms_per_launch": m["kernel_ms_per_launch"],
+        "kernel_share": m["kernel_share"],
+        "ms_per_step_with_kernel_events": m["ms_per_step_with_kernel_events"],
         "graph": not args.no_graph,
-        "roofline": roofline,
-        "kernel_GB_s": per_kernel,
+        "roofline": m["roofline"],
+        "kernel_GB_s": m["kernel_GB_s"],
     }
     if g is None:
         out["scenes"] = len(cfg)
         out["scene_steps_per_s"] = round(len(cfg) * args.steps / r["elapsed"], 1)
         v1, idx = one_at_a_time(cfg, args.precision, args.steps)
         out["one_at_a_time"] = {"value": round(v1, 1), "unit": "particle-steps/s",
-                                "sample_scenes": idx, "speedup_batched": round(value / v1, 2)}
-    if args.compare_precision:
+                                "sample_scenes": idx, "speedup_batched": round(m["value"] / v1, 2)}
+    if not args.no_compare_precision:
         other = "mixed" if args.precision == "fp64" else "fp64"
         r2 = run_engine(cfg, g, w, bonded, other, args.steps, args.warmup,
                         graph=not args.no_graph)
-        out[f"value_{other}"] = round(n * args.steps / r2["elapsed"], 1)
-        out[f"ms_per_step_{other}"] = round(1e3 * r2["elapsed"] / args.steps, 5)
+        m2 = summarize(r2, args.workload, other, len(w), n, args.steps)
+        out[f"value_{other}"] = round(m2["value"], 1)
+        out[f"ms_per_step_{other}"] = m2["ms_per_step"]
+        out[f"{other}_speedup_over_{args.precision}"] = round(m2["value"] / m["value"], 3)
+        out[f"kernel_ms_per_launch_{other}"] = m2["kernel_ms_per_launch"]
         out["position_error_after_1000_steps"] = precision_error(cfg, g, w, bonded)
+    if args.workload == "pour" and not args.no_contact_heavy:
+        # the same grains in their contact-heavy state (the pour has no contacts for its first
+        # thousands of steps): where the contact kernels are measured at this size
+        progress("contact-heavy state of the pour")
+        cfg_p, g_p, w_p, _, desc_p = build_workload("pour_packed", args.n)
+        heavy = {"workload": desc_p}
+        for prec in ("fp64", "mixed"):
+            rp = run_engine(cfg_p, g_p, w_p, False, prec, args.steps, args.warmup,
+                            graph=not args.no_graph)
+            mp = summarize(rp, "pour_packed", prec, len(w_p), len(g_p), args.steps)
+            heavy[prec] = {"value": round(mp["value"], 1), "ms_per_step": mp["ms_per_step"],
+                           "kernel_ms_per_launch": mp["kernel_ms_per_launch"],
+                           "roofline": mp["roofline"]}
+            heavy["counters"] = {k: rp["counters"][k] for k in ("contacts", "hertz",
+                                                                "wall_contacts", "examined",
+                                                                "candidate_pairs")}
+        heavy["mixed_speedup_over_fp64"] = round(heavy["mixed"]["value"] / heavy["fp64"]["value"], 3)
+        out["contact_heavy"] = heavy
     if not args.no_cpu_baseline:
         from oracle.oracle import num_threads
         if args.cpu_steps <= 0:
@@ -394,6 +443,7 @@ def main():
                     f"{dt:.1f} s") if g is not None else (
                     f"first 8 scenes of the batch one after another, {steps} steps each after "
                     f"initialize + 1 warm-up step, {how}, {dt:.1f} s")
+        progress("CPU baseline: serial and OpenMP oracle")
         v1, dt1 = cpu_baseline(cfg, g, w, bonded, args.cpu_steps)
         threads = num_threads(True)
         vp, dtp = cpu_baseline(cfg, g, w, bonded, omp_steps, parallel=True)

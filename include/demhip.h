@@ -140,7 +140,8 @@ typedef struct dem_counters {
     int64_t step;              /* cumulative simulate! calls */
     int64_t n;                 /* particles */
     int64_t hash_table_size;   /* H */
-    int64_t candidate_pairs;   /* P: cp_list length the reference would build (executor.jl:63) */
+    int64_t candidate_pairs;   /* P: cp_list length the reference would build (executor.jl:63);
+                                  -1 unless the diagnostic passes ran (dem_set_diagnostics) */
     int64_t contacts;          /* C: active contacts after the step (save_single's count) */
     int64_t bonded;            /* C_b: active bonded contacts */
     int64_t hertz;             /* C_h = C - C_b */
@@ -151,7 +152,8 @@ typedef struct dem_counters {
     int64_t bonds_broken_total;/* cumulative */
     int64_t fatal_gap;         /* unbonded active contacts evaluated with gap > 0 */
     int64_t overflow;          /* non-zero: contact capacity exceeded (results invalid) */
-    int64_t examined;          /* candidate pairs examined by the engine's own broad phase */
+    int64_t examined;          /* candidate pairs examined by the engine's own broad phase;
+                                  -1 unless the diagnostic passes ran */
     int64_t grid_cells;        /* cells of the engine's dense sort grid (0: reference grid used) */
 } dem_counters_t;
 
@@ -175,6 +177,11 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps);    /* nsteps x simulate!  exec
 /* ---- read-back / state transfer --------------------------------------------------------- */
 int32_t dem_get_derived(const dem_scene_t* s, dem_derived_t* out);
 int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out);
+/* Diagnostic passes on the last step of every dem_step / dem_initialize call (off by default):
+ * the reference's cp_list length P (total_cps, executor.jl:62-63), counted on the reference's own
+ * hashed grid, and the candidates the engine's broad phase examined.  They cost about two
+ * steps' worth of extra kernels per call, so the production path leaves them off. */
+int32_t dem_set_diagnostics(dem_scene_t* s, int32_t on);
 /* Array(grains) (utils.jl:40): all n grains in file order. */
 int32_t dem_read_grains(dem_scene_t* s, dem_grain_t* out, int64_t n);
 /* Overwrite the dynamic grain state (k, v, a, q, w, dw) of all n grains. */

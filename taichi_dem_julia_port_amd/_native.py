@@ -142,6 +142,7 @@ SIGNATURES = [
     ("dem_step", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64]),
     ("dem_get_derived", ctypes.c_int32, [ctypes.c_void_p, P(DemDerived)]),
     ("dem_get_counters", ctypes.c_int32, [ctypes.c_void_p, P(DemCounters)]),
+    ("dem_set_diagnostics", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int32]),
     ("dem_read_grains", ctypes.c_int32, [ctypes.c_void_p, P(DemGrain), ctypes.c_int64]),
     ("dem_write_grains", ctypes.c_int32, [ctypes.c_void_p, P(DemGrain), ctypes.c_int64]),
     ("dem_read_contacts", ctypes.c_int32, [ctypes.c_void_p, P(DemContact), P(ctypes.c_uint8),

@@ -161,6 +161,14 @@ hipError_t to_device(const dem_scene* s, void* dst, const void* src, size_t byte
 // executor.jl:223 r = cbrt(V / (4/3 pi))
 double grain_radius(double V) { return cbrt(V / (4.0 / 3.0 * M_PI)); }
 
+void drop_graphs(dem_scene* s) {
+    for (int b = 0; b < 2; ++b) {
+        if (s->graph_exec[b]) { hipGraphExecDestroy(s->graph_exec[b]); s->graph_exec[b] = nullptr; }
+        if (s->graph[b]) { hipGraphDestroy(s->graph[b]); s->graph[b] = nullptr; }
+        s->graph_force_tpb[b] = 0;
+    }
+}
+
 void free_scene(dem_scene* s) {
     if (!s) return;
     DevState& d = s->d;
@@ -172,8 +180,7 @@ void free_scene(dem_scene* s) {
     if (s->h_ctr_raw) hipHostFree(s->h_ctr_raw);
     for (hipEvent_t e : s->ev)
         if (e) hipEventDestroy(e);
-    if (s->graph_exec) hipGraphExecDestroy(s->graph_exec);
-    if (s->graph) hipGraphDestroy(s->graph);
+    drop_graphs(s);
     if (s->stream) hipStreamDestroy(s->stream);
     delete[] s->h_id; delete[] s->h_gid; delete[] s->h_mid; delete[] s->h_V; delete[] s->h_m;
     delete[] s->h_r; delete[] s->h_r0; delete[] s->h_I;
@@ -236,8 +243,7 @@ int32_t resize_rows(dem_scene* s, int32_t k) {
     s->K = k;
     s->P.K = k;
     s->der.contact_slots = k;
-    if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
-    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
+    drop_graphs(s);
     return DEM_OK;
 }
 
@@ -1015,11 +1021,12 @@ int32_t dem_initialize(dem_scene_t* s) {
     return DEM_OK;
 }
 
-// Capture two plain steps (no output writes, no diagnostics) starting from the current buffer.
+// Capture two plain steps (no output writes, no diagnostics) starting from the current buffer
+// parity, kept as that parity's graph.
 static int32_t capture_graph(dem_scene* s) {
-    if (s->graph_exec) { hipGraphExecDestroy(s->graph_exec); s->graph_exec = nullptr; }
-    if (s->graph) { hipGraphDestroy(s->graph); s->graph = nullptr; }
     const int start = s->cur;
+    if (s->graph_exec[start]) { hipGraphExecDestroy(s->graph_exec[start]); s->graph_exec[start] = nullptr; }
+    if (s->graph[start]) { hipGraphDestroy(s->graph[start]); s->graph[start] = nullptr; }
     HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
@@ -1031,10 +1038,9 @@ static int32_t capture_graph(dem_scene* s) {
     s->cur = start;  // capture only recorded the launches
     if (e != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "graph capture", e); }
     if (e2 != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "hipStreamEndCapture", e2); }
-    s->graph = g;
-    HIPCHK(s, hipGraphInstantiate(&s->graph_exec, g, nullptr, nullptr, 0));
-    s->graph_parity = start;
-    s->graph_force_tpb = s->force_tpb;
+    s->graph[start] = g;
+    HIPCHK(s, hipGraphInstantiate(&s->graph_exec[start], g, nullptr, nullptr, 0));
+    s->graph_force_tpb[start] = s->force_tpb;
     return DEM_OK;
 }
 
@@ -1044,12 +1050,13 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
     int64_t t = 0;
     // body: pairs of steps replayed from a captured graph (not while timing individual kernels)
     if (s->use_graph && !s->timing && nsteps >= 3) {
-        if (!s->graph_exec || s->graph_parity != s->cur || s->graph_force_tpb != s->force_tpb) {
+        const int b = s->cur;  // a graph of two steps ends at the parity it starts from
+        if (!s->graph_exec[b] || s->graph_force_tpb[b] != s->force_tpb) {
             int32_t rc = capture_graph(s);
             if (rc) return rc;
         }
         for (; t + 2 < nsteps; t += 2) {
-            HIPCHK(s, hipGraphLaunch(s->graph_exec, s->stream));
+            HIPCHK(s, hipGraphLaunch(s->graph_exec[b], s->stream));
             s->steps += 2;
         }
     }
@@ -1067,6 +1074,12 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
     return DEM_OK;
 }
 
+int32_t dem_set_diagnostics(dem_scene_t* s, int32_t on) {
+    if (!s) return DEM_ERR_INVALID;
+    s->diagnostics = on != 0;
+    return DEM_OK;
+}
+
 int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
     if (!s || !out) return DEM_ERR_INVALID;
     HIPCHK(s, hipSetDevice(s->device));
@@ -1077,7 +1090,8 @@ int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
     out->step = s->steps;
     out->n = s->n;
     out->hash_table_size = s->der.hash_table_size;
-    out->candidate_pairs = (int64_t)c[CT_CAND];
+    // counted only by the diagnostic passes (dem_set_diagnostics); -1: not counted
+    out->candidate_pairs = s->diag_valid ? (int64_t)c[CT_CAND] : -1;
     if (s->steps == 0) {  // after initialize only: every created contact is a bond
         out->contacts = (int64_t)c[CT_CREATED];
         out->bonded = (int64_t)c[CT_CREATED];
@@ -1093,7 +1107,7 @@ int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
     out->bonds_broken_total = (int64_t)c[CT_BROKEN_TOTAL];
     out->fatal_gap = (int64_t)c[CT_FATAL];
     out->overflow = (int64_t)c[CT_OVERFLOW];
-    out->examined = (int64_t)c[CT_EXAMINED];
+    out->examined = s->diag_valid ? (int64_t)c[CT_EXAMINED] : -1;
     out->grid_cells = s->dense ? (int64_t)s->P.sort_keys : 0;
     return DEM_OK;
 }

@@ -222,14 +222,18 @@ struct dem_scene {
     unsigned long long* h_ctr_raw = nullptr;  // pinned, CTR_WORDS
     std::string err;
     // two steps captured as one HIP graph (the particle buffers alternate, so two steps restore
-    // the buffer parity); replayed for the body of every dem_step call
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-    int graph_parity = -1;   // value of cur the graph was captured from
+    // the buffer parity), one graph for each buffer parity cur it starts from; replayed for the
+    // body of every dem_step call
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
     // threads per block of the force kernel: 64 when the last counted step had at least one row
-    // entry per four particles, else 256; the graph holds the value it was captured with
+    // entry per four particles, else 256; each graph holds the value it was captured with
     int force_tpb = 256;
-    int graph_force_tpb = 0;
+    int graph_force_tpb[2] = {0, 0};
+    // diagnostic passes on the last step of a call (candidate pairs of the reference's cp_list,
+    // candidates examined by the broad phase): off unless dem_set_diagnostics turns them on
+    bool diagnostics = false;
+    bool diag_valid = false;  // the last counted step ran them
     bool use_graph = true;
     // timing: an event pair around every launch, collected when the pool fills or on request
     bool timing = false;
@@ -243,7 +247,9 @@ struct dem_scene {
 namespace dem {
 // kernel launchers (kernels.hip)
 enum KernelId { K_HASH = 0, K_SCAN, K_SCATTER, K_BROAD, K_MERGE, K_FORCE, K_INIT, K_OTHER };
-hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates);
+// last: the last step of a call, which counts the per-step counters (and runs the diagnostic
+// passes when the scene has them on)
+hipError_t run_contact_detection(dem_scene* s, bool init, bool last);
 hipError_t run_force_integrate(dem_scene* s, bool write_forces);
 hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc, uint32_t* nbr);
 hipError_t run_debug_sort(dem_scene* s);

@@ -1643,9 +1643,11 @@ static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) 
 }
 
 template <bool SEG>
-static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
-    // per-step diagnostic counters are only read after the last step of a call
-    const int diag = count_candidates ? 1 : 0;
+static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
+    // per-step counters are only read after the last step of a call; the diagnostic passes (the
+    // reference's cp_list length, candidates examined) run there only when asked for
+    const int diag = last ? 1 : 0;
+    const bool count_candidates = last && s->diagnostics;
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
@@ -1726,9 +1728,9 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool count_candidates) {
     return hipSuccess;
 }
 
-hipError_t run_contact_detection(dem_scene* s, bool init, bool count_candidates) {
-    return s->batched ? detect_impl<true>(s, init, count_candidates)
-                      : detect_impl<false>(s, init, count_candidates);
+hipError_t run_contact_detection(dem_scene* s, bool init, bool last) {
+    if (last) s->diag_valid = s->diagnostics;
+    return s->batched ? detect_impl<true>(s, init, last) : detect_impl<false>(s, init, last);
 }
 
 template <bool SEG, int BS>

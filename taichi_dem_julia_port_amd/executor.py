@@ -105,6 +105,12 @@ class Scene:
         self._check(self._lib.dem_get_derived(self._h, ctypes.byref(d)))
         return {f: getattr(d, f) for f, _ in N.DemDerived._fields_}
 
+    def set_diagnostics(self, on: bool) -> None:
+        """Count the reference's cp_list length (executor.jl:62-63) and the broad phase's
+        examined candidates on the last step of every call (off by default; costs extra kernels
+        per call).  Without them ``counters()`` reports both as -1."""
+        self._check(self._lib.dem_set_diagnostics(self._h, 1 if on else 0))
+
     def counters(self) -> dict:
         c = N.DemCounters()
         self._check(self._lib.dem_get_counters(self._h, ctypes.byref(c)))

@@ -22,6 +22,7 @@ def test_million_grain_steps_match_the_oracle(name):
     print(f"[case] {name} n={len(g)}", flush=True)
     o = OracleScene(cfg, g, w, parallel=True)
     with Scene(cfg, g, w) as s:
+        s.set_diagnostics(True)
         for step in (1, 2):
             o.step(1)
             s.step(1)

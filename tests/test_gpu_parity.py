@@ -232,9 +232,14 @@ def test_candidate_pair_counter(name, variant):
     cfg, g, w = _build(name)
     o = OracleScene(cfg, g, w)
     with Scene(cfg, g, w, **VARIANTS[variant]) as s:
+        assert s.counters()["candidate_pairs"] == -1  # diagnostics are off by default
+        s.set_diagnostics(True)
         s.step(1)
         o.step(1)
         assert s.counters()["candidate_pairs"] == o.counters()["total_cps"]
+        s.set_diagnostics(False)
+        s.step(1)
+        assert s.counters()["candidate_pairs"] == -1 and s.counters()["examined"] == -1
 
 
 @pytest.mark.parametrize("name", ["lattice_10x10x9", "cfg2_cube911"])
