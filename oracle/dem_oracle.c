@@ -12,10 +12,14 @@
  * same 64-bit pair index (i-1)*n + j with identical get/set/clear semantics, so the oracle also
  * runs at N = 1e6 where the BitTables would need 128 GiB each.
  *
- * Parity pins (tests/test_oracle_*.py): Morton known answer (README.md:23) and brute-force
- * interleave; closed forms for cell/center/neighbors and the Int32-wrapped hash-table size of
- * cfg/1-3; counting-sort invariants; O(N^2) brute-force broad phase; t=0 bond counts of the
- * reference's own particle files; Hertz / EBPM / wall closed forms; free flight; momentum.
+ * Parity pins (tests/test_oracle_*.py, tests/oracle_pins.py): Morton known answer (README.md:23)
+ * and brute-force interleave; closed forms for cell/center/neighbors and the Int32-wrapped
+ * hash-table size of cfg/1-3; counting-sort invariants; O(N^2) brute-force broad phase; t=0 bond
+ * counts of the reference's own particle files; Hertz static force, Coulomb cap, restitution;
+ * the EBPM stiffness matrix against a Timoshenko beam element (symmetry, rigid-body modes,
+ * entries) and its break thresholds against textbook stresses; frame equivariance; force and
+ * moment balance; spin-up lever arms and d omega = tau / I; free flight with and without global
+ * damping; the quaternion recursion; momentum.
  * Parity unpinned (no reference run is possible here): bit-level rounding of StaticArrays
  * norm/inv, Quaternionic to_rotation_matrix/normalize, Base cbrt/log, and Float64^4.
  *
@@ -25,6 +29,12 @@
  * atomics), and update_contacts decides in parallel but applies its set changes in pair order,
  * which keeps the serial outcome.  Without -fopenmp the pragmas are ignored and the code is the
  * serial restatement.
+ *
+ * Planted mistakes: built with -DORC_MUTATION=k (k >= 1), the oracle carries one deliberate
+ * error of the kind a restatement can plausibly make (a swapped stiffness, a flipped sign, a
+ * dropped factor, a transposed rotation; the list is MUT_* below).  tests/test_oracle_mutations.py
+ * builds each such variant and checks that the physics pins reject it, which shows the pins can
+ * see that mistake.  The default build (ORC_MUTATION = 0) has none of them.
  */
 #include "dem_oracle.h"
 
@@ -44,6 +54,26 @@ int32_t orc_num_threads(void) {
 }
 
 typedef struct { double x, y, z; } v3;
+
+#ifndef ORC_MUTATION
+#define ORC_MUTATION 0
+#endif
+enum {
+    MUT_K5K6_SWAP = 1,       /* bond moment on i takes k6 theta_i + k5 theta_j */
+    MUT_K3_SIGN = 2,         /* bond shear force with -k3 (theta_i + theta_j)_z */
+    MUT_GAMMA_NO_SQRT56 = 3, /* Hertz normal damping without sqrt(5/6) */
+    MUT_MOMENT_ORDER = 4,    /* Hertz contact moment F x r instead of r x F */
+    MUT_TORSION_SHEAR = 5,   /* bond torsional shear stress r/I instead of r/(2I) */
+    MUT_DAMPING_SIGN = 6,    /* global damping adds +d|F|sign(v) */
+    MUT_BOND_SCATTER_R = 7,  /* bond forces scattered with R instead of R^-1 */
+    MUT_CONTACT_POINT = 8,   /* Hertz contact point at r_i + delta from i */
+    MUT_VC_SIGN = 9,         /* Hertz contact velocity v_i - v_j */
+    MUT_SIGMA_C_FI = 10,     /* bond normal stress from F_i instead of F_j */
+    MUT_QUAT_DT = 11,        /* quaternion increment scaled by dt */
+    MUT_K4_NO_NU = 12,       /* bond torsion stiffness E I / L */
+    MUT_UPDATE_NEW_V = 13    /* position advanced with the new velocity */
+};
+#define MUT(k) (ORC_MUTATION == (k))
 
 static inline v3 mk(double x, double y, double z) { v3 r = {x, y, z}; return r; }
 static inline v3 vadd(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -301,6 +331,10 @@ struct orc_scene {
     orc_contact* wall_contacts; /* [n][nwall] */
     double* forces;   /* [3n] */
     double* moments;  /* [3n] */
+    /* sums of |contribution| per component of forces / moments (orc_set_track_scale) */
+    int track_scale;
+    double* fscale;   /* [3n] */
+    double* mscale;   /* [3n] */
     orc_counters cnt;
 };
 
@@ -379,7 +413,7 @@ void orc_destroy(orc_scene* s) {
     free(s->g); free(s->walls); free(s->kid); free(s->hid); free(s->kcenter); free(s->nbr);
     free(s->ht); free(s->htc); free(s->pid); free(s->cpr); free(s->cprc); free(s->cpl);
     free(s->contacts); free(s->ctemp); ps_free(&s->st); free(s->wall_contacts);
-    free(s->forces); free(s->moments); free(s);
+    free(s->forces); free(s->moments); free(s->fscale); free(s->mscale); free(s);
 }
 
 void orc_get_derived(const orc_scene* s, orc_derived* out) { *out = s->der; }
@@ -565,6 +599,15 @@ static void atomic_add_vec3(double* arr, int64_t i0, v3 v) { /* kernels.jl:1-5 *
 static void add_vec3(double* arr, int64_t i0, v3 v) {
     arr[i0] += v.x; arr[i0 + 1] += v.y; arr[i0 + 2] += v.z;
 }
+/* |v| per component into a scale array (sc may be NULL: not tracked) */
+static void atomic_add_abs3(double* sc, int64_t i0, v3 v) {
+    if (!sc) return;
+    atomic_add_vec3(sc, i0, mk(fabs(v.x), fabs(v.y), fabs(v.z)));
+}
+static void add_abs3(double* sc, int64_t i0, v3 v) {
+    if (!sc) return;
+    add_vec3(sc, i0, mk(fabs(v.x), fabs(v.y), fabs(v.z)));
+}
 
 /* Hertz-Mindlin tangential/normal law shared by particle and wall contacts
  * (kernels.jl:261-303 and 369-406); returns local force, updates s. */
@@ -579,7 +622,8 @@ static v3 hertz_mindlin(double gap, double dn, v3 vc, double nui, double Ei, dou
     double Sn = 2.0 * Ys * srd;
     double St = 8.0 * Gs * srd;
     double kn = 4.0 / 3.0 * Ys * srd;
-    double gn = -2.0 * beta * sqrt(5.0 / 6.0 * Sn * ms);
+    double gn = MUT(MUT_GAMMA_NO_SQRT56) ? -2.0 * beta * sqrt(Sn * ms)
+                                         : -2.0 * beta * sqrt(5.0 / 6.0 * Sn * ms);
     if (gn < 0) *fatal += 1;
     double kt = 8.0 * Gs * srd;
     double gt = -2.0 * beta * sqrt(5.0 / 6.0 * St * ms);
@@ -608,6 +652,8 @@ static void resolve_collision(orc_scene* s) {
     const double dt = s->cfg.dt, tol = s->cfg.tolerance;
     int64_t total = s->cptr[0];
     int64_t nbroken = 0, nfatal = 0;
+    double* fsc = s->track_scale ? s->fscale : NULL;
+    double* msc = s->track_scale ? s->mscale : NULL;
 #pragma omp parallel for schedule(dynamic, 256) reduction(+ : nbroken, nfatal)
     for (int64_t idx = 0; idx < total; ++idx) {
         orc_contact* c = &s->contacts[idx];
@@ -640,15 +686,19 @@ static void resolve_collision(orc_scene* s) {
             double k1 = Ej * Aj / Lj;
             double k2 = 12.0 * Ej * Ij / (Lj * Lj * Lj) / (1.0 + phi);
             double k3 = 6.0 * Ej * Ij / (Lj * Lj) / (1.0 + phi);
-            double k4 = Ej * Ij / Lj / (1.0 + nu);
+            double k4 = MUT(MUT_K4_NO_NU) ? Ej * Ij / Lj : Ej * Ij / Lj / (1.0 + nu);
             double k5 = Ej * Ij * (4.0 + phi) / Lj / (1.0 + phi);
             double k6 = Ej * Ij * (2.0 - phi) / Lj / (1.0 + phi);
             v3 dF = mk(k1 * (di.x - dj.x),
-                       k2 * (di.y - dj.y) + k3 * (thi.z + thj.z),
+                       MUT(MUT_K3_SIGN) ? k2 * (di.y - dj.y) - k3 * (thi.z + thj.z)
+                                        : k2 * (di.y - dj.y) + k3 * (thi.z + thj.z),
                        k2 * (di.z - dj.z) - k3 * (thi.y + thj.y));
-            v3 dTi = mk(k4 * (thi.x - thj.x),
-                        k3 * (dj.z - di.z) + k5 * thi.y + k6 * thj.y,
-                        k3 * (di.y - dj.y) + k5 * thi.z + k6 * thj.z);
+            v3 dTi = MUT(MUT_K5K6_SWAP)
+                         ? mk(k4 * (thi.x - thj.x), k3 * (dj.z - di.z) + k6 * thi.y + k5 * thj.y,
+                              k3 * (di.y - dj.y) + k6 * thi.z + k5 * thj.z)
+                         : mk(k4 * (thi.x - thj.x),
+                              k3 * (dj.z - di.z) + k5 * thi.y + k6 * thj.y,
+                              k3 * (di.y - dj.y) + k5 * thi.z + k6 * thj.z);
             v3 dTj = mk(k4 * (thj.x - thi.x),
                         k3 * (dj.z - di.z) + k6 * thi.y + k5 * thj.y,
                         k3 * (di.y - dj.y) + k6 * thi.z + k5 * thj.z);
@@ -656,22 +706,30 @@ static void resolve_collision(orc_scene* s) {
             v3 Fj = vneg(Fi);
             v3 Ti = vadd(ld3(c->ti), dTi);
             v3 Tj = vadd(ld3(c->tj), dTj);
-            double sci = Fj.x / Aj - rj / Ij * sqrt(Ti.y * Ti.y + Ti.z * Ti.z);
-            double scj = Fj.x / Aj - rj / Ij * sqrt(Tj.y * Tj.y + Tj.z * Tj.z);
+            double Fn = MUT(MUT_SIGMA_C_FI) ? Fi.x : Fj.x;
+            double sci = Fn / Aj - rj / Ij * sqrt(Ti.y * Ti.y + Ti.z * Ti.z);
+            double scj = Fn / Aj - rj / Ij * sqrt(Tj.y * Tj.y + Tj.z * Tj.z);
             double sc = -(sci < scj ? sci : scj);
             double st = sci > scj ? sci : scj;
-            double ss = fabs(Ti.x) * rj / 2.0 / Ij + 4.0 / 3.0 / Aj * sqrt(Fi.y * Fi.y + Fi.z * Fi.z);
+            double ss = (MUT(MUT_TORSION_SHEAR) ? fabs(Ti.x) * rj / Ij : fabs(Ti.x) * rj / 2.0 / Ij) +
+                        4.0 / 3.0 / Aj * sqrt(Fi.y * Fi.y + Fi.z * Fi.z);
             if (sc >= sf->sc || st >= sf->st || ss >= sf->ss) {
                 /* flags of distinct pairs are distinct bytes; the set is not resized here */
                 clear_bit(&s->st, ij, F_ACTIVE);
                 clear_bit(&s->st, ij, F_BONDED);
                 nbroken += 1;
             } else {
-                m33 Ri = minv(&R);
-                atomic_add_vec3(s->forces, 3 * (i - 1), mv(&Ri, vneg(Fi)));
-                atomic_add_vec3(s->forces, 3 * (j - 1), mv(&Ri, vneg(Fj)));
-                atomic_add_vec3(s->moments, 3 * (i - 1), mv(&Ri, vneg(Ti)));
-                atomic_add_vec3(s->moments, 3 * (j - 1), mv(&Ri, vneg(Tj)));
+                m33 Ri = MUT(MUT_BOND_SCATTER_R) ? R : minv(&R);
+                v3 fi = mv(&Ri, vneg(Fi)), fj = mv(&Ri, vneg(Fj));
+                v3 ti = mv(&Ri, vneg(Ti)), tj = mv(&Ri, vneg(Tj));
+                atomic_add_vec3(s->forces, 3 * (i - 1), fi);
+                atomic_add_vec3(s->forces, 3 * (j - 1), fj);
+                atomic_add_vec3(s->moments, 3 * (i - 1), ti);
+                atomic_add_vec3(s->moments, 3 * (j - 1), tj);
+                atomic_add_abs3(fsc, 3 * (i - 1), fi);
+                atomic_add_abs3(fsc, 3 * (j - 1), fj);
+                atomic_add_abs3(msc, 3 * (i - 1), ti);
+                atomic_add_abs3(msc, 3 * (j - 1), tj);
                 st3(c->k, k); st3(c->F, Fi); st3(c->ti, Ti); st3(c->tj, Tj);
                 st3(c->s, mk(0.0, 0.0, 0.0));
             }
@@ -680,11 +738,11 @@ static void resolve_collision(orc_scene* s) {
             double gap = Li - gi->r - gj->r;
             if (gap > 0) nfatal += 1;
             double dn = fabs(gap);
-            v3 k = vadd(ki, vscl(vnormalize(vsub(kj, ki)), gi->r - dn));
+            v3 k = vadd(ki, vscl(vnormalize(vsub(kj, ki)), MUT(MUT_CONTACT_POINT) ? gi->r + dn : gi->r - dn));
             v3 kip = vsub(k, ki), kjp = vsub(k, kj);
             v3 vci = vadd(vcross(ld3(gi->w), kip), ld3(gi->v));
             v3 vcj = vadd(vcross(ld3(gj->w), kjp), ld3(gj->v));
-            v3 vc = mv(&R, vsub(vcj, vci));
+            v3 vc = MUT(MUT_VC_SIGN) ? mv(&R, vsub(vci, vcj)) : mv(&R, vsub(vcj, vci));
             int32_t midi = gi->mid, midj = gj->mid;
             const surface* sf = &s->surf[midi - 1][midj - 1];
             double Rs = 1.0 / (1.0 / gi->r + 1.0 / gj->r);
@@ -695,10 +753,16 @@ static void resolve_collision(orc_scene* s) {
                                  sf->mu, dt, &sv, &nfatal);
             m33 Ri = minv(&R);
             v3 Fg = mv(&Ri, vneg(F));
+            v3 mi = MUT(MUT_MOMENT_ORDER) ? vcross(Fg, kip) : vcross(kip, Fg);
+            v3 mj = MUT(MUT_MOMENT_ORDER) ? vcross(vneg(Fg), kjp) : vcross(kjp, vneg(Fg));
             atomic_add_vec3(s->forces, 3 * (i - 1), Fg);
             atomic_add_vec3(s->forces, 3 * (j - 1), vneg(Fg));
-            atomic_add_vec3(s->moments, 3 * (i - 1), vcross(kip, Fg));
-            atomic_add_vec3(s->moments, 3 * (j - 1), vcross(kjp, vneg(Fg)));
+            atomic_add_vec3(s->moments, 3 * (i - 1), mi);
+            atomic_add_vec3(s->moments, 3 * (j - 1), mj);
+            atomic_add_abs3(fsc, 3 * (i - 1), Fg);
+            atomic_add_abs3(fsc, 3 * (j - 1), Fg);
+            atomic_add_abs3(msc, 3 * (i - 1), mi);
+            atomic_add_abs3(msc, 3 * (j - 1), mj);
             st3(c->k, k); st3(c->F, F); st3(c->s, sv);
         }
     }
@@ -739,6 +803,10 @@ static void resolve_wall(orc_scene* s) {
                 v3 Fg = mv(&Ri, vneg(F));
                 add_vec3(s->forces, 3 * i, Fg);
                 add_vec3(s->moments, 3 * i, vcross(kip, Fg));
+                if (s->track_scale) {
+                    add_abs3(s->fscale, 3 * i, Fg);
+                    add_abs3(s->mscale, 3 * i, vcross(kip, Fg));
+                }
                 c->i = (int32_t)(i + 1); c->j = 0; c->midi = midi; c->midj = midj;
                 st3(c->k, k); st3(c->F, F);
                 st3(c->ti, mk(0, 0, 0)); st3(c->tj, mk(0, 0, 0)); st3(c->s, sv);
@@ -760,15 +828,21 @@ static void apply_body_force(orc_scene* s) {
 #pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < s->n; ++i) {
         const orc_grain* g = &s->g[i];
+        const double nd = MUT(MUT_DAMPING_SIGN) ? dmp : -dmp;
         add_vec3(s->forces, 3 * i, vscl(ld3(gr), g->m));
         v3 F = ld3(&s->forces[3 * i]);
         v3 T = ld3(&s->moments[3 * i]);
-        v3 dF = mk(-dmp * fabs(F.x) * jsign(g->v[0]), -dmp * fabs(F.y) * jsign(g->v[1]),
-                   -dmp * fabs(F.z) * jsign(g->v[2]));
-        v3 dT = mk(-dmp * fabs(T.x) * jsign(g->w[0]), -dmp * fabs(T.y) * jsign(g->w[1]),
-                   -dmp * fabs(T.z) * jsign(g->w[2]));
+        v3 dF = mk(nd * fabs(F.x) * jsign(g->v[0]), nd * fabs(F.y) * jsign(g->v[1]),
+                   nd * fabs(F.z) * jsign(g->v[2]));
+        v3 dT = mk(nd * fabs(T.x) * jsign(g->w[0]), nd * fabs(T.y) * jsign(g->w[1]),
+                   nd * fabs(T.z) * jsign(g->w[2]));
         add_vec3(s->forces, 3 * i, dF);
         add_vec3(s->moments, 3 * i, dT);
+        if (s->track_scale) {
+            add_abs3(s->fscale, 3 * i, vscl(ld3(gr), g->m));
+            add_abs3(s->fscale, 3 * i, dF);
+            add_abs3(s->mscale, 3 * i, dT);
+        }
     }
 }
 
@@ -797,8 +871,9 @@ static void update(orc_scene* s) {
         orc_grain* g = &s->g[i];
         v3 F = ld3(&s->forces[3 * i]);
         v3 a = vdiv(F, g->m);
-        v3 k = vadd(vadd(ld3(g->k), vscl(ld3(g->v), dt)), vscl(sclv(0.5, a), dt * dt));
         v3 v = vadd(ld3(g->v), vscl(a, dt));
+        v3 k = vadd(vadd(ld3(g->k), vscl(MUT(MUT_UPDATE_NEW_V) ? v : ld3(g->v), dt)),
+                    vscl(sclv(0.5, a), dt * dt));
         double q[4] = {g->q[0], g->q[1], g->q[2], g->q[3]};
         v3 w = ld3(g->w);
         m33 R = quat_matrix(q);
@@ -816,7 +891,8 @@ static void update(orc_scene* s) {
         double dq1 = 0.5 * (q[0] * w.x + q[2] * w.z - q[3] * w.y);
         double dq2 = 0.5 * (q[0] * w.y + q[1] * w.z + q[3] * w.x);
         double dq3 = 0.5 * (q[0] * w.z + q[1] * w.y - q[2] * w.x);
-        double nq[4] = {q[0] + dq0, q[1] + dq1, q[2] + dq2, q[3] + dq3};
+        const double qs = MUT(MUT_QUAT_DT) ? dt : 1.0;
+        double nq[4] = {q[0] + qs * dq0, q[1] + qs * dq1, q[2] + qs * dq2, q[3] + qs * dq3};
         double qn = sqrt(nq[0] * nq[0] + nq[1] * nq[1] + nq[2] * nq[2] + nq[3] * nq[3]);
         for (int c = 0; c < 4; ++c) g->q[c] = nq[c] / qn;
         w = vadd(w, vscl(dw, dt));
@@ -872,6 +948,10 @@ int32_t orc_step(orc_scene* s, int64_t nsteps) { /* executor.jl:162-172 */
     for (int64_t t = 0; t < nsteps; ++t) {
         memset(s->forces, 0, 3 * s->n * sizeof(double)); /* clear_state! */
         memset(s->moments, 0, 3 * s->n * sizeof(double));
+        if (s->track_scale) {
+            memset(s->fscale, 0, 3 * s->n * sizeof(double));
+            memset(s->mscale, 0, 3 * s->n * sizeof(double));
+        }
         contact_pass(s);
         update_contacts(s);
         uint32_t total = s->cptr[0], inactive = s->cptr[1];
@@ -954,6 +1034,24 @@ void orc_set_wall_shear(orc_scene* s, const double* in) {
 void orc_get_forces(const orc_scene* s, double* F, double* T) {
     memcpy(F, s->forces, 3 * s->n * sizeof(double));
     memcpy(T, s->moments, 3 * s->n * sizeof(double));
+}
+
+void orc_set_track_scale(orc_scene* s, int32_t on) {
+    if (on && !s->fscale) {
+        s->fscale = (double*)calloc(3 * s->n, sizeof(double));
+        s->mscale = (double*)calloc(3 * s->n, sizeof(double));
+    }
+    s->track_scale = on != 0;
+}
+
+void orc_get_force_scale(const orc_scene* s, double* F, double* T) {
+    if (!s->fscale) {
+        memset(F, 0, 3 * s->n * sizeof(double));
+        memset(T, 0, 3 * s->n * sizeof(double));
+        return;
+    }
+    memcpy(F, s->fscale, 3 * s->n * sizeof(double));
+    memcpy(T, s->mscale, 3 * s->n * sizeof(double));
 }
 
 void orc_get_cells(const orc_scene* s, int32_t* kid, uint32_t* hid, double* kcenter,

@@ -129,6 +129,10 @@ int32_t orc_set_contacts(orc_scene* s, const orc_contact* in, const uint8_t* bon
 void orc_get_wall_shear(const orc_scene* s, double* out);      /* [n][nwall][3] */
 void orc_set_wall_shear(orc_scene* s, const double* in);
 void orc_get_forces(const orc_scene* s, double* F, double* T); /* [n][3] of the last step */
+/* Sums of |contribution| per component of the last step's forces and moments ([n][3]): the scale
+ * against which a per-particle sum taken in another order is compared.  Tracked once turned on. */
+void orc_set_track_scale(orc_scene* s, int32_t on);
+void orc_get_force_scale(const orc_scene* s, double* F, double* T);
 
 /* intermediate arrays of the last contact pass */
 void orc_get_cells(const orc_scene* s, int32_t* kid, uint32_t* hid, double* kcenter,

@@ -52,6 +52,22 @@ def load(parallel: bool = False) -> ctypes.CDLL:
     path = LIB_OMP_PATH if parallel else LIB_PATH
     if not os.path.exists(path):
         build()
+    _LIBS[parallel] = load_path(path)
+    return _LIBS[parallel]
+
+
+def build_mutant(k: int, out_dir: str) -> str:
+    """The serial oracle built with planted mistake k (-DORC_MUTATION=k; see dem_oracle.c) into
+    out_dir; returns the library path."""
+    path = os.path.join(out_dir, f"liboracle_mut{k}.so")
+    subprocess.run(["gcc", "-O2", "-std=gnu11", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+                    "-Wno-unknown-pragmas", f"-DORC_MUTATION={int(k)}", "-shared", "-o", path,
+                    os.path.join(HERE, "dem_oracle.c"), "-lm"], check=True)
+    return path
+
+
+def load_path(path: str) -> ctypes.CDLL:
+    """An oracle build at path, with every function's signature declared."""
     lib = ctypes.CDLL(path)
     P = ctypes.POINTER
     vp = ctypes.c_void_p
@@ -80,6 +96,8 @@ def load(parallel: bool = False) -> ctypes.CDLL:
         "orc_get_wall_shear": (None, [vp, P(ctypes.c_double)]),
         "orc_set_wall_shear": (None, [vp, P(ctypes.c_double)]),
         "orc_get_forces": (None, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
+        "orc_set_track_scale": (None, [vp, ctypes.c_int32]),
+        "orc_get_force_scale": (None, [vp, P(ctypes.c_double), P(ctypes.c_double)]),
         "orc_get_cells": (None, [vp, P(ctypes.c_int32), P(ctypes.c_uint32), P(ctypes.c_double),
                                  P(ctypes.c_uint32)]),
         "orc_get_hash_table": (None, [vp, P(ctypes.c_uint32), P(ctypes.c_uint32), P(ctypes.c_uint32)]),
@@ -89,7 +107,6 @@ def load(parallel: bool = False) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    _LIBS[parallel] = lib
     return lib
 
 
@@ -115,8 +132,8 @@ class OracleScene:
     """Serial FP64 CPU restatement of the reference step (same method names as Scene).
     ``parallel=True`` runs the OpenMP build of the same code."""
 
-    def __init__(self, cfg, grains, walls=None, parallel: bool = False):
-        self._lib = load(parallel)
+    def __init__(self, cfg, grains, walls=None, parallel: bool = False, lib=None):
+        self._lib = lib if lib is not None else load(parallel)
         self.cfg = cfg
         g = np.ascontiguousarray(grains, dtype=GRAIN_IN_DTYPE)
         self.n = len(g)
@@ -201,6 +218,17 @@ class OracleScene:
         F = np.zeros((self.n, 3))
         T = np.zeros((self.n, 3))
         self._lib.orc_get_forces(self._h, _p(F, ctypes.c_double), _p(T, ctypes.c_double))
+        return F, T
+
+    def track_force_scale(self, on: bool = True):
+        """Keep sums of |contribution| per force and moment component from the next step on."""
+        self._lib.orc_set_track_scale(self._h, 1 if on else 0)
+
+    def read_force_scale(self):
+        """Sums of |contribution| per component of the last step's forces and moments."""
+        F = np.zeros((self.n, 3))
+        T = np.zeros((self.n, 3))
+        self._lib.orc_get_force_scale(self._h, _p(F, ctypes.c_double), _p(T, ctypes.c_double))
         return F, T
 
     # intermediate arrays of the most recent contact pass (call contact_pass() first)
