@@ -223,6 +223,7 @@ class OracleScene:
     def track_force_scale(self, on: bool = True):
         """Keep sums of |contribution| per force and moment component from the next step on."""
         self._lib.orc_set_track_scale(self._h, 1 if on else 0)
+        self.tracking_force_scale = bool(on)
 
     def read_force_scale(self):
         """Sums of |contribution| per component of the last step's forces and moments."""

@@ -8,7 +8,8 @@ difference is the order in which contact forces are summed).
 import numpy as np
 import pytest
 
-from helpers import packed_sample, pair_keys, pour_sample, reference_case, vec_rel_err
+from helpers import (compare_state, packed_sample, pair_keys, pour_sample, reference_case,
+                     reseed_oracle)
 from oracle.oracle import OracleScene
 from taichi_dem_julia_port_amd import scenes
 
@@ -159,21 +160,7 @@ def test_initialize_bond_set(name, variant):
 
 
 def _compare_state(s, o, rel=FP64_REL):
-    c, b = s.read_contacts()
-    co, bo = o.read_contacts()
-    assert np.array_equal(pair_keys(c), pair_keys(co)), "contact set differs"
-    assert np.array_equal(b, bo), "bonded flags differ"
-    for f in ("k", "F", "ti", "tj", "s"):
-        assert vec_rel_err(c[f], co[f]) <= rel, f"contact field {f}"
-    gs, go = s.read_grains(), o.read_grains()
-    for f in ("k", "v"):
-        assert vec_rel_err(gs[f], go[f]) <= rel, f"grain field {f}"
-    # sums of contributions may cancel: compare angular quantities against a scale
-    for f in ("w",):
-        scale = max(np.abs(go[f]).max(), 1e-300)
-        assert np.abs(gs[f] - go[f]).max() <= 1e-10 * scale + 1e-300, f"grain field {f}"
-    ws, wo = s.read_wall_shear(), o.read_wall_shear()
-    assert np.abs(ws - wo).max() <= 1e-10 * max(np.abs(wo).max(), 1e-30)
+    compare_state(s, o, rel)
 
 
 ONE_STEP_CASES = CASES + ["cfg2_nobond"]
@@ -192,6 +179,7 @@ def test_one_step_parity(name, variant):
     cfg, g, w = _build("cfg2_cube911" if name == "cfg2_nobond" else name)
     bonded = name not in UNBONDED
     o = OracleScene(cfg, g, w)
+    o.track_force_scale()
     with Scene(cfg, g, w, **VARIANTS[variant]) as s:
         if bonded:
             o.initialize()
@@ -214,11 +202,7 @@ def test_stepwise_parity_from_identical_states(name, variant):
             s.initialize()
         for _ in range(3):
             s.step(20)
-            o = OracleScene(cfg, g, w)
-            o.write_grains(s.read_grains())
-            c, b = s.read_contacts()
-            o.write_contacts(c, b)
-            o.write_wall_shear(s.read_wall_shear())
+            o = reseed_oracle(s, cfg, g, w)
             s.step(1)
             o.step(1)
             _compare_state(s, o)
@@ -262,10 +246,7 @@ def test_contact_restore_round_trip(name):
         assert np.array_equal(b3, b)
         for f in ("i", "j", "k", "F", "ti", "tj", "s"):
             assert np.array_equal(c3[f], c[f]), f
-        o = OracleScene(cfg, g, w)
-        o.write_grains(s.read_grains())
-        o.write_contacts(c, b)
-        o.write_wall_shear(s.read_wall_shear())
+        o = reseed_oracle(s, cfg, g, w)
         s.step(1)
         o.step(1)
         _compare_state(s, o)
