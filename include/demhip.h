@@ -31,7 +31,7 @@ enum dem_status {
     DEM_OK = 0,
     DEM_ERR_INVALID = -1,   /* bad argument or configuration */
     DEM_ERR_HIP = -2,       /* HIP runtime error (message names the call) */
-    DEM_ERR_CAPACITY = -3,  /* per-particle contact capacity exceeded */
+    DEM_ERR_CAPACITY = -3,  /* a particle needs more than 256 contact slots */
     DEM_ERR_DUPLICATE = -4, /* duplicate contact pair in supplied state */
     DEM_ERR_NONFINITE = -5, /* non-finite particle state */
     DEM_ERR_STATE = -6      /* call not valid in the scene's current state */
@@ -172,7 +172,11 @@ const char* dem_last_error(const dem_scene_t* s); /* s may be NULL: last create 
 
 /* ---- the hot path -------------------------------------------------------------------- */
 int32_t dem_initialize(dem_scene_t* s);              /* initialize!  executor.jl:155-160 */
-int32_t dem_step(dem_scene_t* s, int64_t nsteps);    /* nsteps x simulate!  executor.jl:162-172 */
+/* nsteps x simulate! (executor.jl:162-172).  A step in which a particle needs more contact slots
+ * than the scene has changes nothing; the rows are widened (their contents kept, up to 256 slots)
+ * and the remaining steps run, with results equal to those of a scene created with the wider
+ * rows.  Beyond 256 slots it returns DEM_ERR_CAPACITY with the state of the last completed step. */
+int32_t dem_step(dem_scene_t* s, int64_t nsteps);
 
 /* ---- read-back / state transfer --------------------------------------------------------- */
 int32_t dem_get_derived(const dem_scene_t* s, dem_derived_t* out);

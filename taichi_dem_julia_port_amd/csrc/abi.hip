@@ -226,25 +226,50 @@ namespace {
 
 int32_t collect_timing(dem_scene* s) { return s->timing ? timing_flush(s) : DEM_OK; }
 
-// Reallocate empty contact rows with k slots per particle (before the first step only).
-int32_t resize_rows(dem_scene* s, int32_t k) {
+// Widen the contact rows to k2 slots per particle, keeping every committed row and its state:
+// slot m of particle p stays at m * n + p in the partner words, and each state component moves
+// to its new stride.  On an allocation failure the scene keeps its rows.
+int32_t widen_rows(dem_scene* s, int32_t k2) {
     DevState& d = s->d;
+    const int64_t n = s->n, K = s->K;
     HIPCHK(s, hipStreamSynchronize(s->stream));
-    block_free(s->device, d.rows_arena, d.rows_arena_bytes);
-    d.rows_arena = nullptr;
-    d.touch = nullptr; d.pq[0] = d.pq[1] = nullptr; d.st[0] = d.st[1] = nullptr;
-    HIPCHK(s, alloc_rows(s, k));
-    for (int b = 0; b < 2; ++b) HIPCHK(s, hipMemset(d.rcnt[b], 0, s->n * sizeof(uint32_t)));
-    HIPCHK(s, hipMemset(d.sel, 0, s->n));
-    HIPCHK(s, hipMemset(d.tcnt, 0, s->n * sizeof(uint32_t)));
-    HIPCHK(s, hipMemset(d.ctr, 0, CTR_WORDS * sizeof(unsigned long long)));
-    // the fills run on the null stream, which the scene's non-blocking stream does not wait for
-    HIPCHK(s, hipDeviceSynchronize());
-    s->K = k;
-    s->P.K = k;
-    s->der.contact_slots = k;
+    uint32_t* o_touch = d.touch;
+    uint32_t* o_pq[2] = {d.pq[0], d.pq[1]};
+    double* o_st[2] = {d.st[0], d.st[1]};
+    void* o_arena = d.rows_arena;
+    const size_t o_bytes = d.rows_arena_bytes;
+    hipError_t e = alloc_rows(s, k2);
+    // the zero fill runs on the null stream, which the scene's non-blocking stream does not wait for
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        if (d.rows_arena && d.rows_arena != o_arena) block_free(s->device, d.rows_arena, d.rows_arena_bytes);
+        d.touch = o_touch;
+        for (int b = 0; b < 2; ++b) { d.pq[b] = o_pq[b]; d.st[b] = o_st[b]; }
+        d.rows_arena = o_arena;
+        d.rows_arena_bytes = o_bytes;
+        return hip_fail(s, "widening the contact rows", e);
+    }
+    for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+        e = hipMemcpyAsync(d.pq[b], o_pq[b], (size_t)K * n * sizeof(uint32_t),
+                           hipMemcpyDeviceToDevice, s->stream);
+        for (int c = 0; c < S_NCOMP && e == hipSuccess; ++c)
+            e = hipMemcpyAsync(d.st[b] + (int64_t)c * k2 * n, o_st[b] + (int64_t)c * K * n,
+                               (size_t)K * n * sizeof(double), hipMemcpyDeviceToDevice, s->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) return hip_fail(s, "copying the contact rows", e);
+    block_free(s->device, o_arena, o_bytes);
+    s->K = k2;
+    s->P.K = k2;
+    s->der.contact_slots = k2;
     drop_graphs(s);
     return DEM_OK;
+}
+
+// The next row width after an overflow that needed `need` slots: at least twice the current one.
+int32_t wider_slots(const dem_scene* s, unsigned long long need) {
+    const int64_t want = std::max<int64_t>(2 * (int64_t)s->K, ((int64_t)need + 7) & ~(int64_t)7);
+    return (int32_t)std::min<int64_t>(want, MAX_SLOTS);
 }
 
 }  // namespace
@@ -584,7 +609,7 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     dem_options_t o;
     if ((rc = options_of(opt, o))) return rc;
     const int32_t K = slots_of(o, cfg->max_coordinate_number);
-    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
+    if (K < 1 || K > MAX_SLOTS) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 256]");
     if (hipSetDevice(o.device) != hipSuccess) return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
 
     dem_scene* s = new dem_scene();
@@ -650,7 +675,7 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
     }
     if (total > (int64_t)IDX_MASK) return fail(nullptr, DEM_ERR_INVALID, "too many particles");
     const int32_t K = slots_of(o, maxcoord);
-    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
+    if (K < 1 || K > MAX_SLOTS) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 256]");
     if (hipSetDevice(o.device) != hipSuccess) return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
 
     dem_scene* s = new dem_scene();
@@ -756,7 +781,8 @@ int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, i
     HIPCHK(dst, hipStreamSynchronize(dst->stream));
     uint32_t* ovf = nullptr;
     HIPCHK(dst, hipMalloc(&ovf, sizeof(uint32_t)));
-    hipError_t e = hipMemset(ovf, 0, sizeof(uint32_t));
+    // ordered on the stream k_adopt runs on (a null-stream fill would not be waited for)
+    hipError_t e = hipMemsetAsync(ovf, 0, sizeof(uint32_t), dst->stream);
     if (e == hipSuccess)
         e = run_adopt(dst, dst->seg_off[dst_scene], src, src->seg_off[src_scene], n, Ss.nwall, ovf);
     uint32_t o = 0;
@@ -836,7 +862,7 @@ int32_t dem_batch_repack(dem_scene_t* src, int32_t nkeep, const int32_t* keep, i
     int32_t K = slots_of(o, maxcoord);
     // rows that dem_initialize widened stay wide: the kept scenes' bonds must fit
     if (o.contact_slots <= 0 && nkeep > 0) K = std::max(K, src->K);
-    if (K < 1 || K > 128) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 128]");
+    if (K < 1 || K > MAX_SLOTS) return fail(nullptr, DEM_ERR_INVALID, "contact_slots must be in [1, 256]");
     if (hipSetDevice(o.device) != hipSuccess) return fail(nullptr, DEM_ERR_HIP, "hipSetDevice failed");
 
     dem_scene* s = new dem_scene();
@@ -1010,11 +1036,17 @@ int32_t dem_initialize(dem_scene_t* s) {
         int32_t rc = sync_counters(s);
         if (rc) return rc;
         if ((rc = collect_timing(s))) return rc;
-        if (!s->h_ctr[CT_OVERFLOW]) break;
-        // the rows are still empty before the first step: widen them and redo the bond pass
-        if (s->K >= 128)
-            return fail(s, DEM_ERR_CAPACITY, "more than 128 bonds on one particle during bond init");
-        if ((rc = resize_rows(s, std::min(128, 2 * s->K)))) return rc;
+        if (!s->h_ctr[CT_FAIL]) {
+            HIPCHK(s, run_rows_finish(s, false));  // commit the rows the bond pass built
+            HIPCHK(s, hipStreamSynchronize(s->stream));
+            break;
+        }
+        // a particle has more bonds than its row holds: drop the pass's rows (rows that existed
+        // before it, such as those of a re-packed batch's kept scenes, stay), widen, redo it
+        HIPCHK(s, run_rows_finish(s, true));
+        if (s->K >= MAX_SLOTS)
+            return fail(s, DEM_ERR_CAPACITY, "more than 256 contacts on one particle during bond init");
+        if ((rc = widen_rows(s, wider_slots(s, s->h_ctr[CT_MAXROW])))) return rc;
     }
     s->initialized = true;
     choose_force_blocks(s);
@@ -1044,11 +1076,10 @@ static int32_t capture_graph(dem_scene* s) {
     return DEM_OK;
 }
 
-int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
-    if (!s || nsteps < 0) return DEM_ERR_INVALID;
-    HIPCHK(s, hipSetDevice(s->device));
+// Enqueue n steps: pairs of steps replayed from the captured graph of the current buffer parity
+// (not while timing individual kernels), the last step eager with its outputs and counters.
+static int32_t enqueue_steps(dem_scene* s, int64_t nsteps) {
     int64_t t = 0;
-    // body: pairs of steps replayed from a captured graph (not while timing individual kernels)
     if (s->use_graph && !s->timing && nsteps >= 3) {
         const int b = s->cur;  // a graph of two steps ends at the parity it starts from
         if (!s->graph_exec[b] || s->graph_force_tpb[b] != s->force_tpb) {
@@ -1065,11 +1096,37 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
         HIPCHK(s, run_force_integrate(s, t == nsteps - 1));
         s->steps++;
     }
-    int32_t rc = sync_counters(s);
-    if (rc) return rc;
-    if (s->h_ctr[CT_OVERFLOW])
-        return fail(s, DEM_ERR_CAPACITY,
-                    "contact slots per particle exceeded (raise contact_slots / max_coordinate_number)");
+    return DEM_OK;
+}
+
+int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
+    if (!s || nsteps < 0) return DEM_ERR_INVALID;
+    HIPCHK(s, hipSetDevice(s->device));
+    int64_t left = nsteps;
+    while (left > 0) {
+        const int64_t steps0 = s->steps;
+        const int cur0 = s->cur;
+        const unsigned long long done0 = s->done_count;
+        int32_t rc = enqueue_steps(s, left);
+        if (rc) return rc;
+        if ((rc = sync_counters(s))) return rc;
+        s->done_count = s->h_ctr[CT_DONE];
+        if (!s->h_ctr[CT_FAIL]) break;
+        // A row overflowed: the steps from the failed one on changed nothing.  Resume from the
+        // state after the steps that completed, with wider rows.
+        const int64_t done = (int64_t)(s->done_count - done0);
+        s->steps = steps0 + done;
+        s->cur = (done & 1) ? 1 - cur0 : cur0;
+        left -= done;
+        HIPCHK(s, run_rows_finish(s, true));
+        if (s->K >= MAX_SLOTS) {
+            HIPCHK(s, hipStreamSynchronize(s->stream));
+            return fail(s, DEM_ERR_CAPACITY,
+                        "more than 256 contacts on one particle (the state is that of the last "
+                        "completed step)");
+        }
+        if ((rc = widen_rows(s, wider_slots(s, s->h_ctr[CT_MAXROW])))) return rc;
+    }
     if (nsteps > 0) choose_force_blocks(s);
     return DEM_OK;
 }
@@ -1257,8 +1314,9 @@ int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_
     if (e == hipSuccess) e = to_device(s, dword, word.data(), ents.size() * sizeof(uint32_t));
     if (e == hipSuccess) e = to_device(s, dstate, state.data(), ents.size() * S_NCOMP * sizeof(double));
     if (e == hipSuccess) e = to_device(s, d.rcnt[0], rc.data(), n * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(d.rcnt[1], 0, n * sizeof(uint32_t));
-    if (e == hipSuccess) e = hipMemset(d.sel, 0, n);
+    // on the scene's stream, ahead of k_write_rows and the next merge
+    if (e == hipSuccess) e = hipMemsetAsync(d.rcnt[1], 0, n * sizeof(uint32_t), s->stream);
+    if (e == hipSuccess) e = hipMemsetAsync(d.sel, 0, n, s->stream);
     if (e == hipSuccess) e = run_write_contacts(s, ddst, dword, dstate, (int64_t)ents.size());
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     hipFree(ddst); hipFree(dword); hipFree(dstate);

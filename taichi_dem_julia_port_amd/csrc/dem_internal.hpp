@@ -12,6 +12,7 @@
 namespace dem {
 
 constexpr int MAX_WALLS = 16;
+constexpr int MAX_SLOTS = 256;  // contact slots per particle the rows may widen to
 constexpr uint32_t IDX_MASK = 0x1fffffffu;  // partner index bits of a row entry
 constexpr uint32_t IS_I = 1u << 29;         // the row's particle is the contact's i (the lower
                                             // file index): the canonical orientation
@@ -36,8 +37,11 @@ enum CounterIdx {
     CT_EXAMINED,     // candidate pairs examined by the engine's broad phase
     CT_NSTEP,        // counters below are reset at the start of every step
     CT_BROKEN_TOTAL = 12,  // cumulative
-    CT_OVERFLOW = 13,      // sticky: a capacity overflow invalidates the scene
-    CT_NCOUNTERS = 16
+    CT_OVERFLOW = 13,      // a row overflowed in the failed step (cleared by the recovery)
+    CT_DONE = 14,          // cumulative: steps whose force kernel ran (copy 0 only)
+    CT_FAIL = 15,          // a row overflowed: every later step of the call does nothing (copy 0)
+    CT_MAXROW = 16,        // the longest row an overflowing particle needed (copy 0, maximum)
+    CT_NCOUNTERS = 20
 };
 
 // Counters are kept in CTR_COPIES copies, 256 B apart, and a block adds to copy blockIdx % CTR_COPIES:
@@ -49,6 +53,24 @@ constexpr int CTR_WORDS = CTR_COPIES * CTR_STRIDE;
 
 __device__ __forceinline__ unsigned long long* ctr_at(unsigned long long* ctr, int k) {
     return ctr + (size_t)(blockIdx.x % CTR_COPIES) * CTR_STRIDE + k;
+}
+
+// A step is a transaction: the merge writes changed rows into the other buffer and marks them
+// pending (ROW_PENDING | committed buffer) in the row selector, and the force kernel commits them.
+// A row that does not fit fails the step: the force kernel of that step and of every later step
+// of the call then changes nothing (the merge only ever writes a particle's uncommitted buffer),
+// and the host drops the pending rows, widens the rows and replays the steps that did not run.
+constexpr uint8_t ROW_PENDING = 2;
+
+// The failure flag as the force kernel reads it: written by an earlier kernel of the stream, so a
+// plain load sees it (a kernel starts with the caches invalidated).
+__device__ __forceinline__ bool step_failed(const unsigned long long* ctr) {
+    return ctr[CT_FAIL] != 0ull;
+}
+__device__ __forceinline__ void fail_step(unsigned long long* ctr, uint32_t need) {
+    atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
+    atomicMax(&ctr[CT_MAXROW], (unsigned long long)need);
+    atomicOr(&ctr[CT_FAIL], 1ull);
 }
 
 struct SurfaceP {
@@ -234,6 +256,8 @@ struct dem_scene {
     // candidates examined by the broad phase): off unless dem_set_diagnostics turns them on
     bool diagnostics = false;
     bool diag_valid = false;  // the last counted step ran them
+    // steps the device has completed (its CT_DONE counter) as of the last counter read
+    unsigned long long done_count = 0;
     bool use_graph = true;
     // timing: an event pair around every launch, collected when the pool fills or on request
     bool timing = false;
@@ -251,6 +275,9 @@ enum KernelId { K_HASH = 0, K_SCAN, K_SCATTER, K_BROAD, K_MERGE, K_FORCE, K_INIT
 // passes when the scene has them on)
 hipError_t run_contact_detection(dem_scene* s, bool init, bool last);
 hipError_t run_force_integrate(dem_scene* s, bool write_forces);
+// after a bond pass or a failed step: commit the pending rows (rollback = false), or drop them,
+// clear the touching counts and the failure flags (rollback = true)
+hipError_t run_rows_finish(dem_scene* s, bool rollback);
 hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc, uint32_t* nbr);
 hipError_t run_debug_sort(dem_scene* s);
 hipError_t run_debug_cp_list(dem_scene* s, int32_t** pairs_dev, int64_t* count);

@@ -232,7 +232,7 @@ k_broad(const Params P, const uint32_t* __restrict__ sidx, const double4* __rest
     }
     wave_add(ctr_at(ctr, CT_CAND), cand);
     wave_add(ctr_at(ctr, CT_TOUCH), found);
-    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
+    if (overflow) fail_step(ctr, (uint32_t)P.K + 1u);
 }
 
 // Slot-owned broad phase.  A touching pair lies inside both particles' windows (the window
@@ -305,7 +305,7 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
         }
     }
     wave_add(ctr_at(ctr, CT_TOUCH), found);
-    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
+    if (overflow) fail_step(ctr, (uint32_t)P.K + 1u);
 }
 
 // Dense-grid broad phase.  Cells have side >= the largest contact distance, so every touching
@@ -535,7 +535,10 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
             for (int u = 0; u < BROAD_BATCH; ++u) {
                 const uint64_t m = __ballot(hit[u]);
                 if (!m) continue;
-                const uint32_t base = hq_n[wv];
+                // the queue count is shared by the wave's lanes: relaxed atomics keep the compiler
+                // from reusing a value another lane has since replaced
+                const uint32_t base =
+                    __hip_atomic_load(&hq_n[wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
                 const uint32_t cnt = (uint32_t)__popcll(m);
                 const uint32_t rank = (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
                 const bool fits = base + cnt <= (uint32_t)HIT_QUEUE;
@@ -547,7 +550,9 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
                         confirm(S, p, rlo + i + (uint32_t)u);
                     }
                 }
-                if (fits && rank == 0 && hit[u]) hq_n[wv] = base + cnt;
+                if (fits && rank == 0 && hit[u])
+                    __hip_atomic_store(&hq_n[wv], base + cnt, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WAVEFRONT);
             }
         };
         // range j without indexing the arrays by a runtime value (which would spill them)
@@ -586,7 +591,8 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     }
     // every lane of the wave confirms queued survivors (the queue holds only live owners)
     {
-        const uint32_t qn = hq_n[wv];
+        const uint32_t qn =
+            __hip_atomic_load(&hq_n[wv], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         for (uint32_t e = (uint32_t)lane; e < qn; e += 64u) {
             const uint32_t k = hq_k[wv][e];
             const int64_t to = t0 + (int64_t)wv * 64 + hq_l[wv][e];
@@ -595,7 +601,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
         }
     }
     if (diag) wave_add(ctr_at(ctr, CT_TOUCH), found);
-    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
+    if (overflow) fail_step(ctr, (uint32_t)P.K + 1u);
 }
 
 // Candidate pairs the dense broad phase examines (its five trimmed slot ranges): a diagnostic,
@@ -772,7 +778,7 @@ __device__ __forceinline__ void merge_rows(const SceneP& S, const double4* __res
         }
     }
     (nb ? rcnt1 : rcnt0)[p] = o;
-    sel[p] = (uint8_t)nb;
+    sel[p] = (uint8_t)(ROW_PENDING | b);  // committed by the force kernel (or the bond pass)
 }
 
 // Sort the touch words v[0..nt) by partner index (insertion sort: rows are short) and drop
@@ -808,11 +814,14 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
     const int64_t n = P.n;
     const int K = P.K;
     uint64_t created = 0, deact = 0, entries = 0;
-    uint32_t overflow = 0;
+    uint32_t overflow = 0, need = 0;
     if (p64 < n) {
         const uint32_t p = (uint32_t)p64;
         uint32_t nt = tcnt[p];
-        const int b = sel[p];
+        need = nt;
+        // the committed buffer: after a failed step of the call a row may still be marked
+        // pending; this merge then overwrites the uncommitted buffer again, which the host drops
+        const int b = sel[p] & 1;
         const uint32_t c0 = rcnt0[p], c1 = rcnt1[p];
         const uint32_t nold = b ? c1 : c0;
         if (nt | nold) {
@@ -860,7 +869,8 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
         wave_add(ctr_at(ctr, CT_DEACT), deact);
         wave_add(ctr_at(ctr, CT_ENTRIES), entries);
     }
-    if (overflow) atomicAdd(ctr_at(ctr, CT_OVERFLOW), 1ull);
+    // the row this particle needs: its merged length, or at least its touching count
+    if (overflow) fail_step(ctr, (uint32_t)entries > need ? (uint32_t)entries : need);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -994,11 +1004,14 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         const double4* __restrict__ omg, const double4* __restrict__ quat,
         double4* __restrict__ pos_n, double4* __restrict__ vel_n, double4* __restrict__ omg_n,
         double4* __restrict__ quat_n, double4* __restrict__ acc, double4* __restrict__ dwo,
-        double4* __restrict__ ftot, double4* __restrict__ ttot, const uint8_t* __restrict__ sel,
+        double4* __restrict__ ftot, double4* __restrict__ ttot, uint8_t* __restrict__ sel,
         const uint32_t* __restrict__ rcnt0, const uint32_t* __restrict__ rcnt1,
         uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
         double* __restrict__ st1, uint32_t* __restrict__ wact, double4* __restrict__ wshear,
         unsigned long long* ctr, int write_forces) {
+    // a row of this step or an earlier one overflowed: change nothing.  Loaded first and tested
+    // before the first store, so the load's latency hides behind the state loads.
+    const bool failed = step_failed(ctr);
     // XCD-contiguous blocks: partners are spatial neighbours, so their state is in this XCD's L2
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     const int64_t n = P.n;
@@ -1011,7 +1024,11 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
         d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
         CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
-        const int b = sel[p];
+        // commit a row the merge rebuilt this step (it sits in the other buffer)
+        const uint8_t sl = sel[p];
+        const int b = (sl & ROW_PENDING) ? ((sl & 1) ^ 1) : sl;
+        if (failed) return;
+        if (sl & ROW_PENDING) sel[p] = (uint8_t)b;
         uint32_t* pq = b ? pq1 : pq0;
         double* st = b ? st1 : st0;
         const uint32_t cnt = (b ? rcnt1 : rcnt0)[p];
@@ -1356,6 +1373,8 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
         }
     }
+    if (failed) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[CT_DONE], 1ull);
     wave_add(ctr_at(ctr, CT_BROKEN), broken);
     wave_add(ctr_at(ctr, CT_BROKEN_TOTAL), broken);
     if (write_forces) {
@@ -1363,6 +1382,24 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         wave_add(ctr_at(ctr, CT_FATAL), fatal);
         wave_add(ctr_at(ctr, CT_ACTIVE), active);
         wave_add(ctr_at(ctr, CT_ACTIVE_BOND), active_b);
+    }
+}
+
+// After a bond pass (rollback = 0): commit the rows it rebuilt.  After a failed step
+// (rollback = 1): keep the committed rows, clear the touching counts the skipped merges left, and
+// clear the failure flags (copy 0 of the flags; every copy of the overflow count).
+__global__ void __launch_bounds__(TPB)
+k_rows_finish(int64_t n, uint8_t* __restrict__ sel, uint32_t* __restrict__ tcnt,
+              unsigned long long* __restrict__ ctr, int rollback) {
+    const int64_t p = (int64_t)blockIdx.x * TPB + threadIdx.x;
+    if (p < n) {
+        const uint8_t sl = sel[p];
+        if (sl & ROW_PENDING) sel[p] = rollback ? (uint8_t)(sl & 1) : (uint8_t)((sl & 1) ^ 1);
+        if (rollback) tcnt[p] = 0u;
+    }
+    if (rollback && blockIdx.x == 0) {
+        for (int c = threadIdx.x; c < CTR_COPIES; c += TPB) ctr[c * CTR_STRIDE + CT_OVERFLOW] = 0ull;
+        if (threadIdx.x == 0) { ctr[CT_FAIL] = 0ull; ctr[CT_MAXROW] = 0ull; }
     }
 }
 
@@ -1768,6 +1805,12 @@ static hipError_t force_impl(dem_scene* s, bool write_forces) {
 
 hipError_t run_force_integrate(dem_scene* s, bool write_forces) {
     return s->batched ? force_impl<true>(s, write_forces) : force_impl<false>(s, write_forces);
+}
+
+hipError_t run_rows_finish(dem_scene* s, bool rollback) {
+    hipLaunchKernelGGL(k_rows_finish, grid_for(s->n, TPB), dim3(TPB), 0, s->stream, s->n, s->d.sel,
+                       s->d.tcnt, s->d.ctr, rollback ? 1 : 0);
+    return hipGetLastError();
 }
 
 hipError_t run_debug_cells(dem_scene* s, int32_t* kid, uint32_t* hid, double* kc, uint32_t* nbr) {

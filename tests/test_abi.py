@@ -126,7 +126,7 @@ def _grains(n=2):
     ("nwall", b"nwall must be in [0, 16]"),
     ("wall_mid", b"wall material id"),
     ("precision", b"unknown precision"),
-    ("slots", b"contact_slots must be in [1, 128]"),
+    ("slots", b"contact_slots must be in [1, 256]"),
     ("out_null", b"out is NULL"),
 ])
 def test_create_validates_before_any_gpu_work(case, message):
@@ -153,7 +153,7 @@ def test_create_validates_before_any_gpu_work(case, message):
     elif case == "precision":
         opt.precision = 7
     elif case == "slots":
-        opt.contact_slots = 200
+        opt.contact_slots = 300
     elif case == "out_null":
         out = None
     rc = lib.dem_create(ctypes.byref(cfg), walls, nwall, g, n, ctypes.byref(opt), out)
