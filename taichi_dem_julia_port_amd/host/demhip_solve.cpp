@@ -10,12 +10,12 @@
 //       (save_single, executor.jl:351-358, 363-386; utils.jl:30-78).
 //   demhip_solve --print-config CONFIG.toml
 //       The parsed configuration, one "key = value" a line, for checking the loader.
-//   demhip_solve --repr X [X ...]
+//   demhip_solve --format X [X ...]
 //       The number format of the output files, for checking it.
 //
-// Numbers are written in the shortest form that reads back to the same double, laid out like
-// Python's repr (fixed notation for decimal exponents -4 < e <= 16, else d.ddde+XX), the format
-// of this package's Python writer, so both drivers produce the same bytes.
+// Numbers are written as Julia writes a Float64 (save_single's string interpolation,
+// utils.jl:43,67), as this package's Python writer does, so both drivers produce the reference's
+// text and the same bytes.
 #include <zlib.h>
 
 #include <charconv>
@@ -217,34 +217,44 @@ Config load_toml(const std::string& path) {
 
 // ---- numbers and files -------------------------------------------------------------------
 
-// Shortest round-trip digits, laid out like Python's repr.
+// The text Julia writes for a Float64 in string interpolation, as save_single does
+// (utils.jl:43,67): the shortest digits that read back as x, in decimal notation when the decimal
+// point falls at -4 < pt <= 6 (x = 0.d1d2... * 10^pt and, for integral values, the digits dropped
+// to the left of the point are zeros), else d.ddd e N with an unpadded exponent; a whole number
+// keeps ".0" (1.0, 100000.0, 1.0e6, 1.0e-5, 0.0001, -0.0, NaN, Inf).
 std::string num(double x) {
-    if (std::isnan(x)) return "nan";
-    if (std::isinf(x)) return x > 0 ? "inf" : "-inf";
+    if (std::isnan(x)) return "NaN";
+    if (std::isinf(x)) return x > 0 ? "Inf" : "-Inf";
     if (x == 0.0) return std::signbit(x) ? "-0.0" : "0.0";
     char buf[64];
     const auto r = std::to_chars(buf, buf + sizeof buf, x, std::chars_format::scientific);
-    const std::string s(buf, r.ptr);  // [-]d[.ddd]e(+|-)XX
+    const std::string s(buf, r.ptr);  // [-]d[.ddd]e(+|-)XX, shortest digits
     const bool neg = s[0] == '-';
     const size_t e = s.find('e');
     std::string digits;
     for (size_t i = neg ? 1 : 0; i < e; ++i)
         if (s[i] != '.') digits += s[i];
-    const int exp10 = std::atoi(s.c_str() + e + 1);
-    const int decpt = exp10 + 1;  // value = 0.d1d2... x 10^decpt
+    const int olength = (int)digits.size();
+    const int pt = std::atoi(s.c_str() + e + 1) + 1;
+    bool plain = pt > -4 && pt <= 6;
+    if (plain && pt >= olength) {  // Julia's mod(x + 0.05, 10^(pt - olength)) test
+        const double step = std::pow(10.0, pt - olength);
+        double m = std::fmod(x + 0.05, step);
+        if (m != 0.0 && std::signbit(m) != std::signbit(step)) m += step;
+        plain = !(std::fabs(m - 0.05) > 0.05);
+    }
     std::string out = neg ? "-" : "";
-    if (decpt <= -4 || decpt > 16) {
+    if (!plain) {
         out += digits[0];
-        if (digits.size() > 1) out += "." + digits.substr(1);
-        char eb[16];
-        std::snprintf(eb, sizeof eb, "e%c%02d", exp10 < 0 ? '-' : '+', std::abs(exp10));
-        out += eb;
-    } else if (decpt <= 0) {
-        out += "0." + std::string(-decpt, '0') + digits;
-    } else if ((int)digits.size() <= decpt) {
-        out += digits + std::string(decpt - digits.size(), '0') + ".0";
+        out += ".";
+        out += olength > 1 ? digits.substr(1) : "0";
+        out += "e" + std::to_string(pt - 1);
+    } else if (pt <= 0) {
+        out += "0." + std::string(-pt, '0') + digits;
+    } else if (olength <= pt) {
+        out += digits + std::string(pt - olength, '0') + ".0";
     } else {
-        out += digits.substr(0, decpt) + "." + digits.substr(decpt);
+        out += digits.substr(0, pt) + "." + digits.substr(pt);
     }
     return out;
 }
@@ -346,7 +356,7 @@ int usage() {
                  "usage: demhip_solve CONFIG.toml [--no-bond] [--no-save] [--precision fp64|mixed]\n"
                  "                    [--out DIR] [--max-steps N]\n"
                  "       demhip_solve --print-config CONFIG.toml\n"
-                 "       demhip_solve --repr X [X ...]\n");
+                 "       demhip_solve --format X [X ...]\n");
     return 2;
 }
 
@@ -427,7 +437,7 @@ int main(int argc, char** argv) {
     if (argc < 2) return usage();
     try {
         const std::string first = argv[1];
-        if (first == "--repr") {
+        if (first == "--format") {
             for (int i = 2; i < argc; ++i) std::printf("%s\n", num(std::strtod(argv[i], nullptr)).c_str());
             return 0;
         }

@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import gzip
 import math
+from decimal import Decimal
 from typing import Dict, List, Tuple
 
 import numpy as np
@@ -81,8 +82,46 @@ def read_p4c(path: str) -> List[Tuple[float, List[Tuple[Tuple[int, int], tuple]]
     return out
 
 
+def julia_float(x: float) -> str:
+    """The text Julia 1.9 writes for a Float64 in string interpolation ("$x"), as save_single
+    does (utils.jl:43,67): the shortest digits that read back as x, in decimal notation when the
+    decimal point falls at -4 < pt <= 6 (x = 0.d1d2... * 10^pt and, for integral values, the
+    digits dropped to the left of the point are zeros), else d.ddd e N with an unpadded exponent;
+    a whole number keeps ".0" (1.0, 100000.0, 1.0e6, 1.0e-5, 0.0001, -0.0, NaN, Inf).  Python's
+    repr gives the same shortest digits."""
+    x = float(x)
+    if math.isnan(x):
+        return "NaN"
+    if math.isinf(x):
+        return "Inf" if x > 0 else "-Inf"
+    if x == 0.0:
+        return "-0.0" if math.copysign(1.0, x) < 0 else "0.0"
+    sign = "-" if x < 0 else ""
+    dec = Decimal(repr(abs(x))).normalize().as_tuple()
+    digits = "".join(map(str, dec.digits))
+    olength = len(digits)
+    pt = dec.exponent + olength
+    plain = -4 < pt <= 6
+    if plain and pt >= olength:
+        step = float(10 ** (pt - olength))
+        r = math.fmod(x + 0.05, step)
+        if r != 0.0 and (r < 0) != (step < 0):
+            r += step
+        plain = not abs(r - 0.05) > 0.05
+    if plain:
+        if pt <= 0:
+            body = "0." + "0" * (-pt) + digits
+        elif pt >= olength:
+            body = digits + "0" * (pt - olength) + ".0"
+        else:
+            body = digits[:pt] + "." + digits[pt:]
+    else:
+        body = digits[0] + "." + (digits[1:] if olength > 1 else "0") + "e" + str(pt - 1)
+    return sign + body
+
+
 def _fmt(x: float) -> str:
-    return repr(float(x))
+    return julia_float(x)
 
 
 def write_p4p_frame(fh, grains: np.ndarray, t: float) -> None:

@@ -1,7 +1,9 @@
 """The native solve() driver (taichi_dem_julia_port_amd/host/demhip_solve.cpp), without a GPU:
 its TOML loader against the Python loader (from_toml) on the reference's configurations, defaults
-and malformed input, and its number format against Python's repr, which the Python p4p/p4c writer
-uses -- so both drivers write the same bytes (checked on the GPU in test_gpu_cli.py)."""
+and malformed input; and the number format of the output files, Julia's Float64 text
+(save_single, utils.jl:43,67), in both writers: hand-derived cases of Julia's printing rules and
+the native writer against the Python writer on 2,000 values -- so both drivers write the
+reference's text and the same bytes (checked on the GPU in test_gpu_cli.py)."""
 import dataclasses
 import math
 import os
@@ -13,6 +15,7 @@ import pytest
 
 from taichi_dem_julia_port_amd import _build
 from taichi_dem_julia_port_amd.config import DEMConfig, from_toml
+from taichi_dem_julia_port_amd.p4io import julia_float
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -30,13 +33,13 @@ def _python_lines(cfg: DEMConfig):
     for f in dataclasses.fields(cfg):
         v = getattr(cfg, f.name)
         if isinstance(v, list):
-            out.append(f"{f.name} = {' '.join(repr(float(x)) for x in v)}")
+            out.append(f"{f.name} = {' '.join(julia_float(x) for x in v)}")
         elif f.type in ("int", int):
             out.append(f"{f.name} = {int(v)}")
         elif f.type in ("str", str):
             out.append(f"{f.name} = {v}")
         else:
-            out.append(f"{f.name} = {repr(float(v))}")
+            out.append(f"{f.name} = {julia_float(v)}")
     return out
 
 
@@ -72,21 +75,47 @@ def test_loader_rejects_malformed_input(tmp_path, text, message):
     assert r.returncode == 1 and message in r.stderr, r.stderr
 
 
-def test_number_format_is_python_repr():
+# Julia's text for a Float64: shortest round-trip digits; decimal notation when the decimal point
+# position pt (x = 0.d1d2... x 10^pt) is in (-4, 6], else d.ddd e N; whole numbers keep ".0"
+JULIA_TEXT = [
+    (0.0, "0.0"), (-0.0, "-0.0"), (1.0, "1.0"), (-1.5, "-1.5"), (0.1, "0.1"), (25.0, "25.0"),
+    (1e-5, "1.0e-5"), (1.5e-5, "1.5e-5"), (1e-4, "0.0001"), (1.234e-4, "0.0001234"),
+    (0.001, "0.001"), (4.18879e-9, "4.18879e-9"), (100000.0, "100000.0"), (999999.0, "999999.0"),
+    (123456.7, "123456.7"), (1e6, "1.0e6"), (1234567.0, "1.234567e6"), (-9.81, "-9.81"),
+    (28e9, "2.8e10"), (1e16, "1.0e16"), (2.0 / 3.0 * 1e-5, "6.666666666666667e-6"),
+    (0.30000000000000004, "0.30000000000000004"), (5e-324, "5.0e-324"),
+    (1.7976931348623157e308, "1.7976931348623157e308"), (float("nan"), "NaN"),
+    (float("inf"), "Inf"), (float("-inf"), "-Inf"),
+]
+
+
+def test_number_format_is_julias():
+    xs = [x for x, _ in JULIA_TEXT]
+    want = [t for _, t in JULIA_TEXT]
+    assert [julia_float(x) for x in xs] == want
+    assert _run("--format", *[repr(x) for x in xs]).stdout.splitlines() == want
+    # every text reads back as the same double
+    for x, t in JULIA_TEXT:
+        assert math.isnan(x) or float(t) == x
+
+
+def test_native_number_format_matches_the_python_writer():
     vals = [0.0, -0.0, 1.0, -1.5, 0.1, 1e-5, 1e-4, 1.234e-4, 1e16, 1e15, 9.999999999999999e15,
             1.2345678901234567e17, 2.56e-6, 5e-324, 2.2250738585072014e-308, 1.7976931348623157e308,
-            0.30000000000000004, -9.81, 7e10, 28e9, 0.03125, 1234567.0]
+            0.30000000000000004, -9.81, 7e10, 28e9, 0.03125, 1234567.0, 99999.99999999999,
+            100000.00000000001, 999999.9999999999, 0.00010000000000000002, 9.999999999999999e-5]
     rng = random.Random(11)
     while len(vals) < 2000:
         x = struct.unpack("<d", struct.pack("<Q", rng.getrandbits(64)))[0]
         if math.isfinite(x):
             vals.append(x)
         vals.append(rng.uniform(-1, 1) * 10.0 ** rng.randint(-30, 30))
-    args = [repr(v) for v in vals]
+        vals.append(float(rng.randint(-10 ** 7, 10 ** 7)) * 10.0 ** rng.randint(-3, 2))
     out = []
-    for i in range(0, len(args), 400):
-        out += _run("--repr", *args[i:i + 400]).stdout.splitlines()
-    assert out == args
+    for i in range(0, len(vals), 400):
+        out += _run("--format", *[repr(v) for v in vals[i:i + 400]]).stdout.splitlines()
+    assert out == [julia_float(v) for v in vals]
+    assert all(float(t) == v for t, v in zip(out, vals))
 
 
 def test_usage_errors():
