@@ -992,6 +992,251 @@ struct CSum3 {
     }
 };
 
+// What one contact adds to its two grains, in its canonical orientation (i the lower file index,
+// j the higher): the force on i (j receives its negation, which is what the reference computes for
+// j bit for bit), the moments on i and j, and whether the FP64 sums or the compensated FP32 sums of
+// mixed precision take them.  A bond that breaks adds nothing.
+constexpr uint8_t CK_NONE = 0, CK_FP64 = 1, CK_FP32 = 2;
+struct Contrib {
+    d3 F, Ti, Tj;
+    uint8_t kind = CK_NONE;
+    bool broke = false, bonded = false, fatal = false;
+};
+
+// The contact of row entry `word` (kernels.jl:122-315): EBPM for a bond (kernels.jl:170-239),
+// Hertz-Mindlin otherwise (kernels.jl:240-313).  The contact's state (a bond's F, ti, tj; a Hertz
+// contact's spring s) is read from and written to the row entry at st[c * cs + e]; the contact
+// point and a Hertz contact's local force are written only when write_forces is set.  plo: the
+// row's particle is i; inv_r, inv_m are that particle's reciprocal radius and mass.
+template <bool MIXED>
+__device__ __forceinline__ Contrib eval_contact(const SceneP& S, double dt, uint32_t word, bool plo,
+                                                d3 ki, d3 kj, d3 vi, d3 vj, d3 wi, d3 wj, double ri,
+                                                double rj, double mi, double mj, double inv_r,
+                                                double inv_m, double* __restrict__ st, int64_t cs,
+                                                int64_t e, int write_forces) {
+    Contrib o;
+    const double Li = norm(kj - ki);
+    if (word & BONDED) {
+      if (MIXED) {
+        // EBPM with the frame, stiffnesses, increments and stresses in FP32, added to the
+        // FP64 force and moment state.  Relative motion comes from FP64 velocity
+        // differences (bonded grains share most of their motion).  A stress within FP32
+        // rounding of its strength, or a normal near -x, takes the FP64 law below.
+        const f3 d32 = tof(kj - ki);
+        const f3 a32 = fscl(d32, frcp((float)Li));
+        if (a32.x >= -0.99f) {
+            const fm3 R32 = contact_frame_f32(a32, 1e-6f);
+            const float dtf = (float)dt;
+            const f3 dd = fmul(R32, fscl(tof(vi - vj), dtf));    // d_i - d_j
+            const f3 dth = fmul(R32, fscl(tof(wi - wj), dtf));   // theta_i - theta_j
+            const f3 thi = fmul(R32, fscl(tof(wi), dtf)), thj = fmul(R32, fscl(tof(wj), dtf));
+            const SurfaceP& sf = S.surf[0][0];
+            const float rb = (float)sf.rho * (float)(ri < rj ? ri : rj);
+            const float iL = frcp((float)Li);
+            const float E = (float)sf.E, nu = (float)sf.nu;
+            const float rb2 = rb * rb;
+            const float Ib = rb2 * rb2 * (float)(M_PI / 4.0);
+            const float phi = (20.0f / 3.0f) * rb2 * iL * iL * (1.0f + nu);
+            const float Ab = rb2 * (float)M_PI;
+            const float i1p = frcp(1.0f + phi);
+            const float EI = E * Ib;
+            const float k1 = E * Ab * iL;
+            const float k2 = 12.0f * EI * iL * iL * iL * i1p;
+            const float k3 = 6.0f * EI * iL * iL * i1p;
+            const float k4 = EI * iL * frcp(1.0f + nu);
+            const float k5 = EI * (4.0f + phi) * iL * i1p;
+            const float k6 = EI * (2.0f - phi) * iL * i1p;
+            const f3 dF = mkf(k1 * dd.x, k2 * dd.y + k3 * (thi.z + thj.z), k2 * dd.z - k3 * (thi.y + thj.y));
+            const f3 dTi = mkf(k4 * dth.x, -k3 * dd.z + k5 * thi.y + k6 * thj.y, k3 * dd.y + k5 * thi.z + k6 * thj.z);
+            const f3 dTj = mkf(-k4 * dth.x, -k3 * dd.z + k6 * thi.y + k5 * thj.y, k3 * dd.y + k6 * thi.z + k5 * thj.z);
+            const d3 Fi = mk3(st[S_FX * cs + e], st[S_FY * cs + e], st[S_FZ * cs + e]) + tod(dF);
+            const d3 Ti = mk3(st[S_TIX * cs + e], st[S_TIY * cs + e], st[S_TIZ * cs + e]) + tod(dTi);
+            const d3 Tj = mk3(st[S_TJX * cs + e], st[S_TJY * cs + e], st[S_TJZ * cs + e]) + tod(dTj);
+            const f3 F32 = tof(Fi), Ti32 = tof(Ti), Tj32 = tof(Tj);
+            const float iAb = frcp(Ab), rbIb = rb * frcp(Ib);
+            const float sci = -F32.x * iAb - rbIb * fsqrt(Ti32.y * Ti32.y + Ti32.z * Ti32.z);
+            const float scj = -F32.x * iAb - rbIb * fsqrt(Tj32.y * Tj32.y + Tj32.z * Tj32.z);
+            const float sc = -fminf(sci, scj), stt = fmaxf(sci, scj);
+            const float ss = fabsf(Ti32.x) * rbIb * 0.5f + (4.0f / 3.0f) * iAb * fsqrt(F32.y * F32.y + F32.z * F32.z);
+            const float SC = (float)sf.sc, ST = (float)sf.st, SS = (float)sf.ss;
+            auto close = [](float a, float b) { return fabsf(a - b) <= 1e-4f * fmaxf(fabsf(a), fabsf(b)); };
+            if (!(close(sc, SC) || close(stt, ST) || close(ss, SS))) {
+                if (sc >= SC || stt >= ST || ss >= SS) {
+                    o.broke = true;  // both copies take the same decision
+                    return o;
+                }
+                o.kind = CK_FP32;
+                o.F = tod(fmul_t(R32, fneg(F32)));
+                o.Ti = tod(fmul_t(R32, fneg(Ti32)));
+                o.Tj = tod(fmul_t(R32, fneg(Tj32)));
+                const d3 kc = (ki + kj) * 0.5;
+                if (write_forces) {  // the contact point is an output, not state
+                    st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+                }
+                st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
+                st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
+                st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
+                o.bonded = true;
+                return o;
+            }
+        }
+      }
+        // EBPM bond (kernels.jl:170-239)
+        const m3 R = contact_frame(normalize(kj - ki), S.tol);
+        const d3 kc = (ki + kj) * 0.5;
+        const d3 di = mul(R, vi * dt), dj = mul(R, vj * dt);
+        const d3 thi = mul(R, wi * dt), thj = mul(R, wj * dt);
+        const SurfaceP& sf = S.surf[0][0];
+        const double rmin = ri < rj ? ri : rj;
+        const double rb = sf.rho * rmin;
+        const double L = Li;
+        const double E = sf.E, nu = sf.nu;
+        const double rb2 = rb * rb;
+        const double Ib = rb2 * rb2 * M_PI / 4.0;
+        const double phi = 20.0 / 3.0 * rb2 / (L * L) * (1.0 + nu);
+        const double Ab = rb2 * M_PI;
+        const double k1 = E * Ab / L;
+        const double k2 = 12.0 * E * Ib / (L * L * L) / (1.0 + phi);
+        const double k3 = 6.0 * E * Ib / (L * L) / (1.0 + phi);
+        const double k4 = E * Ib / L / (1.0 + nu);
+        const double k5 = E * Ib * (4.0 + phi) / L / (1.0 + phi);
+        const double k6 = E * Ib * (2.0 - phi) / L / (1.0 + phi);
+        const d3 dF = mk3(k1 * (di.x - dj.x), k2 * (di.y - dj.y) + k3 * (thi.z + thj.z),
+                          k2 * (di.z - dj.z) - k3 * (thi.y + thj.y));
+        const d3 dTi = mk3(k4 * (thi.x - thj.x), k3 * (dj.z - di.z) + k5 * thi.y + k6 * thj.y,
+                           k3 * (di.y - dj.y) + k5 * thi.z + k6 * thj.z);
+        const d3 dTj = mk3(k4 * (thj.x - thi.x), k3 * (dj.z - di.z) + k6 * thi.y + k5 * thj.y,
+                           k3 * (di.y - dj.y) + k6 * thi.z + k5 * thj.z);
+        const d3 Fi = mk3(st[S_FX * cs + e], st[S_FY * cs + e], st[S_FZ * cs + e]) + dF;
+        const d3 Fj = -Fi;
+        const d3 Ti = mk3(st[S_TIX * cs + e], st[S_TIY * cs + e], st[S_TIZ * cs + e]) + dTi;
+        const d3 Tj = mk3(st[S_TJX * cs + e], st[S_TJY * cs + e], st[S_TJZ * cs + e]) + dTj;
+        const double sci = Fj.x / Ab - rb / Ib * sqrt(Ti.y * Ti.y + Ti.z * Ti.z);
+        const double scj = Fj.x / Ab - rb / Ib * sqrt(Tj.y * Tj.y + Tj.z * Tj.z);
+        const double sc = -(sci < scj ? sci : scj);
+        const double stt = sci > scj ? sci : scj;
+        const double ss = fabs(Ti.x) * rb / 2.0 / Ib + 4.0 / 3.0 / Ab * sqrt(Fi.y * Fi.y + Fi.z * Fi.z);
+        if (sc >= sf.sc || stt >= sf.st || ss >= sf.ss) {
+            o.broke = true;  // both copies take the same decision
+            return o;
+        }
+        const m3 Ri = inverse(R);
+        // the force on j, mul(Ri, -Fj) with Fj = -Fi, is -o.F bit for bit
+        o.kind = CK_FP64;
+        o.F = mul(Ri, -Fi);
+        o.Ti = mul(Ri, -Ti);
+        o.Tj = mul(Ri, -Tj);
+        if (write_forces) {  // the contact point is an output, not state
+            st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+        }
+        st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
+        st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
+        st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
+        // s of a bond stays at the zero it was created (or restored) with; kernels.jl:230-238 stores zero
+        o.bonded = true;
+        return o;
+    } else {
+      if (MIXED) {
+        // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance.
+        // Contacts it cannot decide like FP64 fall through to the FP64 law below.
+        const double gap = Li - ri - rj;
+        const double dn = fabs(gap);
+        const f3 d32 = tof(kj - ki);
+        const f3 a32 = fscl(d32, frcp((float)Li));
+        // near a = -x the reference's frame is ill-conditioned (its rotation about the
+        // normal follows the normal's tiny transverse components): FP64 there
+        bool fp64 = a32.x < -0.99f;
+        const fm3 R32 = contact_frame_f32(a32, 1e-6f);
+        const float lever = (float)(ri - dn);
+        const f3 kip = fscl(a32, lever);
+        const f3 kjp = fsub(kip, d32);
+        // (w_j x k_j' + v_j) - (w_i x k_i' + v_i) with v_j - v_i taken in FP64: grains in
+        // contact share most of their velocity, and subtracting FP32 copies of two fast
+        // grains' velocities would lose the small relative part
+        const f3 dv = tof(vj - vi);
+        const f3 vc = fmul(R32, fadd(fsub(fcross(tof(wj), kjp), fcross(tof(wi), kip)), dv));
+        const float Rs = frcp(frcp((float)ri) + frcp((float)rj));
+        const float ms = frcp(frcp((float)mi) + frcp((float)mj));
+        const float srd = fsqrt(Rs * (float)dn);
+        const float kn = (4.0f / 3.0f) * S.ys_pp * srd;
+        const float kt = 8.0f * S.gs_pp * srd;
+        const float gn = -2.0f * S.beta_pp * fsqrt((5.0f / 6.0f) * (2.0f * S.ys_pp * srd) * ms);
+        const float gt = -2.0f * S.beta_pp * fsqrt((5.0f / 6.0f) * kt * ms);
+        // the tangential spring accumulates in FP64: v dt increments vanish in FP32
+        d3 sv = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
+        sv = sv + mk3((double)vc.x * 0.0, (double)vc.y * dt, (double)vc.z * dt);
+        const float F1 = -kn * (float)gap - gn * vc.x;
+        const f3 Ft = fscl(tof(sv), -kt);
+        const float nFt = fnorm(Ft);
+        // a slip decision within FP32 rounding of the threshold is taken in FP64: the
+        // reference's slip rule reverses the stored spring, so a flipped decision is an
+        // O(1) change of the tangential force
+        const float thr = S.mu_pp * F1;
+        fp64 = fp64 || fabsf(nFt - thr) <= 1e-4f * fmaxf(fabsf(nFt), fabsf(thr));
+        float F2, F3;
+        if (fp64) {
+            F2 = F3 = 0.0f;
+        } else if (nFt >= thr) {
+            const float ratio = S.mu_pp * F1 * frcp(nFt);
+            F2 = ratio * Ft.y;
+            F3 = ratio * Ft.z;
+            const float ikt = frcp(kt);
+            sv = mk3(sv.x, (double)(F2 * ikt), (double)(F3 * ikt));
+        } else {
+            F2 = Ft.y - gt * vc.y;
+            F3 = Ft.z - gt * vc.z;
+        }
+        if (!fp64) {
+        const f3 Fl = mkf(F1, F2, F3);
+        const f3 Fg = fmul_t(R32, mkf(-F1, -F2, -F3));
+        o.kind = CK_FP32;
+        o.F = tod(Fg);
+        o.Ti = tod(fcross(kip, Fg));
+        o.Tj = tod(fcross(kjp, mkf(-Fg.x, -Fg.y, -Fg.z)));
+        const d3 kc = ki + tod(kip);
+        if (write_forces) {  // contact point and local force are outputs, not state
+            st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+            st[S_FX * cs + e] = (double)Fl.x; st[S_FY * cs + e] = (double)Fl.y; st[S_FZ * cs + e] = (double)Fl.z;
+        }
+        st[S_SX * cs + e] = sv.x; st[S_SY * cs + e] = sv.y; st[S_SZ * cs + e] = sv.z;
+        o.fatal = gap > 0 || gn < 0.0f || gt < 0.0f;
+        return o;
+        }
+      }
+      {
+        // Hertz-Mindlin (kernels.jl:240-313)
+        const m3 R = contact_frame(normalize(kj - ki), S.tol);
+        const double gap = Li - ri - rj;
+        const double dn = fabs(gap);
+        const d3 kc = ki + normalize(kj - ki) * (ri - dn);
+        const d3 kip = kc - ki, kjp = kc - kj;
+        const d3 vci = cross(wi, kip) + vi;
+        const d3 vcj = cross(wj, kjp) + vj;
+        const d3 vc = mul(R, vcj - vci);
+        const SurfaceP& sf = S.surf[0][0];
+        // 1/r_i + 1/r_j with this particle's reciprocal taken once per row (IEEE addition
+        // commutes, so the sum is the canonical one bit for bit)
+        const double Rs = 1.0 / (inv_r + 1.0 / (plo ? rj : ri));
+        const double ms = 1.0 / (inv_m + 1.0 / (plo ? mj : mi));
+        const d3 s0 = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
+        const HMOut h = hertz_mindlin(gap, dn, vc, S.ysm[0], S.gsm[0], Rs, ms, sf, dt, s0);
+        const m3 Ri = inverse(R);
+        const d3 Fg = mul(Ri, -h.F);
+        o.kind = CK_FP64;
+        o.F = Fg;
+        o.Ti = cross(kip, Fg);
+        o.Tj = cross(kjp, -Fg);
+        if (write_forces) {  // contact point and local force are outputs, not state
+            st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
+            st[S_FX * cs + e] = h.F.x; st[S_FY * cs + e] = h.F.y; st[S_FZ * cs + e] = h.F.z;
+        }
+        st[S_SX * cs + e] = h.s.x; st[S_SY * cs + e] = h.s.y; st[S_SZ * cs + e] = h.s.z;
+        o.fatal = gap > 0 || h.fatal;
+        return o;
+      }
+    }
+}
+
 // The single-scene mixed kernel carries both contact laws in two precisions; left alone the
 // compiler gives it more than 256 registers and one wave per SIMD, so it is held to two waves per
 // SIMD. The other variants fit in 256 and keep the schedule the compiler picks without the bound.
@@ -1054,7 +1299,6 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
                 w_cur = w_nxt;
                 if (m + 2 < cnt) w_nxt = pq[(int64_t)(m + 2) * n + p];
             }
-            const uint32_t q = word & IDX_MASK;
             const bool plo = (word & IS_I) != 0u;  // p is the contact's i (the lower file index)
             // canonical orientation: i is the lower file index, j the higher
             const d3 ki = plo ? kp : mk3(Xq.x, Xq.y, Xq.z);
@@ -1065,233 +1309,27 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             const d3 wj = plo ? mk3(Wq.x, Wq.y, Wq.z) : wp;
             const double ri = plo ? X.w : Xq.w, rj = plo ? Xq.w : X.w;
             const double mi = plo ? Vv.w : Vq.w, mj = plo ? Vq.w : Vv.w;
-            const double Li = norm(kj - ki);
-            if (word & BONDED) {
-              if (MIXED) {
-                // EBPM with the frame, stiffnesses, increments and stresses in FP32, added to the
-                // FP64 force and moment state.  Relative motion comes from FP64 velocity
-                // differences (bonded grains share most of their motion).  A stress within FP32
-                // rounding of its strength, or a normal near -x, takes the FP64 law below.
-                const f3 d32 = tof(kj - ki);
-                const f3 a32 = fscl(d32, frcp((float)Li));
-                if (a32.x >= -0.99f) {
-                    const fm3 R32 = contact_frame_f32(a32, 1e-6f);
-                    const float dtf = (float)dt;
-                    const f3 dd = fmul(R32, fscl(tof(vi - vj), dtf));    // d_i - d_j
-                    const f3 dth = fmul(R32, fscl(tof(wi - wj), dtf));   // theta_i - theta_j
-                    const f3 thi = fmul(R32, fscl(tof(wi), dtf)), thj = fmul(R32, fscl(tof(wj), dtf));
-                    const SurfaceP& sf = S.surf[0][0];
-                    const float rb = (float)sf.rho * (float)(ri < rj ? ri : rj);
-                    const float iL = frcp((float)Li);
-                    const float E = (float)sf.E, nu = (float)sf.nu;
-                    const float rb2 = rb * rb;
-                    const float Ib = rb2 * rb2 * (float)(M_PI / 4.0);
-                    const float phi = (20.0f / 3.0f) * rb2 * iL * iL * (1.0f + nu);
-                    const float Ab = rb2 * (float)M_PI;
-                    const float i1p = frcp(1.0f + phi);
-                    const float EI = E * Ib;
-                    const float k1 = E * Ab * iL;
-                    const float k2 = 12.0f * EI * iL * iL * iL * i1p;
-                    const float k3 = 6.0f * EI * iL * iL * i1p;
-                    const float k4 = EI * iL * frcp(1.0f + nu);
-                    const float k5 = EI * (4.0f + phi) * iL * i1p;
-                    const float k6 = EI * (2.0f - phi) * iL * i1p;
-                    const f3 dF = mkf(k1 * dd.x, k2 * dd.y + k3 * (thi.z + thj.z), k2 * dd.z - k3 * (thi.y + thj.y));
-                    const f3 dTi = mkf(k4 * dth.x, -k3 * dd.z + k5 * thi.y + k6 * thj.y, k3 * dd.y + k5 * thi.z + k6 * thj.z);
-                    const f3 dTj = mkf(-k4 * dth.x, -k3 * dd.z + k6 * thi.y + k5 * thj.y, k3 * dd.y + k6 * thi.z + k5 * thj.z);
-                    const d3 Fi = mk3(st[S_FX * cs + e], st[S_FY * cs + e], st[S_FZ * cs + e]) + tod(dF);
-                    const d3 Ti = mk3(st[S_TIX * cs + e], st[S_TIY * cs + e], st[S_TIZ * cs + e]) + tod(dTi);
-                    const d3 Tj = mk3(st[S_TJX * cs + e], st[S_TJY * cs + e], st[S_TJZ * cs + e]) + tod(dTj);
-                    const f3 F32 = tof(Fi), Ti32 = tof(Ti), Tj32 = tof(Tj);
-                    const float iAb = frcp(Ab), rbIb = rb * frcp(Ib);
-                    const float sci = -F32.x * iAb - rbIb * fsqrt(Ti32.y * Ti32.y + Ti32.z * Ti32.z);
-                    const float scj = -F32.x * iAb - rbIb * fsqrt(Tj32.y * Tj32.y + Tj32.z * Tj32.z);
-                    const float sc = -fminf(sci, scj), stt = fmaxf(sci, scj);
-                    const float ss = fabsf(Ti32.x) * rbIb * 0.5f + (4.0f / 3.0f) * iAb * fsqrt(F32.y * F32.y + F32.z * F32.z);
-                    const float SC = (float)sf.sc, ST = (float)sf.st, SS = (float)sf.ss;
-                    auto close = [](float a, float b) { return fabsf(a - b) <= 1e-4f * fmaxf(fabsf(a), fabsf(b)); };
-                    if (!(close(sc, SC) || close(stt, ST) || close(ss, SS))) {
-                        if (sc >= SC || stt >= ST || ss >= SS) {
-                            pq[e] = word | BROKEN;  // both copies take the same decision
-                            if (plo) {
-                                ++broken;
-                                if (SEG) atomicAdd(&P.seg_broken[P.sid[p]], 1ull);
-                            }
-                            continue;
-                        }
-                        if (plo) { FH.add(fmul_t(R32, fneg(F32))); TH.add(fmul_t(R32, fneg(Ti32))); }
-                        else { FH.add(fmul_t(R32, F32)); TH.add(fmul_t(R32, fneg(Tj32))); }
-                        const d3 kc = (ki + kj) * 0.5;
-                        if (write_forces) {  // the contact point is an output, not state
-                            st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
-                        }
-                        st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
-                        st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
-                        st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
-                        if (plo) { ++active; ++active_b; }
-                        continue;
-                    }
-                }
-              }
-                // EBPM bond (kernels.jl:170-239)
-                const m3 R = contact_frame(normalize(kj - ki), S.tol);
-                const d3 kc = (ki + kj) * 0.5;
-                const d3 di = mul(R, vi * dt), dj = mul(R, vj * dt);
-                const d3 thi = mul(R, wi * dt), thj = mul(R, wj * dt);
-                const SurfaceP& sf = S.surf[0][0];
-                const double rmin = ri < rj ? ri : rj;
-                const double rb = sf.rho * rmin;
-                const double L = Li;
-                const double E = sf.E, nu = sf.nu;
-                const double rb2 = rb * rb;
-                const double Ib = rb2 * rb2 * M_PI / 4.0;
-                const double phi = 20.0 / 3.0 * rb2 / (L * L) * (1.0 + nu);
-                const double Ab = rb2 * M_PI;
-                const double k1 = E * Ab / L;
-                const double k2 = 12.0 * E * Ib / (L * L * L) / (1.0 + phi);
-                const double k3 = 6.0 * E * Ib / (L * L) / (1.0 + phi);
-                const double k4 = E * Ib / L / (1.0 + nu);
-                const double k5 = E * Ib * (4.0 + phi) / L / (1.0 + phi);
-                const double k6 = E * Ib * (2.0 - phi) / L / (1.0 + phi);
-                const d3 dF = mk3(k1 * (di.x - dj.x), k2 * (di.y - dj.y) + k3 * (thi.z + thj.z),
-                                  k2 * (di.z - dj.z) - k3 * (thi.y + thj.y));
-                const d3 dTi = mk3(k4 * (thi.x - thj.x), k3 * (dj.z - di.z) + k5 * thi.y + k6 * thj.y,
-                                   k3 * (di.y - dj.y) + k5 * thi.z + k6 * thj.z);
-                const d3 dTj = mk3(k4 * (thj.x - thi.x), k3 * (dj.z - di.z) + k6 * thi.y + k5 * thj.y,
-                                   k3 * (di.y - dj.y) + k6 * thi.z + k5 * thj.z);
-                const d3 Fi = mk3(st[S_FX * cs + e], st[S_FY * cs + e], st[S_FZ * cs + e]) + dF;
-                const d3 Fj = -Fi;
-                const d3 Ti = mk3(st[S_TIX * cs + e], st[S_TIY * cs + e], st[S_TIZ * cs + e]) + dTi;
-                const d3 Tj = mk3(st[S_TJX * cs + e], st[S_TJY * cs + e], st[S_TJZ * cs + e]) + dTj;
-                const double sci = Fj.x / Ab - rb / Ib * sqrt(Ti.y * Ti.y + Ti.z * Ti.z);
-                const double scj = Fj.x / Ab - rb / Ib * sqrt(Tj.y * Tj.y + Tj.z * Tj.z);
-                const double sc = -(sci < scj ? sci : scj);
-                const double stt = sci > scj ? sci : scj;
-                const double ss = fabs(Ti.x) * rb / 2.0 / Ib + 4.0 / 3.0 / Ab * sqrt(Fi.y * Fi.y + Fi.z * Fi.z);
-                if (sc >= sf.sc || stt >= sf.st || ss >= sf.ss) {
-                    pq[e] = word | BROKEN;  // both copies take the same decision
-                    if (plo) {
-                        ++broken;
-                        if (SEG) atomicAdd(&P.seg_broken[P.sid[p]], 1ull);
-                    }
-                    continue;
-                }
-                const m3 Ri = inverse(R);
+            const Contrib o = eval_contact<MIXED>(S, dt, word, plo, ki, kj, vi, vj, wi, wj, ri, rj, mi,
+                                                  mj, inv_r, inv_m, st, cs, e, write_forces);
+            if (o.broke) {
+                pq[e] = word | BROKEN;  // both copies take the same decision
                 if (plo) {
-                    F = F + mul(Ri, -Fi);
-                    T = T + mul(Ri, -Ti);
-                } else {
-                    F = F + mul(Ri, -Fj);
-                    T = T + mul(Ri, -Tj);
-                }
-                if (write_forces) {  // the contact point is an output, not state
-                    st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
-                }
-                st[S_FX * cs + e] = Fi.x; st[S_FY * cs + e] = Fi.y; st[S_FZ * cs + e] = Fi.z;
-                st[S_TIX * cs + e] = Ti.x; st[S_TIY * cs + e] = Ti.y; st[S_TIZ * cs + e] = Ti.z;
-                st[S_TJX * cs + e] = Tj.x; st[S_TJY * cs + e] = Tj.y; st[S_TJZ * cs + e] = Tj.z;
-                // s of a bond stays at the zero it was created (or restored) with; kernels.jl:230-238 stores zero
-                if (plo) { ++active; ++active_b; }
-            } else {
-              if (MIXED) {
-                // Hertz-Mindlin with FP32 geometry and law; overlap from the FP64 distance.
-                // Contacts it cannot decide like FP64 fall through to the FP64 law below.
-                const double gap = Li - ri - rj;
-                const double dn = fabs(gap);
-                const f3 d32 = tof(kj - ki);
-                const f3 a32 = fscl(d32, frcp((float)Li));
-                // near a = -x the reference's frame is ill-conditioned (its rotation about the
-                // normal follows the normal's tiny transverse components): FP64 there
-                bool fp64 = a32.x < -0.99f;
-                const fm3 R32 = contact_frame_f32(a32, 1e-6f);
-                const float lever = (float)(ri - dn);
-                const f3 kip = fscl(a32, lever);
-                const f3 kjp = fsub(kip, d32);
-                // (w_j x k_j' + v_j) - (w_i x k_i' + v_i) with v_j - v_i taken in FP64: grains in
-                // contact share most of their velocity, and subtracting FP32 copies of two fast
-                // grains' velocities would lose the small relative part
-                const f3 dv = tof(vj - vi);
-                const f3 vc = fmul(R32, fadd(fsub(fcross(tof(wj), kjp), fcross(tof(wi), kip)), dv));
-                const float Rs = frcp(frcp((float)ri) + frcp((float)rj));
-                const float ms = frcp(frcp((float)mi) + frcp((float)mj));
-                const float srd = fsqrt(Rs * (float)dn);
-                const float kn = (4.0f / 3.0f) * S.ys_pp * srd;
-                const float kt = 8.0f * S.gs_pp * srd;
-                const float gn = -2.0f * S.beta_pp * fsqrt((5.0f / 6.0f) * (2.0f * S.ys_pp * srd) * ms);
-                const float gt = -2.0f * S.beta_pp * fsqrt((5.0f / 6.0f) * kt * ms);
-                // the tangential spring accumulates in FP64: v dt increments vanish in FP32
-                d3 sv = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
-                sv = sv + mk3((double)vc.x * 0.0, (double)vc.y * dt, (double)vc.z * dt);
-                const float F1 = -kn * (float)gap - gn * vc.x;
-                const f3 Ft = fscl(tof(sv), -kt);
-                const float nFt = fnorm(Ft);
-                // a slip decision within FP32 rounding of the threshold is taken in FP64: the
-                // reference's slip rule reverses the stored spring, so a flipped decision is an
-                // O(1) change of the tangential force
-                const float thr = S.mu_pp * F1;
-                fp64 = fp64 || fabsf(nFt - thr) <= 1e-4f * fmaxf(fabsf(nFt), fabsf(thr));
-                float F2, F3;
-                if (fp64) {
-                    F2 = F3 = 0.0f;
-                } else if (nFt >= thr) {
-                    const float ratio = S.mu_pp * F1 * frcp(nFt);
-                    F2 = ratio * Ft.y;
-                    F3 = ratio * Ft.z;
-                    const float ikt = frcp(kt);
-                    sv = mk3(sv.x, (double)(F2 * ikt), (double)(F3 * ikt));
-                } else {
-                    F2 = Ft.y - gt * vc.y;
-                    F3 = Ft.z - gt * vc.z;
-                }
-                if (!fp64) {
-                const f3 Fl = mkf(F1, F2, F3);
-                const f3 Fg = fmul_t(R32, mkf(-F1, -F2, -F3));
-                if (plo) { FH.add(Fg); TH.add(fcross(kip, Fg)); }
-                else { FH.add(mkf(-Fg.x, -Fg.y, -Fg.z)); TH.add(fcross(kjp, mkf(-Fg.x, -Fg.y, -Fg.z))); }
-                const d3 kc = ki + tod(kip);
-                if (write_forces) {  // contact point and local force are outputs, not state
-                    st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
-                    st[S_FX * cs + e] = (double)Fl.x; st[S_FY * cs + e] = (double)Fl.y; st[S_FZ * cs + e] = (double)Fl.z;
-                }
-                st[S_SX * cs + e] = sv.x; st[S_SY * cs + e] = sv.y; st[S_SZ * cs + e] = sv.z;
-                if (plo) {
-                    ++active;
-                    if (gap > 0 || gn < 0.0f || gt < 0.0f) ++fatal;
+                    ++broken;
+                    if (SEG) atomicAdd(&P.seg_broken[P.sid[p]], 1ull);
                 }
                 continue;
-                }
-              }
-              {
-                // Hertz-Mindlin (kernels.jl:240-313)
-                const m3 R = contact_frame(normalize(kj - ki), S.tol);
-                const double gap = Li - ri - rj;
-                const double dn = fabs(gap);
-                const d3 kc = ki + normalize(kj - ki) * (ri - dn);
-                const d3 kip = kc - ki, kjp = kc - kj;
-                const d3 vci = cross(wi, kip) + vi;
-                const d3 vcj = cross(wj, kjp) + vj;
-                const d3 vc = mul(R, vcj - vci);
-                const SurfaceP& sf = S.surf[0][0];
-                // 1/r_i + 1/r_j with this particle's reciprocal taken once per row (IEEE addition
-                // commutes, so the sum is the canonical one bit for bit)
-                const double Rs = 1.0 / (inv_r + 1.0 / (plo ? rj : ri));
-                const double ms = 1.0 / (inv_m + 1.0 / (plo ? mj : mi));
-                const d3 s0 = mk3(st[S_SX * cs + e], st[S_SY * cs + e], st[S_SZ * cs + e]);
-                const HMOut h = hertz_mindlin(gap, dn, vc, S.ysm[0], S.gsm[0], Rs, ms, sf, dt, s0);
-                const m3 Ri = inverse(R);
-                const d3 Fg = mul(Ri, -h.F);
-                if (plo) { F = F + Fg; T = T + cross(kip, Fg); }
-                else { F = F + (-Fg); T = T + cross(kjp, -Fg); }
-                if (write_forces) {  // contact point and local force are outputs, not state
-                    st[S_KX * cs + e] = kc.x; st[S_KY * cs + e] = kc.y; st[S_KZ * cs + e] = kc.z;
-                    st[S_FX * cs + e] = h.F.x; st[S_FY * cs + e] = h.F.y; st[S_FZ * cs + e] = h.F.z;
-                }
-                st[S_SX * cs + e] = h.s.x; st[S_SY * cs + e] = h.s.y; st[S_SZ * cs + e] = h.s.z;
-                if (plo) {
-                    ++active;
-                    if (gap > 0 || h.fatal) ++fatal;
-                }
-              }
+            }
+            if (o.kind == CK_FP32) {
+                if (plo) { FH.add(tof(o.F)); TH.add(tof(o.Ti)); }
+                else { FH.add(tof(-o.F)); TH.add(tof(o.Tj)); }
+            } else {
+                if (plo) { F = F + o.F; T = T + o.Ti; }
+                else { F = F + (-o.F); T = T + o.Tj; }
+            }
+            if (plo) {
+                ++active;
+                if (o.bonded) ++active_b;
+                if (o.fatal) ++fatal;
             }
         }
         if (MIXED) {
