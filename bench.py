@@ -121,11 +121,12 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
+def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True, near_lists=True):
     from taichi_dem_julia_port_amd.executor import Batch, Scene
-    progress(f"engine run: {precision}, {steps} steps after {warmup} warm-up")
-    s = (Batch(cfg, precision=precision, graph=graph) if g is None
-         else Scene(cfg, g, w, precision=precision, graph=graph))
+    progress(f"engine run: {precision}, {steps} steps after {warmup} warm-up"
+             f"{'' if near_lists else ', sort and broad phase on every step'}")
+    s = (Batch(cfg, precision=precision, graph=graph, near_lists=near_lists) if g is None
+         else Scene(cfg, g, w, precision=precision, graph=graph, near_lists=near_lists))
     if bonded:
         s.initialize()
     s.step(warmup)
@@ -141,11 +142,13 @@ def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
     t1 = time.perf_counter()
     times = s.kernel_times()
     s.set_timing(False)
+    rebuilds0 = s.counters()["rebuilds"]
     # timed region: the next K steps on the production path (graph replays), state in HBM;
     # dem_step returns after a stream synchronisation
     t2 = time.perf_counter()
     s.step(steps)
     t3 = time.perf_counter()
+    rebuilds = s.counters()["rebuilds"] - rebuilds0
     # one more step, untimed, with the diagnostic passes: the reference's cp_list length and the
     # candidates the broad phase examined
     s.set_diagnostics(True)
@@ -154,7 +157,7 @@ def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True):
     d = s.derived()
     s.close()
     return {"elapsed": t3 - t2, "elapsed_events": t1 - t0, "kernels": times, "counters": c,
-            "derived": d}
+            "derived": d, "rebuilds": rebuilds}
 
 
 def precision_error(cfg, g, w, bonded, steps=1000):
@@ -313,7 +316,8 @@ def summarize(r, workload, precision, nwall, n, steps):
             "kernel_ms_per_launch": {k: round(v, 5) for k, v in step_ms.items()},
             "kernel_share": {k: round(ms / total_kernel_ms, 4) for k, (ms, l) in kt.items() if l},
             "ms_per_step_with_kernel_events": round(1e3 * r["elapsed_events"] / steps, 5),
-            "roofline": roofline, "kernel_GB_s": per_kernel}
+            "roofline": roofline, "kernel_GB_s": per_kernel,
+            "rebuilds_in_timed_steps": r["rebuilds"]}
 
 
 def main():
@@ -339,6 +343,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graph", action="store_true",
                     help="launch every step eagerly (no hipGraph replay)")
+    ap.add_argument("--no-near-lists", action="store_true",
+                    help="sort and run the broad phase on every step (no near lists)")
+    ap.add_argument("--no-every-step", action="store_true",
+                    help="skip timing the every-step sort and broad phase path beside the "
+                         "near-list one")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="first run a plain copy kernel of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -366,8 +375,9 @@ def main():
     n = len(g) if g is not None else sum(len(m[1]) for m in cfg)
     if g is None:
         w = cfg[0][2]
+    near = not args.no_near_lists
     r = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup,
-                   graph=not args.no_graph)
+                   graph=not args.no_graph, near_lists=near)
     m = summarize(r, args.workload, args.precision, len(w), n, args.steps)
     c = r["counters"]
     out = {
@@ -393,9 +403,20 @@ def main():
         "kernel_share": m["kernel_share"],
         "ms_per_step_with_kernel_events": m["ms_per_step_with_kernel_events"],
         "graph": not args.no_graph,
+        "near_lists": near,
+        "rebuilds_in_timed_steps": m["rebuilds_in_timed_steps"],
         "roofline": m["roofline"],
         "kernel_GB_s": m["kernel_GB_s"],
     }
+    if near and not args.no_every_step:
+        # the same run with the sort and the broad phase on every step: a step that rebuilds the
+        # near lists costs about this much
+        r0 = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup,
+                        graph=not args.no_graph, near_lists=False)
+        m0 = summarize(r0, args.workload, args.precision, len(w), n, args.steps)
+        out["every_step_rebuild"] = {"value": round(m0["value"], 1), "ms_per_step": m0["ms_per_step"],
+                                     "kernel_ms_per_launch": m0["kernel_ms_per_launch"],
+                                     "near_list_speedup": round(m["value"] / m0["value"], 3)}
     if g is None:
         out["scenes"] = len(cfg)
         out["scene_steps_per_s"] = round(len(cfg) * args.steps / r["elapsed"], 1)
@@ -405,12 +426,13 @@ def main():
     if not args.no_compare_precision:
         other = "mixed" if args.precision == "fp64" else "fp64"
         r2 = run_engine(cfg, g, w, bonded, other, args.steps, args.warmup,
-                        graph=not args.no_graph)
+                        graph=not args.no_graph, near_lists=near)
         m2 = summarize(r2, args.workload, other, len(w), n, args.steps)
         out[f"value_{other}"] = round(m2["value"], 1)
         out[f"ms_per_step_{other}"] = m2["ms_per_step"]
         out[f"{other}_speedup_over_{args.precision}"] = round(m2["value"] / m["value"], 3)
         out[f"kernel_ms_per_launch_{other}"] = m2["kernel_ms_per_launch"]
+        out[f"rebuilds_in_timed_steps_{other}"] = m2["rebuilds_in_timed_steps"]
         out["position_error_after_1000_steps"] = precision_error(cfg, g, w, bonded)
     if args.workload == "pour" and not args.no_contact_heavy:
         # the same grains in their contact-heavy state (the pour has no contacts for its first
@@ -420,11 +442,12 @@ def main():
         heavy = {"workload": desc_p}
         for prec in ("fp64", "mixed"):
             rp = run_engine(cfg_p, g_p, w_p, False, prec, args.steps, args.warmup,
-                            graph=not args.no_graph)
+                            graph=not args.no_graph, near_lists=near)
             mp = summarize(rp, "pour_packed", prec, len(w_p), len(g_p), args.steps)
             heavy[prec] = {"value": round(mp["value"], 1), "ms_per_step": mp["ms_per_step"],
                            "kernel_ms_per_launch": mp["kernel_ms_per_launch"],
-                           "roofline": mp["roofline"]}
+                           "roofline": mp["roofline"],
+                           "rebuilds_in_timed_steps": mp["rebuilds_in_timed_steps"]}
             heavy["counters"] = {k: rp["counters"][k] for k in ("contacts", "hertz",
                                                                 "wall_contacts", "examined",
                                                                 "candidate_pairs")}

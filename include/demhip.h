@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define DEM_ABI_VERSION 1
+#define DEM_ABI_VERSION 2
 
 enum dem_status {
     DEM_OK = 0,
@@ -134,6 +134,11 @@ typedef struct dem_options {
 #define DEM_FLAG_REFERENCE_GRID 2
 /* Launch every step's kernels individually instead of replaying a captured two-step HIP graph. */
 #define DEM_FLAG_NO_GRAPH 4
+/* Sort and run the broad phase on every step.  By default (on the dense grid) a step that sorts
+ * also keeps each particle's near list, the partners within r_i + r_j + 2 delta
+ * (delta = 0.05 max r), and later steps test only those lists until some particle has moved
+ * delta from where the lists were built; the contact sets are the same either way. */
+#define DEM_FLAG_NO_NEAR_LISTS 8
 
 /* Per-step counters (values of the last step unless marked cumulative). */
 typedef struct dem_counters {
@@ -155,6 +160,8 @@ typedef struct dem_counters {
     int64_t examined;          /* candidate pairs examined by the engine's own broad phase;
                                   -1 unless the diagnostic passes ran */
     int64_t grid_cells;        /* cells of the engine's dense sort grid (0: reference grid used) */
+    int64_t rebuilds;          /* cumulative steps that sorted and ran the broad phase (every step
+                                  without near lists) */
 } dem_counters_t;
 
 typedef struct dem_scene dem_scene_t;

@@ -98,13 +98,14 @@ class DemOptions(ctypes.Structure):
 DEM_FLAG_INDEX_OWNED_BROAD_PHASE = 1
 DEM_FLAG_REFERENCE_GRID = 2
 DEM_FLAG_NO_GRAPH = 4
+DEM_FLAG_NO_NEAR_LISTS = 8
 
 
 class DemCounters(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "step", "n", "hash_table_size", "candidate_pairs", "contacts", "bonded", "hertz",
         "wall_contacts", "created", "deactivated", "bonds_broken", "bonds_broken_total",
-        "fatal_gap", "overflow", "examined", "grid_cells")]
+        "fatal_gap", "overflow", "examined", "grid_cells", "rebuilds")]
 
 
 DEM_NUM_KERNELS = 8
@@ -201,7 +202,7 @@ def load_library(path: str = LIB_PATH) -> ctypes.CDLL:
         fn = getattr(lib, name)
         fn.restype = res
         fn.argtypes = args
-    if lib.dem_abi_version() != 1:
+    if lib.dem_abi_version() != 2:
         raise ImportError("libdemhip ABI version mismatch")
     _LIB = lib
     return lib

@@ -147,6 +147,22 @@ hipError_t alloc_rows(dem_scene* s, int32_t K) {
     return hipMemset(d.rows_arena, 0, bytes);
 }
 
+// The near lists of KN slots per particle (nothing when the scene does not use them).  Entries
+// are written by a rebuilding step's broad phase before any step reads them, so no fill.
+hipError_t alloc_near(dem_scene* s, int32_t KN) {
+    DevState& d = s->d;
+    d.nb = nullptr;
+    if (!s->P.skin) return hipSuccess;
+    hipError_t e = block_alloc(s->device, &d.nb_arena, (size_t)KN * s->n * sizeof(uint32_t),
+                               &d.nb_arena_bytes);
+    if (e != hipSuccess) return e;
+    d.nb = (uint32_t*)d.nb_arena;
+    return hipSuccess;
+}
+
+// Near-list slots per particle at scene creation: the contact slots, at least 16.
+int32_t near_slots(int32_t K) { return std::max(16, K); }
+
 // Copies between host memory and a scene's device arrays, on the scene's stream and complete on
 // return: ordered after every launch on that stream, and finished before the next one.
 hipError_t to_host(const dem_scene* s, void* dst, const void* src, size_t bytes) {
@@ -176,6 +192,7 @@ void free_scene(dem_scene* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     block_free(s->device, d.arena, d.arena_bytes);
     block_free(s->device, d.rows_arena, d.rows_arena_bytes);
+    block_free(s->device, d.nb_arena, d.nb_arena_bytes);
     if (s->h_ctr) hipHostFree(s->h_ctr);
     if (s->h_ctr_raw) hipHostFree(s->h_ctr_raw);
     for (hipEvent_t e : s->ev)
@@ -266,6 +283,28 @@ int32_t widen_rows(dem_scene* s, int32_t k2) {
     return DEM_OK;
 }
 
+// Widen the near lists to hold `need` entries per particle (at least twice as many slots, at most
+// MAX_NEAR_SLOTS), after a rebuilding step found a longer list than they hold.  Their contents
+// are not kept: the next step rebuilds them.  Past MAX_NEAR_SLOTS every step keeps rebuilding
+// (a rebuilding step finds the contacts by itself, so results do not change).
+constexpr int32_t MAX_NEAR_SLOTS = 512;
+int32_t widen_near(dem_scene* s, unsigned long long need) {
+    DevState& d = s->d;
+    const int64_t want = std::min<int64_t>(
+        std::max<int64_t>(2 * (int64_t)s->P.KN, ((int64_t)need + 7) & ~(int64_t)7), MAX_NEAR_SLOTS);
+    if (want <= s->P.KN) return DEM_OK;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    block_free(s->device, d.nb_arena, d.nb_arena_bytes);
+    d.nb_arena = nullptr;
+    d.nb_arena_bytes = 0;
+    s->P.KN = (int32_t)want;
+    hipError_t e = alloc_near(s, s->P.KN);
+    if (e != hipSuccess) return hip_fail(s, "widening the near lists", e);
+    drop_graphs(s);
+    s->skin_dirty = true;
+    return DEM_OK;
+}
+
 // The next row width after an overflow that needed `need` slots: at least twice the current one.
 int32_t wider_slots(const dem_scene* s, unsigned long long need) {
     const int64_t want = std::max<int64_t>(2 * (int64_t)s->K, ((int64_t)need + 7) & ~(int64_t)7);
@@ -308,6 +347,9 @@ int32_t check_scene(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nw
     return DEM_OK;
 }
 
+// Near-list skin delta as a fraction of the largest radius.
+constexpr double NEAR_SKIN = 0.05;
+
 // The setup half of solve() for one scene (executor.jl:219-315): per-grain derivations into the
 // host arrays at [off, off + n), sizing, and the kernel parameters.  want_dense selects the
 // engine's dense sort grid; force_dense widens its cells instead of falling back to the
@@ -348,6 +390,8 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
                  !(flags & DEM_FLAG_INDEX_OWNED_BROAD_PHASE);
 
     memset(&S, 0, sizeof(S));
+    // near-list skin delta: the lists hold the partners within r_i + r_j + 2 delta
+    S.skin = (flags & DEM_FLAG_NO_NEAR_LISTS) ? 0.0 : NEAR_SKIN * rmax_r;
     // A touching (or bond-distance) pair lies in both particles' reference windows when the
     // contact reach is below cs/2 with margin; otherwise the broad phase checks the rule per pair.
     S.ref_check = reach * (1.0 + 1e-6) < 0.5 * der.cell_size ? 0 : 1;
@@ -391,7 +435,7 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
     }
     // Dense sort/broad-phase grids covering the domain with at most max(4 n, 2^20) cells
     // (batches widen the cells to at most 8 n + 4096): the per-step grid has cells of at least
-    // the touching reach 2 max r, the bond pass's of at least the bond reach 2 max r0.
+    // the near-list reach 2 max r + 2 delta, the bond pass's of at least the bond reach 2 max r0.
     S.dense.cells = 0;
     S.dense_init.cells = 0;
     auto make_grid = [&](double cell, DenseGrid& G) -> bool {
@@ -421,12 +465,14 @@ int32_t derive_scene(dem_scene* s, int64_t off, const dem_config_t* cfg, const d
         return false;
     };
     if (want_dense) {
-        const double touch = 2.0 * rmax_r * (1.0 + 1e-6);
+        const double touch = (2.0 * rmax_r + 2.0 * S.skin) * (1.0 + 1e-6);
         const bool ok = make_grid(touch, S.dense) && make_grid(reach * (1.0 + 1e-6), S.dense_init);
         if (!ok) {
             S.dense.cells = 0;
             S.dense_init.cells = 0;
             if (force_dense) return fail(nullptr, DEM_ERR_INVALID, "scene domain cannot be gridded");
+        } else if (!make_grid(2.0 * rmax_r * (1.0 + 1e-6), S.order)) {
+            S.order = S.dense;
         }
     }
     return DEM_OK;
@@ -449,7 +495,7 @@ void spatial_order(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase, int
         const SceneP& S = s->batched ? s->seg_host[k] : s->P.sc;
         if (S.dense.cells > 0) {
             // counting sort by dense cell (stable: file order within a cell)
-            const DenseGrid& G = S.dense;
+            const DenseGrid& G = S.order;
             key.assign(b - a, 0);
             cnt.assign((size_t)G.cells + 1, 0u);
             for (int64_t i = a; i < b; ++i) {
@@ -510,6 +556,7 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase,
         c.take(&d.bstart, nkeys + 1); c.take(&d.sidxb[0], n); c.take(&d.sidxb[1], n); c.take(&d.skey, n);
         c.take(&d.spos, n); c.take(&d.spos32, n); c.take(&d.eid, n); c.take(&d.sfid, n);
         c.take(&d.tcnt, n); c.take(&d.sel, n);
+        c.take(&d.nbc, n); c.take(&d.xref, n); c.take(&d.gmin, n);
         c.take(&d.ctr, CTR_WORDS);
         c.take(&d.scan.status, max_tiles); c.take(&d.scan.counters, 2);
         c.take(&d.scan.tagg, max_tiles * TAGG_COPIES);
@@ -526,6 +573,7 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase,
     d.scan.max_tiles = max_tiles;
     e = hipMemset(d.arena, 0, bytes);
     if (e == hipSuccess) e = alloc_rows(s, s->K);
+    if (e == hipSuccess) e = alloc_near(s, s->P.KN);
     if (e != hipSuccess)
         return fail(nullptr, DEM_ERR_HIP, std::string("device allocation failed: ") + hipGetErrorString(e));
     if (hipHostMalloc((void**)&s->h_ctr, sizeof(unsigned long long) * CT_NCOUNTERS) != hipSuccess)
@@ -636,6 +684,8 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     P.K = K;
     P.nseg = 1;
     s->dense = P.sc.dense.cells > 0;
+    P.skin = s->dense && !(o.flags & DEM_FLAG_NO_NEAR_LISTS) ? 1 : 0;
+    P.KN = near_slots(K);
     P.sort_keys = s->dense ? P.sc.dense.cells : (uint32_t)P.sc.grid.H;
     P.sort_keys_init = s->dense ? P.sc.dense_init.cells : (uint32_t)P.sc.grid.H;
     const int64_t nkeys = std::max<int64_t>(
@@ -728,6 +778,8 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
     P.nseg = nscenes;
     P.sc = s->seg_host[0];
     s->dense = true;
+    P.skin = (o.flags & DEM_FLAG_NO_NEAR_LISTS) ? 0 : 1;
+    P.KN = near_slots(K);
     P.sort_keys = (uint32_t)keys;
     P.sort_keys_init = (uint32_t)keys_init;
     if ((rc = allocate_scene(s, grains, 0, 0, nwall_max, (int64_t)std::max(keys, keys_init)))) {
@@ -790,6 +842,7 @@ int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, i
     if (e == hipSuccess) e = to_host(dst, &o, ovf, sizeof(uint32_t));
     hipFree(ovf);
     if (e != hipSuccess) return hip_fail(dst, "dem_batch_adopt", e);
+    dst->skin_dirty = true;  // the destination scene's particles were replaced
     if (o) return fail(dst, DEM_ERR_CAPACITY, "destination has fewer contact slots than needed");
     // cumulative bond breaks of the scene travel with it
     unsigned long long nb = 0;
@@ -946,6 +999,8 @@ int32_t dem_batch_repack(dem_scene_t* src, int32_t nkeep, const int32_t* keep, i
     P.nseg = nscenes;
     P.sc = s->seg_host[0];
     s->dense = true;
+    P.skin = (o.flags & DEM_FLAG_NO_NEAR_LISTS) ? 0 : 1;
+    P.KN = near_slots(K);
     P.sort_keys = (uint32_t)keys;
     P.sort_keys_init = (uint32_t)keys_init;
     if ((rc = allocate_scene(s, grains, kept_total, nkeep, nwall_max, (int64_t)std::max(keys, keys_init)))) {
@@ -1076,10 +1131,19 @@ static int32_t capture_graph(dem_scene* s) {
     return DEM_OK;
 }
 
+// Make the next step (buffer parity cur) rebuild the sort, the broad phase and the near lists.
+static hipError_t request_rebuild(dem_scene* s) {
+    return hipMemsetAsync(s->d.ctr + CT_SKIN + s->cur, 0xff, sizeof(unsigned long long), s->stream);
+}
+
 // Enqueue n steps: pairs of steps replayed from the captured graph of the current buffer parity
 // (not while timing individual kernels), the last step eager with its outputs and counters.
 static int32_t enqueue_steps(dem_scene* s, int64_t nsteps) {
     int64_t t = 0;
+    if (s->P.skin && s->skin_dirty && nsteps > 0) {
+        HIPCHK(s, request_rebuild(s));
+        s->skin_dirty = false;
+    }
     if (s->use_graph && !s->timing && nsteps >= 3) {
         const int b = s->cur;  // a graph of two steps ends at the parity it starts from
         if (!s->graph_exec[b] || s->graph_force_tpb[b] != s->force_tpb) {
@@ -1092,6 +1156,8 @@ static int32_t enqueue_steps(dem_scene* s, int64_t nsteps) {
         }
     }
     for (; t < nsteps; ++t) {
+        // the diagnostic passes read this step's sort
+        if (s->P.skin && s->diagnostics && t == nsteps - 1) HIPCHK(s, request_rebuild(s));
         HIPCHK(s, run_contact_detection(s, false, t == nsteps - 1));
         HIPCHK(s, run_force_integrate(s, t == nsteps - 1));
         s->steps++;
@@ -1118,6 +1184,7 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
         s->steps = steps0 + done;
         s->cur = (done & 1) ? 1 - cur0 : cur0;
         left -= done;
+        s->skin_dirty = true;  // the failed steps' broad phases may have rewritten the near lists
         HIPCHK(s, run_rows_finish(s, true));
         if (s->K >= MAX_SLOTS) {
             HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -1126,6 +1193,14 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
                         "completed step)");
         }
         if ((rc = widen_rows(s, wider_slots(s, s->h_ctr[CT_MAXROW])))) return rc;
+    }
+    if (s->P.skin && s->h_ctr[CT_NBNEED]) {
+        // a near list overflowed (the steps after it rebuilt): widen them for the next call
+        const unsigned long long need = s->h_ctr[CT_NBNEED];
+        HIPCHK(s, hipMemsetAsync(s->d.ctr + CT_NBNEED, 0, sizeof(unsigned long long), s->stream));
+        s->h_ctr[CT_NBNEED] = 0;
+        int32_t rc = widen_near(s, need);
+        if (rc) return rc;
     }
     if (nsteps > 0) choose_force_blocks(s);
     return DEM_OK;
@@ -1166,6 +1241,7 @@ int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
     out->overflow = (int64_t)c[CT_OVERFLOW];
     out->examined = s->diag_valid ? (int64_t)c[CT_EXAMINED] : -1;
     out->grid_cells = s->dense ? (int64_t)s->P.sort_keys : 0;
+    out->rebuilds = s->P.skin ? (int64_t)c[CT_REBUILDS] : s->steps;
     return DEM_OK;
 }
 
@@ -1222,6 +1298,7 @@ int32_t dem_write_grains(dem_scene_t* s, const dem_grain_t* in, int64_t n) {
     HIPCHK(s, to_device(s, d.quat[c], q.data(), n * sizeof(double4)));
     HIPCHK(s, to_device(s, d.acc, a.data(), n * sizeof(double4)));
     HIPCHK(s, to_device(s, d.dw, dw.data(), n * sizeof(double4)));
+    s->skin_dirty = true;  // the positions moved outside a step
     return DEM_OK;
 }
 

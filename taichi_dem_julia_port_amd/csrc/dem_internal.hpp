@@ -41,7 +41,12 @@ enum CounterIdx {
     CT_DONE = 14,          // cumulative: steps whose force kernel ran (copy 0 only)
     CT_FAIL = 15,          // a row overflowed: every later step of the call does nothing (copy 0)
     CT_MAXROW = 16,        // the longest row an overflowing particle needed (copy 0, maximum)
-    CT_NCOUNTERS = 20
+    CT_REBUILDS = 17,      // cumulative: completed steps that rebuilt the sort and near lists (copy 0)
+    CT_NBNEED = 18,        // the longest near list a particle needed when one overflowed (copy 0)
+    CT_SKIN = 20,          // CT_SKIN + c: the step of buffer parity c rebuilds (copy 0; two words)
+    CT_NCOUNTERS = 22,     // the host sums copies of the counters below this index
+    CT_DISP = 22           // CT_DISP + c: FP32 bits of a bound on every particle's displacement since
+                           // the last rebuild, for the step of parity c (the maximum over all copies)
 };
 
 // Counters are kept in CTR_COPIES copies, 256 B apart, and a block adds to copy blockIdx % CTR_COPIES:
@@ -73,6 +78,27 @@ __device__ __forceinline__ void fail_step(unsigned long long* ctr, uint32_t need
     atomicOr(&ctr[CT_FAIL], 1ull);
 }
 
+// Near lists (a Verlet skin).  A step that rebuilds (sorts, runs the broad phase) also records,
+// for every particle, the partners within r_i + r_j + 2 delta of it, and the force kernel keeps
+// each particle's position at that step.  While no particle has moved delta or more from it, a
+// pair that touches is in both partners' near lists (|x_i - x_j| can only have shrunk by less
+// than 2 delta), so the following steps skip the sort and the broad phase and test only the
+// near lists, with the broad phase's exact test: the touching set is the same.  The flag word
+// CT_SKIN + c says whether the step of buffer parity c rebuilds; the step of parity c clears
+// the other parity's word in its first kernel, and its broad phase (when a near list
+// overflows) and force kernel (when a particle has moved too far) set it.
+__device__ __forceinline__ bool skin_rebuilds(const unsigned long long* ctr, int cur) {
+    return ctr[CT_SKIN + cur] != 0ull;
+}
+__device__ __forceinline__ void skin_request(unsigned long long* ctr, int cur) {
+    __hip_atomic_store(&ctr[CT_SKIN + (1 - cur)], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A particle whose nearest near-list partner was a gap g >= 0 away at the rebuild (gmin) touches
+// none of them while g > 2 D, D bounding every particle's displacement since then: the gap of a
+// pair shrinks by at most the sum of its grains' displacements.  The force kernel keeps D for the
+// next step as the maximum of per-wave maxima over CTR_COPIES copies (non-negative FP32 bits
+// order like unsigned integers).
+
 struct SurfaceP {
     double mu, e, beta;          // friction, restitution, log(e)/sqrt(log(e)^2 + pi^2)
     double rho, E, nu;           // bond radius ratio, modulus, Poisson ratio
@@ -87,11 +113,14 @@ struct SceneP {
     DenseGrid dense;    // dense grid of the per-step sort/broad phase, cells >= 2 max r
                         // (dense.cells == 0: unused)
     DenseGrid dense_init;    // dense grid of the bond pass, cells >= the bond reach
+    DenseGrid order;    // host only: the grid of cells >= 2 max r whose cells order the particles
+                        // in memory at creation (the same with and without near lists)
     uint32_t cell_base;      // first global sort key of this scene's dense cells
     uint32_t cell_base_init; // the same for dense_init
     int32_t nwall;
     int32_t bond_init;  // scene takes part in dem_initialize
     int32_t ref_check;  // touching pairs must be checked against the reference window rule
+    double skin;        // near-list skin delta (the per-step dense grid's cells are >= 2 max r + 2 delta)
     double bond_mult;   // particle_contact_radius_multiplier (r0 = r * bond_mult)
     double dt, tol, damping;
     double g[3];
@@ -111,6 +140,8 @@ struct Params {
     int64_t n;
     int32_t K;            // contact slots per particle
     int32_t nseg;         // scenes (1 for a single scene)
+    int32_t skin;         // near lists in use (dense grid only)
+    int32_t KN;           // near-list slots per particle
     uint32_t sort_keys;   // keys of the per-step sort (dense cells of all scenes, or H)
     uint32_t sort_keys_init;  // keys of the bond-pass sort
     SceneP sc;
@@ -143,8 +174,10 @@ int scan_tile_shift(int64_t n);     // log2 of the tile length a scan of n value
 hipError_t launch_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n, bool zero_input,
                        uint64_t* total, hipStream_t stream);
 // Exclusive scan of n counts whose per-tile totals are in ws.tagg; clears the counts.
+// gate: when not null, the scan does nothing unless *gate is nonzero (a step that reuses the
+// near lists skips its sort).
 hipError_t launch_tile_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n,
-                            hipStream_t stream);
+                            hipStream_t stream, const unsigned long long* gate = nullptr);
 
 // Device arrays of one scene.
 struct DevState {
@@ -178,6 +211,12 @@ struct DevState {
     // broad phase output
     uint32_t* tcnt = nullptr;    // [n]
     uint32_t* touch = nullptr;   // [K][n]
+    // near lists: partner | IS_I words of every pair within r_i + r_j + 2 delta at the last
+    // rebuild, and each particle's position then (FP32)
+    uint32_t* nbc = nullptr;     // [n] entries (may exceed KN after an overflow)
+    uint32_t* nb = nullptr;      // [KN][n]
+    float4* xref = nullptr;      // [n]
+    uint32_t* gmin = nullptr;    // [n] FP32 bits: the smallest near-list gap at the rebuild (>= 0)
     // contact rows, double-buffered per particle (sel[p])
     uint8_t* sel = nullptr;
     uint32_t* rcnt[2] = {nullptr, nullptr};
@@ -198,7 +237,8 @@ struct DevState {
     // change size when rows are widened, and everything else
     void* arena = nullptr;
     void* rows_arena = nullptr;
-    size_t arena_bytes = 0, rows_arena_bytes = 0;
+    void* nb_arena = nullptr;  // the near lists nb, which change size when they widen
+    size_t arena_bytes = 0, rows_arena_bytes = 0, nb_arena_bytes = 0;
 };
 
 }  // namespace dem
@@ -258,6 +298,9 @@ struct dem_scene {
     bool diag_valid = false;  // the last counted step ran them
     // steps the device has completed (its CT_DONE counter) as of the last counter read
     unsigned long long done_count = 0;
+    // the near lists may not hold the current positions' partners (scene creation, a host write,
+    // a replayed step): the next step rebuilds
+    bool skin_dirty = true;
     bool use_graph = true;
     // timing: an event pair around every launch, collected when the pool fills or on request
     bool timing = false;

@@ -70,10 +70,21 @@ __global__ void __launch_bounds__(TPB)
 k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ sidx_prev,
        uint32_t* __restrict__ hid, uint32_t* __restrict__ rank, float4* __restrict__ x32,
        uint32_t* __restrict__ bcnt, uint32_t* __restrict__ tagg, int tile_shift,
-       unsigned long long* ctr, int reset_counters, int init) {
-    if (reset_counters && blockIdx.x == 0)
+       unsigned long long* ctr, int reset_counters, int init, uint32_t* __restrict__ nbc,
+       uint32_t* __restrict__ gmin, int skin_cur) {
+    if (reset_counters && blockIdx.x == 0) {
         for (int i = threadIdx.x; i < CTR_COPIES * CT_NSTEP; i += TPB)
             ctr[(i / CT_NSTEP) * CTR_STRIDE + i % CT_NSTEP] = 0ull;
+        if (skin_cur >= 0) {
+            // the next step's rebuild flag and displacement bound, set by this step's broad
+            // phase and force kernel
+            if (threadIdx.x == 0) ctr[CT_SKIN + 1 - skin_cur] = 0ull;
+            for (int c = threadIdx.x; c < CTR_COPIES; c += TPB)
+                ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
+        }
+    }
+    // near lists in use (skin_cur = the buffer parity): a step that reuses them does not sort
+    if (skin_cur >= 0 && !skin_rebuilds(ctr, skin_cur)) return;
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     const bool live = t < P.n;
     uint32_t b = 0xffffffffu;
@@ -81,6 +92,10 @@ k_hash(const Params P, const double4* __restrict__ pos, const uint32_t* __restri
     if (live) {
         const uint32_t p = sidx_prev[t];
         x = pos[p];
+        if (skin_cur >= 0) {  // this step's broad phase refills the near lists
+            nbc[p] = 0u;
+            gmin[p] = 0x7f800000u;  // +inf
+        }
         const SceneP& S = scene_of<SEG>(P, p);
         int32_t c[3];
         b = DENSE ? (init ? S.cell_base_init + dense_cell(S.dense_init, mk3(x.x, x.y, x.z), c)
@@ -150,7 +165,8 @@ k_scatter(const Params P, const double4* __restrict__ pos, const uint32_t* __res
           const float4* __restrict__ x32, const uint32_t* __restrict__ bstart,
           uint32_t* __restrict__ sidx, uint32_t* __restrict__ skey, double4* __restrict__ spos,
           float4* __restrict__ spos32, const uint32_t* __restrict__ eid, uint32_t* __restrict__ sfid,
-          uint32_t* __restrict__ tagg, int ntiles) {
+          uint32_t* __restrict__ tagg, int ntiles, const unsigned long long* __restrict__ gate) {
+    if (gate && *gate == 0ull) return;  // the step reuses the near lists: no sort
     // the scan that read the histogram's tile totals has finished: clear them for the next sort
     if (blockIdx.x == 0)
         for (int i = threadIdx.x; i < ntiles * TAGG_COPIES; i += TPB) tagg[i] = 0u;
@@ -336,13 +352,14 @@ k_broad_slot(const Params P, const uint32_t* __restrict__ sidx, const double4* _
 // holds no partner: every particle of cell x - 1 lies below this cell's lower x face, and so on.
 // Per row the perpendicular gap is combined with the x gap.  The gaps are FP32 with a margin over
 // their rounding; a particle clamped into a boundary cell only has larger gaps to the cells it is
-// clamped away from.
+// clamped away from.  `extra` widens the reach by the near lists' 2 delta (the cells are then at
+// least 2 max r + 2 delta).
 __device__ __forceinline__ void broad_ranges(const DenseGrid& G, uint32_t cbase, const int32_t c[3],
-                                             float4 me32, double mult, int64_t t, bool work,
-                                             const uint32_t* __restrict__ bstart, uint32_t lo[5],
-                                             uint32_t len[5]) {
+                                             float4 me32, double mult, float extra, int64_t t,
+                                             bool work, const uint32_t* __restrict__ bstart,
+                                             uint32_t lo[5], uint32_t len[5]) {
     const float cell = (float)G.cell;
-    const float R = fmaf(me32.w, (float)mult, 0.5f * cell) * (1.0f + 1e-5f) +
+    const float R = fmaf(me32.w, (float)mult, 0.5f * cell + extra) * (1.0f + 1e-5f) +
                     16.0f * FLT_EPSILON * (fabsf(me32.x) + fabsf(me32.y) + fabsf(me32.z) + 3.0f * cell);
     const float R2 = R * R;
     float gl[3], gh[3];  // distances to the cell's lower and upper faces (0 if outside)
@@ -382,16 +399,36 @@ constexpr int BROAD_BATCH = 4;
 constexpr int STAGE_CAP = 1344;  // float4 per block (21 KiB): six blocks per CU fit in the LDS
 constexpr int HIT_QUEUE = 256;   // pre-test survivors queued per wave
 
+// Append a near-list entry; `need` keeps the longest list that did not fit.
+__device__ __forceinline__ void near_append(uint32_t* __restrict__ nbc, uint32_t* __restrict__ nb,
+                                            int64_t n, int KN, uint32_t row, uint32_t word,
+                                            uint32_t& need) {
+    const uint32_t m = atomicAdd(&nbc[row], 1u);
+    if ((int)m < KN) nb[(int64_t)m * n + row] = word;
+    else need = need > m + 1u ? need : m + 1u;
+}
+
 // A thread waits on a chain of dependent loads (key, bucket bounds, union bounds, stage), so the
 // kernel is held to the register count of six waves per SIMD (80 VGPRs) for single scenes and
 // five (96) for batches; neither per-step variant spills.
-template <bool INIT, bool SEG>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(SEG ? 5 : 6)))
+//
+// With near lists (NEAR; skin_cur is the buffer parity) the broad phase runs only on steps that
+// rebuild; its FP32 pre-test then reaches r_i + r_j + 2 delta, every pair that passes it goes to
+// both particles' near lists, and the touching ones to the touching rows as before.  That variant
+// needs more registers and runs at five waves per SIMD (four for batches).
+template <bool INIT, bool SEG, bool NEAR>
+__global__ void __launch_bounds__(TPB)
+__attribute__((amdgpu_waves_per_eu(SEG && NEAR ? 4 : (SEG || NEAR ? 5 : 6))))
 k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
               const double4* __restrict__ pos, const float4* __restrict__ spos32,
               const uint32_t* __restrict__ bstart, const double* __restrict__ r0,
               const uint32_t* __restrict__ eid, uint32_t* __restrict__ tcnt,
-              uint32_t* __restrict__ touch, unsigned long long* ctr, int diag) {
+              uint32_t* __restrict__ touch, unsigned long long* ctr, int diag,
+              uint32_t* __restrict__ nbc, uint32_t* __restrict__ nb, uint32_t* __restrict__ gmin,
+              int skin_cur) {
+    static_assert(!(INIT && NEAR), "the bond pass keeps no near lists");
+    if (NEAR && !skin_rebuilds(ctr, skin_cur)) return;
+    constexpr bool near_lists = NEAR;
     // BROAD_BATCH - 1 entries of padding: a batch's reads past the end of the last range stay
     // inside the array (and are masked)
     __shared__ float4 stage[STAGE_CAP + BROAD_BATCH];
@@ -427,8 +464,11 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     if (t == tl) { b_cell[1] = mycell; b_scene[1] = SEG ? (uint32_t)P.sid[p] : 0u; }
     // this thread's five slot ranges [lo_j, lo_j + len_j), all above t
     const bool work = live && (!INIT || S.bond_init);
+    // the near lists' reach past r_i + r_j (rounded up to FP32)
+    const float ext2 = near_lists ? (float)(2.0 * S.skin) * (1.0f + FLT_EPSILON) : 0.0f;
     uint32_t lo[5], len[5];
-    broad_ranges(G, cbase, c, me32, INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0, t, work, bstart, lo, len);
+    broad_ranges(G, cbase, c, me32, INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0, ext2, t, work, bstart,
+                 lo, len);
     __syncthreads();
     // union ranges (threads 0-4), then their LDS offsets (thread 0)
     const bool one_scene = !SEG || b_scene[0] == b_scene[1];
@@ -474,14 +514,27 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     __syncthreads();
 
     uint32_t found = 0u;  // touching pairs this thread owns (few)
-    uint32_t overflow = 0;
+    uint32_t overflow = 0, nb_need = 0;
     const double mult = INIT ? S.bond_mult * (1.0 + 1e-9) : 1.0;
     // FP64 tests of a pre-test survivor (slot k) of the particle in slot to (index po), and the
-    // row appends of a touching pair
+    // row appends of a touching pair; with near lists every survivor is a near pair
     auto confirm = [&](const SceneP& Sc, uint32_t po, uint32_t k) {
         const uint32_t q = sidx[k];
+        if (near_lists) {
+            const bool pn = eid[po] < eid[q];
+            near_append(nbc, nb, n, P.KN, po, q | (pn ? IS_I : 0u), nb_need);
+            near_append(nbc, nb, n, P.KN, q, po | (pn ? 0u : IS_I), nb_need);
+        }
         const double4 mo = pos[po], ov = pos[q];
         const double ex = mo.x - ov.x, ey = mo.y - ov.y, ez = mo.z - ov.z;
+        if (near_lists) {
+            // the pair's gap, rounded down (two roundings of sqrt and difference, then FP32)
+            const double dd = sqrt(ex * ex + ey * ey + ez * ez), rr = mo.w + ov.w;
+            const double gap = (dd - rr) - 4e-16 * (dd + rr);
+            const uint32_t gb = __float_as_uint(gap > 0.0 ? __double2float_rd(gap) : 0.0f);
+            atomicMin(&gmin[po], gb);
+            atomicMin(&gmin[q], gb);
+        }
         const double rs = (mo.w + ov.w) * (INIT ? Sc.bond_mult * (1.0 + 1e-9) : 1.0);
         if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) return;  // sqrt(d2) >= rs
         const bool pi = eid[po] < eid[q];  // po is the pair's i (file order)
@@ -510,9 +563,9 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
                 if (lo[j] < u_lo[j] || lo[j] - u_lo[j] + len[j] > ulen) in_stage = false;
             }
         }
-        // FP32 pre-test |d|^2 <= thr^2, thr = (r_i + r_j) * fmult + slack, evaluated with fused
-        // multiply-adds; the 64-ulp margin in fmult covers their rounding
-        const float c0 = fmaf(me32.w, fmult, slack);
+        // FP32 pre-test |d|^2 <= thr^2, thr = (r_i + r_j + ext2) * fmult + slack, evaluated with
+        // fused multiply-adds; the 64-ulp margin in fmult covers their rounding
+        const float c0 = fmaf(me32.w + ext2, fmult, slack);
         auto near = [&](const float4 o) -> bool {
             const float ex = me32.x - o.x, ey = me32.y - o.y, ez = me32.z - o.z;
             const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
@@ -602,6 +655,10 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
     }
     if (diag) wave_add(ctr_at(ctr, CT_TOUCH), found);
     if (overflow) fail_step(ctr, (uint32_t)P.K + 1u);
+    if (nb_need) {  // a near list is incomplete: the next step rebuilds (and the host widens them)
+        atomicMax(&ctr[CT_NBNEED], (unsigned long long)nb_need);
+        skin_request(ctr, skin_cur);
+    }
 }
 
 // Candidate pairs the dense broad phase examines (its five trimmed slot ranges): a diagnostic,
@@ -612,6 +669,7 @@ __global__ void __launch_bounds__(TPB)
 k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32_t* __restrict__ skey,
                  const float4* __restrict__ spos32, const uint32_t* __restrict__ bstart,
                  unsigned long long* ctr) {
+    // the ranges the broad phase walks, with the near lists' reach when they are in use
     const int64_t t = (int64_t)blockIdx.x * TPB + threadIdx.x;
     uint64_t c = 0;
     if (t < P.n) {
@@ -620,7 +678,8 @@ k_examined_count(const Params P, const uint32_t* __restrict__ sidx, const uint32
         int32_t cc[3];
         dense_coords(G, skey[t] - S.cell_base, cc);
         uint32_t lo[5], len[5];
-        broad_ranges(G, S.cell_base, cc, spos32[t], 1.0, t, true, bstart, lo, len);
+        const float ext2 = P.skin ? (float)(2.0 * S.skin) * (1.0f + FLT_EPSILON) : 0.0f;
+        broad_ranges(G, S.cell_base, cc, spos32[t], 1.0, ext2, t, true, bstart, lo, len);
 #pragma unroll
         for (int j = 0; j < 5; ++j) c += len[j];
     }
@@ -797,41 +856,114 @@ __device__ __forceinline__ uint32_t sort_unique(uint32_t nt, Get get, Set set) {
     return nu;
 }
 
+// The broad phase's test of a touching pair (kernels.jl:103, norm(k_i - k_j) < r_i + r_j, and the
+// reference's window rule when the contact reach exceeds half its cell), for a particle at mo and
+// its partner at ov; pi: the particle is the pair's i.  The same operations as the broad phase's
+// confirmation, so a near-list step finds the touching set a rebuilding step would.
+__device__ __forceinline__ bool touching_pair(const SceneP& S, const double4 mo, const double4 ov,
+                                              bool pi) {
+    const double ex = mo.x - ov.x, ey = mo.y - ov.y, ez = mo.z - ov.z;
+    const double rs = mo.w + ov.w;
+    if (ex * ex + ey * ey + ez * ez > rs * rs * (1.0 + 1e-12)) return false;  // sqrt(d2) >= rs
+    const double4 xl = pi ? mo : ov, xh = pi ? ov : mo;
+    if (!(norm(mk3(xl.x, xl.y, xl.z) - mk3(xh.x, xh.y, xh.z)) < xl.w + xh.w)) return false;
+    return !S.ref_check || ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z));
+}
+
 // The merge kernel.  Rows of up to MERGE_LDS touching partners and old entries are staged in LDS
-// with batched independent loads; longer rows are merged in place in global memory.
+// with batched independent loads; longer rows are merged in place in global memory.  On a step
+// that reuses the near lists the touching partners are the near-list entries that pass the
+// broad phase's test, found here (the broad phase did not run).
 constexpr int MERGE_LDS = 16;
 
 template <bool INIT, bool SEG, int BS>
 __global__ void __launch_bounds__(BS)
-k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ hid,
-        uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch, uint8_t* __restrict__ sel,
-        uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1, uint32_t* __restrict__ pq0,
-        uint32_t* __restrict__ pq1, double* __restrict__ st0, double* __restrict__ st1,
-        unsigned long long* ctr, int diag) {
+k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ nbc,
+        const uint32_t* __restrict__ nb, const uint32_t* __restrict__ gmin,
+        uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
+        uint8_t* __restrict__ sel, uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1,
+        uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
+        double* __restrict__ st1, unsigned long long* ctr, int diag, int skin_cur) {
     __shared__ uint32_t lt[MERGE_LDS * BS];  // touching partners
     __shared__ uint32_t lw[MERGE_LDS * BS];  // old-row words
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     const int64_t n = P.n;
     const int K = P.K;
-    uint64_t created = 0, deact = 0, entries = 0;
+    const bool reuse = !INIT && skin_cur >= 0 && !skin_rebuilds(ctr, skin_cur);
+    uint64_t created = 0, deact = 0, entries = 0, found = 0;
     uint32_t overflow = 0, need = 0;
+    // twice the displacement bound D (rounded up): a particle whose near-list gaps at the rebuild
+    // all exceed it touches none of those partners now
+    float twoD = 0.0f;
+    if (reuse) {
+        __shared__ float s_disp;
+        if (threadIdx.x < 64) {
+            float v = 0.0f;
+            for (int c = threadIdx.x; c < CTR_COPIES; c += 64)
+                v = fmaxf(v, __uint_as_float((uint32_t)ctr[c * CTR_STRIDE + CT_DISP + skin_cur]));
+            for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+            if (threadIdx.x == 0) s_disp = v;
+        }
+        __syncthreads();
+        twoD = 2.0f * s_disp * (1.0f + 1e-6f);
+    }
     if (p64 < n) {
         const uint32_t p = (uint32_t)p64;
-        uint32_t nt = tcnt[p];
-        need = nt;
-        // the committed buffer: after a failed step of the call a row may still be marked
-        // pending; this merge then overwrites the uncommitted buffer again, which the host drops
+        const int tid = threadIdx.x;
+        // independent loads first: the touching (or near-list) count, the row selector and counts.
+        // The committed buffer: after a failed step of the call a row may still be marked
+        // pending; this merge then overwrites the uncommitted buffer again, which the host drops.
+        const uint32_t cnt_in = reuse ? nbc[p] : tcnt[p];
+        const float gm = reuse ? __uint_as_float(gmin[p]) : 0.0f;
         const int b = sel[p] & 1;
         const uint32_t c0 = rcnt0[p], c1 = rcnt1[p];
         const uint32_t nold = b ? c1 : c0;
+        uint32_t nt = cnt_in;
+        bool t_lds = false;  // the touching partners are in lt already
+        if (reuse) {
+            // the first MERGE_LDS touching partners go to lt, later ones (when there are more) to
+            // the touching row, which then also takes a copy of the first ones
+            nt = 0;
+            const uint32_t nn = gm > twoD ? 0u : min(cnt_in, (uint32_t)P.KN);
+            if (nn) {
+                const SceneP& S = scene_of<SEG>(P, p);
+                const double4 mo = pos[p];
+                // entry m + 1's word and partner position are loaded while entry m is tested
+                uint32_t w = nb[p];
+                double4 o = pos[w & IDX_MASK];
+                for (uint32_t m = 0; m < nn; ++m) {
+                    uint32_t wn = 0u;
+                    double4 on = mo;
+                    if (m + 1 < nn) {
+                        wn = nb[(int64_t)(m + 1) * n + p];
+                        on = pos[wn & IDX_MASK];
+                    }
+                    if (touching_pair(S, mo, o, (w & IS_I) != 0u)) {
+                        if (w & IS_I) ++found;
+                        if (nt < (uint32_t)MERGE_LDS) {
+                            lt[nt * BS + tid] = w;
+                        } else {
+                            if (nt == (uint32_t)MERGE_LDS)
+                                for (uint32_t a = 0; a < (uint32_t)MERGE_LDS && (int)a < K; ++a)
+                                    touch[(int64_t)a * n + p] = lt[a * BS + tid];
+                            if ((int)nt < K) touch[(int64_t)nt * n + p] = w;
+                        }
+                        ++nt;
+                    }
+                    w = wn;
+                    o = on;
+                }
+            }
+            t_lds = nt <= (uint32_t)MERGE_LDS;
+        }
+        need = nt;
         if (nt | nold) {
             const SceneP& S = scene_of<SEG>(P, p);
-            if (nt) tcnt[p] = 0u;
+            if (nt && !reuse) tcnt[p] = 0u;
             if ((int)nt > K) { overflow = 1; nt = K; }
             const uint32_t* pqo = b ? pq1 : pq0;
             if (nt <= (uint32_t)MERGE_LDS && nold <= (uint32_t)MERGE_LDS) {
-                const int tid = threadIdx.x;
-                for (uint32_t m = 0; m < nt; m += 8) {
+                for (uint32_t m = 0; m < (t_lds ? 0u : nt); m += 8) {
                     uint32_t v[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) v[u] = m + u < nt ? touch[(int64_t)(m + u) * n + p] : 0u;
@@ -853,6 +985,8 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
                                  [&](uint32_t m) { return lt[m * BS + tid]; }, sel, rcnt0,
                                  rcnt1, pq0, pq1, st0, st1, created, deact, entries, overflow);
             } else {
+                if (t_lds)
+                    for (uint32_t a = 0; a < nt; ++a) touch[(int64_t)a * n + p] = lt[a * BS + tid];
                 // long rows: the touching list is sorted in place (it is rebuilt every step)
                 const uint32_t nu = sort_unique(
                     nt, [&](uint32_t m) { return touch[(int64_t)m * n + p]; },
@@ -868,6 +1002,7 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
         wave_add(ctr_at(ctr, CT_CREATED), created);
         wave_add(ctr_at(ctr, CT_DEACT), deact);
         wave_add(ctr_at(ctr, CT_ENTRIES), entries);
+        if (reuse) wave_add(ctr_at(ctr, CT_TOUCH), found);
     }
     // the row this particle needs: its merged length, or at least its touching count
     if (overflow) fail_step(ctr, (uint32_t)entries > need ? (uint32_t)entries : need);
@@ -1253,10 +1388,14 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         const uint32_t* __restrict__ rcnt0, const uint32_t* __restrict__ rcnt1,
         uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
         double* __restrict__ st1, uint32_t* __restrict__ wact, double4* __restrict__ wshear,
-        unsigned long long* ctr, int write_forces) {
+        unsigned long long* ctr, int write_forces, float4* __restrict__ xref, int skin_cur) {
     // a row of this step or an earlier one overflowed: change nothing.  Loaded first and tested
     // before the first store, so the load's latency hides behind the state loads.
     const bool failed = step_failed(ctr);
+    // near lists in use (skin_cur >= 0): did this step rebuild them?
+    const bool rebuilt = skin_cur >= 0 && skin_rebuilds(ctr, skin_cur);
+    bool moved = false;  // this particle's new position is delta or more from its rebuild position
+    float disp = 0.0f;   // and its distance from there (an upper bound)
     // XCD-contiguous blocks: partners are spatial neighbours, so their state is in this XCD's L2
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     const int64_t n = P.n;
@@ -1265,7 +1404,18 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         const uint32_t p = (uint32_t)p64;
         const SceneP& S = scene_of<SEG>(P, p);
         const double dt = S.dt;
-        const double4 X = pos[p], Vv = vel[p], W = omg[p];  // the orientation is read after the row
+        const double4 X = pos[p], Vv = vel[p], W = omg[p];
+        // the orientation: with the state in FP64 it is loaded with the rest (one memory round
+        // trip less per wave); mixed precision loads it after the row, where it needs no registers
+        // during the contact loop
+        double4 Q = make_double4(0.0, 0.0, 0.0, 0.0);
+        float4 xr = make_float4(0.f, 0.f, 0.f, 0.f);  // the position at the last rebuild
+        if constexpr (!MIXED) {
+            Q = quat[p];
+            if (skin_cur >= 0 && !rebuilt) xr = xref[p];
+        }
+        uint32_t wa = 0u;  // the walls this particle touched last step
+        if (!MIXED && S.nwall) wa = wact[p];
         const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
         d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
         CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
@@ -1337,7 +1487,8 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             T = T + TH.value();
         }
         // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
-        uint32_t wa = S.nwall ? wact[p] : 0u, wa_new = 0u;
+        if (MIXED && S.nwall) wa = wact[p];
+        uint32_t wa_new = 0u;
         for (int j = 0; j < S.nwall; ++j) {
             const WallP& wl = S.walls[j];
             const d3 nrm = mk3(wl.n[0], wl.n[1], wl.n[2]);
@@ -1378,7 +1529,7 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             ttot[p] = make_double4(T.x, T.y, T.z, 0.0);
         }
         // update! (kernels.jl:445-485)
-        const double4 Q = quat[p];
+        if constexpr (MIXED) Q = quat[p];
         const d3 av = F / Vv.w;
         const d3 knew = (kp + vp * dt) + (0.5 * av) * (dt * dt);
         const d3 vnew = vp + av * dt;
@@ -1401,6 +1552,25 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         const double n0 = q0 + dq0, n1 = q1 + dq1, n2 = q2 + dq2, n3 = q3 + dq3;
         const double qn = sqrt(n0 * n0 + n1 * n1 + n2 * n2 + n3 * n3);
         const d3 wnew = wp + dw * dt;
+        if (skin_cur >= 0) {
+            // |x_new - x_rebuild| < delta, with x_rebuild rounded to FP32: the bound is reduced by
+            // that rounding (at most 2^-24 |x| per axis) and by 1e-6 relative for the FP64 sums
+            if (rebuilt) {
+                xr = make_float4((float)X.x, (float)X.y, (float)X.z, 0.f);
+                xref[p] = xr;
+            } else if (MIXED) {
+                xr = xref[p];
+            }
+            const double ex = knew.x - (double)xr.x, ey = knew.y - (double)xr.y,
+                         ez = knew.z - (double)xr.z;
+            const double chk = S.skin * (1.0 - 1e-6) -
+                               0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) + fabs((double)xr.z));
+            const double d2 = ex * ex + ey * ey + ez * ez;
+            moved = !(chk > 0.0) || !(d2 < chk * chk);
+            // the displacement bound for the next step: |x_new - x_rebuild| rounded up
+            disp = __double2float_ru((sqrt(d2) + 0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) +
+                                                            fabs((double)xr.z))) * (1.0 + 1e-12));
+        }
         // streaming stores: the next state is not read again before this launch ends
         store_nt(pos_n + p, make_double4(knew.x, knew.y, knew.z, X.w));
         store_nt(vel_n + p, make_double4(vnew.x, vnew.y, vnew.z, Vv.w));
@@ -1412,7 +1582,16 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
         }
     }
     if (failed) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&ctr[CT_DONE], 1ull);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&ctr[CT_DONE], 1ull);
+        if (rebuilt) atomicAdd(&ctr[CT_REBUILDS], 1ull);
+    }
+    if (skin_cur >= 0) {
+        if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0) skin_request(ctr, skin_cur);
+        for (int o = 32; o > 0; o >>= 1) disp = fmaxf(disp, __shfl_xor(disp, o, 64));
+        if ((threadIdx.x & 63) == 0 && disp > 0.0f)
+            atomicMax(ctr_at(ctr, CT_DISP + 1 - skin_cur), (unsigned long long)__float_as_uint(disp));
+    }
     wave_add(ctr_at(ctr, CT_BROKEN), broken);
     wave_add(ctr_at(ctr, CT_BROKEN_TOTAL), broken);
     if (write_forces) {
@@ -1676,8 +1855,11 @@ void timing_end(dem_scene* s) {
         if (e_ != hipSuccess) return e_;    \
     } while (0)
 
+// skin_cur >= 0: the per-step sort of buffer parity skin_cur with near lists in use, which does
+// nothing on a step that reuses them
 template <bool SEG>
-static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool init = false) {
+static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool init = false,
+                            int skin_cur = -1) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
@@ -1692,15 +1874,16 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool 
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<true, SEG>), grid_for(P.n, TPB), dim3(TPB), 0,
                                              s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
                                              d.bcnt, d.scan.tagg, shift, d.ctr, reset_counters ? 1 : 0,
-                                             init ? 1 : 0));
+                                             init ? 1 : 0, d.nbc, d.gmin, skin_cur));
     } else {  // the reference's hashed grid: single scenes only
         LAUNCH(s, K_HASH, hipLaunchKernelGGL((k_hash<false, false>), grid_for(P.n, TPB), dim3(TPB), 0,
                                              s->stream, P, d.pos[c], prev, d.hid, d.rank, d.x32,
                                              d.bcnt, d.scan.tagg, shift, d.ctr, reset_counters ? 1 : 0,
-                                             0));
+                                             0, d.nbc, d.gmin, -1));
     }
+    const unsigned long long* gate = skin_cur >= 0 ? d.ctr + CT_SKIN + skin_cur : nullptr;
     timing_begin(s, K_SCAN);
-    hipError_t e = dense ? launch_tile_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, s->stream)
+    hipError_t e = dense ? launch_tile_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, s->stream, gate)
                          : launch_scan(d.scan, d.bcnt, d.bstart, nkeys + 1, true, nullptr, s->stream);
     timing_end(s);
     if (e != hipSuccess) return e;
@@ -1708,7 +1891,7 @@ static hipError_t sort_impl(dem_scene* s, bool reset_counters, bool dense, bool 
                                             P, d.pos[c], prev, d.hid, d.rank, d.x32, d.bstart, d.sidx,
                                             d.skey, dense ? nullptr : d.spos, d.spos32, d.eid,
                                             dense ? nullptr : d.sfid, d.scan.tagg,
-                                            dense ? (int)ntiles : 0));
+                                            dense ? (int)ntiles : 0, gate));
     return hipSuccess;
 }
 
@@ -1728,7 +1911,9 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
     const int c = s->cur;
     const bool dense = s->dense;
     if (SEG && !dense) return hipErrorInvalidValue;
-    hipError_t e = sort_impl<SEG>(s, true, dense, init);
+    // near lists: the buffer parity, whose flag word says whether this step rebuilds
+    const int skin_cur = (P.skin && dense && !init) ? c : -1;
+    hipError_t e = sort_impl<SEG>(s, true, dense, init, skin_cur);
     if (e != hipSuccess) return e;
     const bool mixed = s->precision == DEM_PRECISION_MIXED && !init;
     const dim3 g = grid_for(P.n, TPB);
@@ -1737,13 +1922,20 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
                                               d.sfid, d.bstart, d.eid, d.ctr));
     if (dense) {
         if (init) {
-            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, SEG>), g, dim3(TPB), 0,
+            LAUNCH(s, K_INIT, hipLaunchKernelGGL((k_broad_dense<true, SEG, false>), g, dim3(TPB), 0,
                                                  s->stream, P, d.sidx, d.skey, d.pos[c], d.spos32,
-                                                 d.bstart, d.r0, d.eid, d.tcnt, d.touch, d.ctr, diag));
-        } else {
-            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, SEG>), g, dim3(TPB), 0,
+                                                 d.bstart, d.r0, d.eid, d.tcnt, d.touch, d.ctr, diag,
+                                                 d.nbc, d.nb, d.gmin, -1));
+        } else if (skin_cur >= 0) {
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, SEG, true>), g, dim3(TPB), 0,
                                                   s->stream, P, d.sidx, d.skey, d.pos[c], d.spos32,
-                                                  d.bstart, d.r0, d.eid, d.tcnt, d.touch, d.ctr, diag));
+                                                  d.bstart, d.r0, d.eid, d.tcnt, d.touch, d.ctr, diag,
+                                                  d.nbc, d.nb, d.gmin, skin_cur));
+        } else {
+            LAUNCH(s, K_BROAD, hipLaunchKernelGGL((k_broad_dense<false, SEG, false>), g, dim3(TPB), 0,
+                                                  s->stream, P, d.sidx, d.skey, d.pos[c], d.spos32,
+                                                  d.bstart, d.r0, d.eid, d.tcnt, d.touch, d.ctr, diag,
+                                                  d.nbc, d.nb, d.gmin, -1));
         }
     } else if (s->slot_owned) {
         if (init) {
@@ -1776,20 +1968,20 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
     const dim3 gm = grid_for(P.n, MERGE_TPB);
     if (init) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
-                                              s->stream, P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel,
+                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
                                               d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag));
+                                              d.ctr, diag, skin_cur));
     } else if (s->force_tpb == 64) {
         // contact-dense, as the force kernel's block choice: one-wave blocks for uneven rows
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), lds,
-                                              s->stream, P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel,
+                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
                                               d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag));
+                                              d.ctr, diag, skin_cur));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
-                                              s->stream, P, d.pos[c], d.hid, d.tcnt, d.touch, d.sel,
+                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
                                               d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag));
+                                              d.ctr, diag, skin_cur));
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
@@ -1813,6 +2005,7 @@ static hipError_t launch_force(dem_scene* s, bool write_forces) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur, nx = 1 - s->cur;
+    const int skin = (P.skin && s->dense) ? c : -1;
     const dim3 g = grid_for(P.n, BS);
     if (s->precision == DEM_PRECISION_MIXED) {
         LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<true, SEG, BS>), g, dim3(BS), 0, s->stream, P,
@@ -1820,14 +2013,14 @@ static hipError_t launch_force(dem_scene* s, bool write_forces) {
                                               d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
                                               d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
                                               d.st[0], d.st[1], d.wact, d.wshear, d.ctr,
-                                              write_forces ? 1 : 0));
+                                              write_forces ? 1 : 0, d.xref, skin));
     } else {
         LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<false, SEG, BS>), g, dim3(BS), 0, s->stream, P,
                                               d.pos[c], d.vel[c], d.omg[c], d.quat[c], d.pos[nx],
                                               d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
                                               d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
                                               d.st[0], d.st[1], d.wact, d.wshear, d.ctr,
-                                              write_forces ? 1 : 0));
+                                              write_forces ? 1 : 0, d.xref, skin));
     }
     return hipSuccess;
 }

@@ -185,7 +185,8 @@ k_scan_exclusive(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t 
 template <int SCAN_ITEMS>
 __global__ void __launch_bounds__(SCAN_THREADS)
 k_scan_tiles(uint32_t* __restrict__ in, uint32_t* __restrict__ out, int64_t n,
-             const uint32_t* __restrict__ tagg) {
+             const uint32_t* __restrict__ tagg, const unsigned long long* __restrict__ gate) {
+    if (gate && *gate == 0ull) return;  // the step reuses the near lists: no sort
     constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
     __shared__ uint32_t s_vals[SCAN_TILE + SCAN_TILE / 32];
     __shared__ uint32_t s_wave[SCAN_THREADS / 64];
@@ -334,16 +335,16 @@ hipError_t noalloc_scratch(int device, uint64_t** out) {
 }  // namespace
 
 hipError_t launch_tile_scan(ScanWorkspace& ws, uint32_t* in, uint32_t* out, int64_t n,
-                            hipStream_t stream) {
+                            hipStream_t stream, const unsigned long long* gate) {
     if (n <= 0) return hipSuccess;
     const int64_t ntiles = scan_tiles(n);
     if (ntiles > ws.max_tiles) return hipErrorInvalidValue;
     if (n >= SCAN_LARGE_FROM)
         hipLaunchKernelGGL(k_scan_tiles<SCAN_ITEMS_LARGE>, dim3((uint32_t)ntiles), dim3(SCAN_THREADS), 0,
-                           stream, in, out, n, ws.tagg);
+                           stream, in, out, n, ws.tagg, gate);
     else
         hipLaunchKernelGGL(k_scan_tiles<SCAN_ITEMS_SMALL>, dim3((uint32_t)ntiles), dim3(SCAN_THREADS), 0,
-                           stream, in, out, n, ws.tagg);
+                           stream, in, out, n, ws.tagg, gate);
     return hipGetLastError();
 }
 

@@ -44,7 +44,7 @@ class Scene:
     def __init__(self, cfg: DEMConfig, grains: np.ndarray, walls: Optional[Sequence] = None,
                  precision: str = "fp64", contact_slots: int = 0, device: int = 0,
                  index_owned_broad_phase: bool = False, reference_grid: bool = False,
-                 graph: bool = True):
+                 graph: bool = True, near_lists: bool = True):
         self._lib = N.load_library()
         self.cfg = cfg
         grains = np.ascontiguousarray(grains, dtype=GRAIN_IN_DTYPE)
@@ -57,7 +57,8 @@ class Scene:
         self._c_cfg = cfg.to_c()
         flags = (N.DEM_FLAG_INDEX_OWNED_BROAD_PHASE if index_owned_broad_phase else 0) | \
                 (N.DEM_FLAG_REFERENCE_GRID if reference_grid else 0) | \
-                (0 if graph else N.DEM_FLAG_NO_GRAPH)
+                (0 if graph else N.DEM_FLAG_NO_GRAPH) | \
+                (0 if near_lists else N.DEM_FLAG_NO_NEAR_LISTS)
         opt = N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), flags)
         h = ctypes.c_void_p()
         rc = self._lib.dem_create(ctypes.byref(self._c_cfg), _ptr(self.walls, N.DemWall),
@@ -208,11 +209,11 @@ class Batch(Scene):
     (dem_create_batch).  Particle indices are global; ``offsets[k]:offsets[k+1]`` is scene k."""
 
     def __init__(self, members, precision: str = "fp64", contact_slots: int = 0, device: int = 0,
-                 graph: bool = True):
+                 graph: bool = True, near_lists: bool = True):
         self._lib = N.load_library()
         members = list(members)
         cfgs, nwalls, walls, counts, allw, allg = self._pack(members)
-        opt = self._options(precision, contact_slots, device, graph)
+        opt = self._options(precision, contact_slots, device, graph, near_lists)
         h = ctypes.c_void_p()
         rc = self._lib.dem_create_batch(len(members), cfgs, _ptr(nwalls, ctypes.c_int32),
                                         _ptr(allw, N.DemWall), _ptr(counts, ctypes.c_int64),
@@ -223,7 +224,8 @@ class Batch(Scene):
 
     @classmethod
     def repack(cls, src: "Scene", keep, members=(), precision: Optional[str] = None,
-               contact_slots: int = 0, device: int = 0, graph: bool = True) -> "Batch":
+               contact_slots: int = 0, device: int = 0, graph: bool = True,
+               near_lists: bool = True) -> "Batch":
         """dem_batch_repack: a new batch of scenes ``keep`` of ``src`` (a Batch or a Scene), their
         state moved over on the device, followed by the new ``members`` (cfg, grains, walls).
         ``src`` is left unchanged; the caller closes it."""
@@ -232,7 +234,7 @@ class Batch(Scene):
         keep = np.ascontiguousarray(np.asarray(keep, dtype=np.int32).reshape(-1))
         members = list(members)
         precision = precision or src.precision
-        opt = self._options(precision, contact_slots, device, graph)
+        opt = self._options(precision, contact_slots, device, graph, near_lists)
         h = ctypes.c_void_p()
         if members:
             cfgs, nwalls, walls, counts, allw, allg = self._pack(members)
@@ -251,8 +253,8 @@ class Batch(Scene):
         return self
 
     @staticmethod
-    def _options(precision, contact_slots, device, graph):
-        flags = 0 if graph else N.DEM_FLAG_NO_GRAPH
+    def _options(precision, contact_slots, device, graph, near_lists=True):
+        flags = (0 if graph else N.DEM_FLAG_NO_GRAPH) | (0 if near_lists else N.DEM_FLAG_NO_NEAR_LISTS)
         return N.DemOptions(PRECISIONS[precision], int(contact_slots), int(device), flags)
 
     @staticmethod

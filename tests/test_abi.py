@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
     missing = [f for f in declared_functions() if not hasattr(lib, f)]
     assert not missing, missing
     assert {name for name, _, _ in N.SIGNATURES} == set(declared_functions())
-    assert lib.dem_abi_version() == 1
+    assert lib.dem_abi_version() == 2
     assert lib.dem_kernel_name(3) == b"broad_phase"
 
 
