@@ -259,7 +259,7 @@ def run_serving(args):
     from taichi_dem_julia_port_amd import serving
     reqs = serving.make_stream(args.requests, rate=args.rate, seed=5)
     work = sum(r.n * r.steps for r in reqs)
-    srv = serving.Server(precision=args.precision, epoch_steps=args.epoch)
+    srv = serving.Server(precision=args.precision, epoch_steps=args.epoch, poll_steps=args.poll)
     done, makespan = srv.run(reqs)
     out_b = serving.summarize(done, makespan)
     base, makespan0 = serving.serve_one_at_a_time(serving.make_stream(args.requests, rate=args.rate, seed=5),
@@ -273,7 +273,8 @@ def run_serving(args):
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": (f"{args.requests} requests, Poisson arrivals at {args.rate}/s, grains "
                                 f"log-uniform 1k-200k, half bonded cubes / half loose pours, steps "
-                                f"uniform 2k-50k, epoch {args.epoch} steps"),
+                                f"uniform 2k-50k, epochs of up to {args.epoch} steps ended early "
+                                f"by an arrival after a slice of {args.poll} steps"),
                    "precision": args.precision, "particle_steps_requested": work},
         "batched": out_b, "repacks": srv.repacks,
         "repack_s": round(srv.repack_s, 3), "stepping_s": round(srv.step_s, 3),
@@ -330,8 +331,11 @@ def main():
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--epoch", type=int, default=1000)
+    ap.add_argument("--poll", type=int, default=100,
+                    help="serving: steps per slice of an epoch (an arrival ends the epoch)")
     ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="default: mixed for the serving stream, fp64 otherwise")
     ap.add_argument("--no-compare-precision", action="store_true",
                     help="skip timing the other precision and the 1000-step position error")
     ap.add_argument("--no-contact-heavy", action="store_true",
@@ -356,6 +360,8 @@ def main():
                                    "and multi-GPU runs are not supported (use --gpus 1)"}))
         sys.exit(2)
 
+    if args.precision is None:
+        args.precision = "mixed" if args.workload == "serve" else "fp64"
     if args.workload == "serve":
         run_serving(args)
         return

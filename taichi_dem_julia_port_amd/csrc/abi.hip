@@ -7,8 +7,11 @@
 #include <string.h>
 
 #include <algorithm>
+#include <functional>
 #include <mutex>
 #include <string>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 #include "dem_internal.hpp"
@@ -145,6 +148,19 @@ hipError_t alloc_rows(dem_scene* s, int32_t K) {
     if (e != hipSuccess) return e;
     layout(d.rows_arena);
     return hipMemset(d.rows_arena, 0, bytes);
+}
+
+// Run independent host tasks on threads of their own (serially where a thread cannot start).
+void run_parallel(std::vector<std::function<void()>>& tasks) {
+    std::vector<std::thread> th;
+    size_t started = 0;
+    try {
+        for (; started < tasks.size(); ++started) th.emplace_back(tasks[started]);
+    } catch (const std::system_error&) {
+        // fewer threads than tasks: the rest run here
+    }
+    for (size_t k = started; k < tasks.size(); ++k) tasks[k]();
+    for (auto& t : th) t.join();
 }
 
 // The near lists of KN slots per particle (nothing when the scene does not use them).  Entries
@@ -946,26 +962,43 @@ int32_t dem_batch_repack(dem_scene_t* src, int32_t nkeep, const int32_t* keep, i
         off += m;
     };
     // kept scenes: their derived host fields, order and parameters move over unchanged
+    std::vector<int64_t> kept_at(nkeep);  // first particle of kept scene j in the new batch
     for (int j = 0; j < nkeep; ++j) {
         const int32_t k = keep[j];
-        const int64_t a = src->seg_off[k], m = src->seg_off[k + 1] - a;
         s->seg_host[j] = src->batched ? src->seg_host[k] : src->P.sc;
         s->seg_host[j].bond_init = 0;
         s->seg_cfg[j] = src->seg_cfg[k];
         s->seg_der[j] = src->seg_der[k];
-        std::copy(src->h_id + a, src->h_id + a + m, s->h_id + off);
-        std::copy(src->h_gid + a, src->h_gid + a + m, s->h_gid + off);
-        std::copy(src->h_mid + a, src->h_mid + a + m, s->h_mid + off);
-        std::copy(src->h_V + a, src->h_V + a + m, s->h_V + off);
-        std::copy(src->h_m + a, src->h_m + a + m, s->h_m + off);
-        std::copy(src->h_r + a, src->h_r + a + m, s->h_r + off);
-        std::copy(src->h_r0 + a, src->h_r0 + a + m, s->h_r0 + off);
-        std::copy(src->h_I + a, src->h_I + a + m, s->h_I + off);
-        for (int64_t u = 0; u < m; ++u) {
-            s->perm[off + u] = (uint32_t)(src->perm[a + u] - a + off);
-            s->iperm[s->perm[off + u]] = (uint32_t)(off + u);
-        }
-        place(j, m);
+        kept_at[j] = off;
+        place(j, src->seg_off[k + 1] - src->seg_off[k]);
+    }
+    {
+        // the per-grain host fields, one task per field: the copies are independent, and the
+        // first touch of the new arrays' pages makes a serial copy of a large batch slow
+        auto each_kept = [&](auto fn) {
+            for (int j = 0; j < nkeep; ++j) {
+                const int64_t a = src->seg_off[keep[j]];
+                fn(a, src->seg_off[keep[j] + 1] - a, kept_at[j]);
+            }
+        };
+        auto field = [&](auto* dst, const auto* from) {
+            return [=, &each_kept] {
+                each_kept([&](int64_t a, int64_t m, int64_t at) { std::copy(from + a, from + a + m, dst + at); });
+            };
+        };
+        std::vector<std::function<void()>> tasks = {
+            field(s->h_id, src->h_id), field(s->h_gid, src->h_gid), field(s->h_mid, src->h_mid),
+            field(s->h_V, src->h_V), field(s->h_m, src->h_m), field(s->h_r, src->h_r),
+            field(s->h_r0, src->h_r0), field(s->h_I, src->h_I),
+            [&] {
+                each_kept([&](int64_t a, int64_t m, int64_t at) {
+                    for (int64_t u = 0; u < m; ++u) {
+                        s->perm[at + u] = (uint32_t)(src->perm[a + u] - a + at);
+                        s->iperm[s->perm[at + u]] = (uint32_t)(at + u);
+                    }
+                });
+            }};
+        run_parallel(tasks);
     }
     // new scenes: derived from their input rows as dem_create_batch does
     woff = 0;

@@ -4,10 +4,14 @@ Requests (a scene and a number of steps) arrive over time.  At every scheduling 
 admits the requests that have arrived -- first come, first served, as long as the estimated HBM
 footprint of all live scenes stays within the budget -- and re-packs the live set into one batch
 (dem_batch_repack: the scenes already running move over device to device, only the arrivals are
-uploaded; dem_initialize for the newly admitted bonded ones).  It then advances every live scene together, one segmented launch per
-stage and one graph replay per two steps, by the smaller of the epoch length and the fewest steps
-any live scene still needs, so no scene runs past its request.  Finished scenes are read back and
-retired at the next epoch.  State stays on the device between epochs.
+uploaded; dem_initialize for the newly admitted bonded ones).  It then advances every live scene
+together, one segmented launch per stage and one graph replay per two steps, by the smaller of the
+epoch length and the fewest steps any live scene still needs, so no scene runs past its request.
+The epoch is stepped in slices of `poll_steps`; a request that arrives during an epoch ends it
+after the current slice once the epoch has run `min_epoch_s` (adaptive epochs), so an arrival
+waits at most that long plus one slice to be admitted, and a burst of arrivals shares one re-pack.
+Finished scenes are read back and retired at the next epoch.  State stays on the device between
+epochs.
 
 ``serve_one_at_a_time`` is the plain path beside it: each request alone, in arrival order, with
 its own FP64 scene.
@@ -127,8 +131,10 @@ def summarize(results: List[Result], makespan: float) -> dict:
 class Server:
     def __init__(self, precision: str = "mixed", hbm_budget_gb: float = 200.0,
                  epoch_steps: int = 1000, device: int = 0, keep_results: bool = False,
-                 memory_cache_gb: float = 64.0):
+                 memory_cache_gb: float = 64.0, poll_steps: int = 100, min_epoch_s: float = 0.1):
         self.precision = precision
+        self.poll = max(1, int(poll_steps))
+        self.min_epoch = float(min_epoch_s)
         self.budget = hbm_budget_gb * 1e9
         # released batches' device memory kept for the next re-pack while the server runs
         self.cache = int(memory_cache_gb * 1e9)
@@ -217,7 +223,16 @@ class Server:
                            slots=int(batch.derived()["contact_slots"]))
             k = min(self.epoch, min(m.remaining for m in live))
             ts = time.perf_counter()
-            batch.step(k)
+            # the epoch in slices; an arrival ends it early (admitted at the next epoch)
+            done_k = 0
+            while done_k < k:
+                sl = min(self.poll, k - done_k)
+                batch.step(sl)
+                done_k += sl
+                if pending and pending[0].arrival <= clock() and \
+                        time.perf_counter() - ts >= self.min_epoch:
+                    break
+            k = done_k
             self.step_s += time.perf_counter() - ts
             rec.update(scenes=len(live), grains=int(batch.n), steps=int(k),
                        step_s=round(time.perf_counter() - ts, 4))
