@@ -198,6 +198,9 @@ void drop_graphs(dem_scene* s) {
         if (s->graph_exec[b]) { hipGraphExecDestroy(s->graph_exec[b]); s->graph_exec[b] = nullptr; }
         if (s->graph[b]) { hipGraphDestroy(s->graph[b]); s->graph[b] = nullptr; }
         s->graph_force_tpb[b] = 0;
+        if (s->graph_lean_exec[b]) { hipGraphExecDestroy(s->graph_lean_exec[b]); s->graph_lean_exec[b] = nullptr; }
+        if (s->graph_lean[b]) { hipGraphDestroy(s->graph_lean[b]); s->graph_lean[b] = nullptr; }
+        s->graph_lean_tpb[b] = 0;
     }
 }
 
@@ -702,6 +705,7 @@ int32_t dem_create(const dem_config_t* cfg, const dem_wall_t* walls, int32_t nwa
     s->dense = P.sc.dense.cells > 0;
     P.skin = s->dense && !(o.flags & DEM_FLAG_NO_NEAR_LISTS) ? 1 : 0;
     P.KN = near_slots(K);
+    s->skin_min = P.sc.skin;
     P.sort_keys = s->dense ? P.sc.dense.cells : (uint32_t)P.sc.grid.H;
     P.sort_keys_init = s->dense ? P.sc.dense_init.cells : (uint32_t)P.sc.grid.H;
     const int64_t nkeys = std::max<int64_t>(
@@ -796,6 +800,8 @@ int32_t dem_create_batch(int32_t nscenes, const dem_config_t* cfgs, const int32_
     s->dense = true;
     P.skin = (o.flags & DEM_FLAG_NO_NEAR_LISTS) ? 0 : 1;
     P.KN = near_slots(K);
+    s->skin_min = s->seg_host[0].skin;
+    for (const SceneP& Sk : s->seg_host) s->skin_min = std::min(s->skin_min, Sk.skin);
     P.sort_keys = (uint32_t)keys;
     P.sort_keys_init = (uint32_t)keys_init;
     if ((rc = allocate_scene(s, grains, 0, 0, nwall_max, (int64_t)std::max(keys, keys_init)))) {
@@ -1034,6 +1040,8 @@ int32_t dem_batch_repack(dem_scene_t* src, int32_t nkeep, const int32_t* keep, i
     s->dense = true;
     P.skin = (o.flags & DEM_FLAG_NO_NEAR_LISTS) ? 0 : 1;
     P.KN = near_slots(K);
+    s->skin_min = s->seg_host[0].skin;
+    for (const SceneP& Sk : s->seg_host) s->skin_min = std::min(s->skin_min, Sk.skin);
     P.sort_keys = (uint32_t)keys;
     P.sort_keys_init = (uint32_t)keys_init;
     if ((rc = allocate_scene(s, grains, kept_total, nkeep, nwall_max, (int64_t)std::max(keys, keys_init)))) {
@@ -1141,26 +1149,32 @@ int32_t dem_initialize(dem_scene_t* s) {
     return DEM_OK;
 }
 
-// Capture two plain steps (no output writes, no diagnostics) starting from the current buffer
-// parity, kept as that parity's graph.
-static int32_t capture_graph(dem_scene* s) {
+// Capture two steps starting from the current buffer parity as that parity's graph: plain steps
+// (no output writes, no diagnostics), or lean steps.
+static int32_t capture_graph(dem_scene* s, bool lean) {
     const int start = s->cur;
-    if (s->graph_exec[start]) { hipGraphExecDestroy(s->graph_exec[start]); s->graph_exec[start] = nullptr; }
-    if (s->graph[start]) { hipGraphDestroy(s->graph[start]); s->graph[start] = nullptr; }
+    hipGraph_t* gr = lean ? s->graph_lean : s->graph;
+    hipGraphExec_t* ge = lean ? s->graph_lean_exec : s->graph_exec;
+    if (ge[start]) { hipGraphExecDestroy(ge[start]); ge[start] = nullptr; }
+    if (gr[start]) { hipGraphDestroy(gr[start]); gr[start] = nullptr; }
     HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
     for (int k = 0; k < 2 && e == hipSuccess; ++k) {
-        e = run_contact_detection(s, false, false);
-        if (e == hipSuccess) e = run_force_integrate(s, false);
+        if (lean) {
+            e = run_lean_step(s);
+        } else {
+            e = run_contact_detection(s, false, false);
+            if (e == hipSuccess) e = run_force_integrate(s, false);
+        }
     }
     hipGraph_t g = nullptr;
     hipError_t e2 = hipStreamEndCapture(s->stream, &g);
     s->cur = start;  // capture only recorded the launches
     if (e != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "graph capture", e); }
     if (e2 != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "hipStreamEndCapture", e2); }
-    s->graph[start] = g;
-    HIPCHK(s, hipGraphInstantiate(&s->graph_exec[start], g, nullptr, nullptr, 0));
-    s->graph_force_tpb[start] = s->force_tpb;
+    gr[start] = g;
+    HIPCHK(s, hipGraphInstantiate(&ge[start], g, nullptr, nullptr, 0));
+    (lean ? s->graph_lean_tpb : s->graph_force_tpb)[start] = s->force_tpb;
     return DEM_OK;
 }
 
@@ -1169,18 +1183,60 @@ static hipError_t request_rebuild(dem_scene* s) {
     return hipMemsetAsync(s->d.ctr + CT_SKIN + s->cur, 0xff, sizeof(unsigned long long), s->stream);
 }
 
-// Enqueue n steps: pairs of steps replayed from the captured graph of the current buffer parity
-// (not while timing individual kernels), the last step eager with its outputs and counters.
-static int32_t enqueue_steps(dem_scene* s, int64_t nsteps) {
+// One full step, eager; `last`: the last step of the call, with its outputs and counters.
+static hipError_t full_step(dem_scene* s, bool last) {
+    // the diagnostic passes read this step's sort
+    if (s->P.skin && s->diagnostics && last) {
+        hipError_t e = request_rebuild(s);
+        if (e != hipSuccess) return e;
+    }
+    hipError_t e = run_contact_detection(s, false, last);
+    if (e == hipSuccess) e = run_force_integrate(s, last);
+    if (e == hipSuccess) s->steps++;
+    return e;
+}
+
+// Enqueue n steps; call_end: the last of them ends the dem_step call.  With lean steps: a full
+// first step when it has to rebuild (the counters read last hold its flag word), lean steps
+// (pairs replayed from the lean graph of the current buffer parity), and a full last step when
+// it ends the call.  Otherwise full steps: pairs replayed from the graph, the call's last step
+// eager.  Graphs are not used while timing individual kernels.
+static int32_t enqueue_steps(dem_scene* s, int64_t nsteps, bool call_end, bool lean) {
     int64_t t = 0;
+    bool rebuild = false;
     if (s->P.skin && s->skin_dirty && nsteps > 0) {
         HIPCHK(s, request_rebuild(s));
         s->skin_dirty = false;
+        rebuild = true;
+    } else if (s->P.skin) {
+        rebuild = s->h_ctr[CT_SKIN + s->cur] != 0ull;
     }
-    if (s->use_graph && !s->timing && nsteps >= 3) {
-        const int b = s->cur;  // a graph of two steps ends at the parity it starts from
+    const bool graphs = s->use_graph && !s->timing;
+    const int64_t body_end = call_end ? nsteps - 1 : nsteps;  // the call's last step is full
+    if (lean) {
+        if (rebuild && t < body_end) {
+            HIPCHK(s, full_step(s, false));
+            ++t;
+        }
+        if (graphs && body_end - t >= 2) {
+            const int b = s->cur;  // a graph of two steps ends at the parity it starts from
+            if (!s->graph_lean_exec[b] || s->graph_lean_tpb[b] != s->force_tpb) {
+                int32_t rc = capture_graph(s, true);
+                if (rc) return rc;
+            }
+            for (; t + 2 <= body_end; t += 2) {
+                HIPCHK(s, hipGraphLaunch(s->graph_lean_exec[b], s->stream));
+                s->steps += 2;
+            }
+        }
+        for (; t < body_end; ++t) {
+            HIPCHK(s, run_lean_step(s));
+            s->steps++;
+        }
+    } else if (graphs && nsteps >= 3) {
+        const int b = s->cur;
         if (!s->graph_exec[b] || s->graph_force_tpb[b] != s->force_tpb) {
-            int32_t rc = capture_graph(s);
+            int32_t rc = capture_graph(s, false);
             if (rc) return rc;
         }
         for (; t + 2 < nsteps; t += 2) {
@@ -1188,14 +1244,38 @@ static int32_t enqueue_steps(dem_scene* s, int64_t nsteps) {
             s->steps += 2;
         }
     }
-    for (; t < nsteps; ++t) {
-        // the diagnostic passes read this step's sort
-        if (s->P.skin && s->diagnostics && t == nsteps - 1) HIPCHK(s, request_rebuild(s));
-        HIPCHK(s, run_contact_detection(s, false, t == nsteps - 1));
-        HIPCHK(s, run_force_integrate(s, t == nsteps - 1));
-        s->steps++;
-    }
+    for (; t < nsteps; ++t) HIPCHK(s, full_step(s, call_end && t == nsteps - 1));
     return DEM_OK;
+}
+
+// Lean-step chunks: between LEAN_CHUNK_MIN and LEAN_CHUNK_MAX steps.  Rebuilds closer together
+// than LEAN_CHUNK_MIN steps switch the scene to full steps; a call of full steps that rebuilt at
+// most once per 2 LEAN_CHUNK_MIN steps switches it back.
+constexpr int64_t LEAN_CHUNK_MIN = 16, LEAN_CHUNK_MAX = 1024;
+
+// The next lean chunk after one that completed: twice as long, unless the displacement bound grew
+// since the previous read with no rebuild between them; then half the steps it needs at that rate
+// to reach the skin (so a rebuild falls on the first step of a chunk, not inside one), at least 4.
+static void next_lean_chunk(dem_scene* s) {
+    const int cur = s->cur;
+    float D = 0.0f;
+    for (int c = 0; c < CTR_COPIES; ++c) {
+        const uint32_t bits = (uint32_t)s->h_ctr_raw[c * CTR_STRIDE + CT_DISP + cur];
+        float v;
+        memcpy(&v, &bits, sizeof(v));
+        D = std::max(D, v);
+    }
+    int64_t chunk = std::min(2 * s->lean_chunk, LEAN_CHUNK_MAX);
+    const unsigned long long rb = s->h_ctr[CT_REBUILDS];
+    if (rb == s->pred_rebuilds && s->steps > s->pred_step && (double)D > s->pred_disp) {
+        const double rate = ((double)D - s->pred_disp) / (double)(s->steps - s->pred_step);
+        const double left = 0.5 * (0.99 * s->skin_min - (double)D) / rate;
+        chunk = std::max<int64_t>(4, std::min<int64_t>(chunk, (int64_t)std::max(0.0, left)));
+    }
+    s->lean_chunk = chunk;
+    s->pred_disp = (double)D;
+    s->pred_step = s->steps;
+    s->pred_rebuilds = rb;
 }
 
 int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
@@ -1205,20 +1285,41 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
     while (left > 0) {
         const int64_t steps0 = s->steps;
         const int cur0 = s->cur;
-        const unsigned long long done0 = s->done_count;
-        int32_t rc = enqueue_steps(s, left);
+        const unsigned long long done0 = s->done_count, rebuilds0 = s->h_ctr[CT_REBUILDS];
+        const bool lean = s->P.skin && s->lean_chunk > 0;
+        const int64_t n = lean ? std::min(left, s->lean_chunk) : left;
+        int32_t rc = enqueue_steps(s, n, n == left, lean);
         if (rc) return rc;
         if ((rc = sync_counters(s))) return rc;
         s->done_count = s->h_ctr[CT_DONE];
-        if (!s->h_ctr[CT_FAIL]) break;
-        // A row overflowed: the steps from the failed one on changed nothing.  Resume from the
-        // state after the steps that completed, with wider rows.
+        if (!s->h_ctr[CT_FAIL]) {
+            left -= n;
+            if (lean) {
+                next_lean_chunk(s);
+            } else if (s->P.skin && (s->h_ctr[CT_REBUILDS] - rebuilds0) * 2 * LEAN_CHUNK_MIN <=
+                                        (unsigned long long)n) {
+                s->lean_chunk = LEAN_CHUNK_MIN;
+            }
+            continue;
+        }
+        // A step failed: it and the later steps changed nothing.  Resume from the state after the
+        // steps that completed.
         const int64_t done = (int64_t)(s->done_count - done0);
         s->steps = steps0 + done;
         s->cur = (done & 1) ? 1 - cur0 : cur0;
         left -= done;
         s->skin_dirty = true;  // the failed steps' broad phases may have rewritten the near lists
+        const bool abort = s->h_ctr[CT_ABORT] && !s->h_ctr[CT_MAXROW];
         HIPCHK(s, run_rows_finish(s, true));
+        if (abort) {
+            // a lean step had to rebuild: the next chunk starts with a full step there, and is
+            // as long as the interval since the previous such rebuild
+            const int64_t gap = s->steps - s->last_rebuild_step;
+            s->last_rebuild_step = s->steps;
+            s->lean_chunk = gap < LEAN_CHUNK_MIN ? 0 : std::min(gap, LEAN_CHUNK_MAX);
+            continue;
+        }
+        // a row overflowed: widen the rows
         if (s->K >= MAX_SLOTS) {
             HIPCHK(s, hipStreamSynchronize(s->stream));
             return fail(s, DEM_ERR_CAPACITY,

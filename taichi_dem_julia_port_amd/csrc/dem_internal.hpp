@@ -43,6 +43,8 @@ enum CounterIdx {
     CT_MAXROW = 16,        // the longest row an overflowing particle needed (copy 0, maximum)
     CT_REBUILDS = 17,      // cumulative: completed steps that rebuilt the sort and near lists (copy 0)
     CT_NBNEED = 18,        // the longest near list a particle needed when one overflowed (copy 0)
+    CT_ABORT = 19,         // a lean step found that it had to rebuild: it and the later steps of the
+                           // call did nothing (copy 0; CT_FAIL is set with it)
     CT_SKIN = 20,          // CT_SKIN + c: the step of buffer parity c rebuilds (copy 0; two words)
     CT_NCOUNTERS = 22,     // the host sums copies of the counters below this index
     CT_DISP = 22           // CT_DISP + c: FP32 bits of a bound on every particle's displacement since
@@ -87,6 +89,11 @@ __device__ __forceinline__ void fail_step(unsigned long long* ctr, uint32_t need
 // CT_SKIN + c says whether the step of buffer parity c rebuilds; the step of parity c clears
 // the other parity's word in its first kernel, and its broad phase (when a near list
 // overflows) and force kernel (when a particle has moved too far) set it.
+//
+// A lean step is the merge and the force kernel alone, the merge testing the near lists; its
+// merge clears the other parity's words itself.  A lean step whose flag word asks for a rebuild
+// fails the step instead (CT_ABORT with CT_FAIL): it and every later step of the call change
+// nothing, and the host replays from it with a full step.
 __device__ __forceinline__ bool skin_rebuilds(const unsigned long long* ctr, int cur) {
     return ctr[CT_SKIN + cur] != 0ull;
 }
@@ -285,9 +292,23 @@ struct dem_scene {
     std::string err;
     // two steps captured as one HIP graph (the particle buffers alternate, so two steps restore
     // the buffer parity), one graph for each buffer parity cur it starts from; replayed for the
-    // body of every dem_step call
+    // body of every dem_step call.  The lean graphs hold two lean steps.
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+    hipGraph_t graph_lean[2] = {nullptr, nullptr};
+    hipGraphExec_t graph_lean_exec[2] = {nullptr, nullptr};
+    int graph_lean_tpb[2] = {0, 0};
+    // lean steps run in chunks of at most lean_chunk steps between host checks: twice as long
+    // after a chunk without a rebuild, the last interval between rebuilds after one
+    int64_t lean_chunk = 64;
+    int64_t last_rebuild_step = 0;
+    // the displacement bound D at the last counter read, its step and the rebuild count then: two
+    // reads with no rebuild between them give D's growth per step, from which the next chunk
+    // ends before D reaches the skin
+    double pred_disp = 0.0;
+    int64_t pred_step = 0;
+    unsigned long long pred_rebuilds = 0;
+    double skin_min = 0.0;  // the smallest skin delta of the scene's (batch's) scenes
     // threads per block of the force kernel: 64 when the last counted step had at least one row
     // entry per four particles, else 256; each graph holds the value it was captured with
     int force_tpb = 256;
@@ -318,6 +339,8 @@ enum KernelId { K_HASH = 0, K_SCAN, K_SCATTER, K_BROAD, K_MERGE, K_FORCE, K_INIT
 // passes when the scene has them on)
 hipError_t run_contact_detection(dem_scene* s, bool init, bool last);
 hipError_t run_force_integrate(dem_scene* s, bool write_forces);
+// one lean step: the merge over the near lists (or the abort) and the force kernel
+hipError_t run_lean_step(dem_scene* s);
 // after a bond pass or a failed step: commit the pending rows (rollback = false), or drop them,
 // clear the touching counts and the failure flags (rollback = true)
 hipError_t run_rows_finish(dem_scene* s, bool rollback);

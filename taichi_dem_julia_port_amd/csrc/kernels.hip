@@ -883,13 +883,37 @@ k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restr
         uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
         uint8_t* __restrict__ sel, uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1,
         uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
-        double* __restrict__ st1, unsigned long long* ctr, int diag, int skin_cur) {
+        double* __restrict__ st1, unsigned long long* ctr, int diag, int skin_cur, int lean) {
     __shared__ uint32_t lt[MERGE_LDS * BS];  // touching partners
     __shared__ uint32_t lw[MERGE_LDS * BS];  // old-row words
+    if (lean) {
+        // a lean step: nothing after a failed step of the call; a step that has to rebuild fails
+        // the call instead; otherwise this kernel starts the step (the next step's words).  One
+        // thread decides for the block, so the block leaves or stays as a whole.
+        __shared__ int s_go;
+        if (threadIdx.x == 0) {
+            int go = 1;
+            if (step_failed(ctr)) {
+                go = 0;
+            } else if (skin_rebuilds(ctr, skin_cur)) {
+                atomicOr(&ctr[CT_ABORT], 1ull);
+                atomicOr(&ctr[CT_FAIL], 1ull);
+                go = 0;
+            } else if (blockIdx.x == 0) {
+                ctr[CT_SKIN + 1 - skin_cur] = 0ull;
+            }
+            s_go = go;
+        }
+        __syncthreads();
+        if (!s_go) return;
+        if (blockIdx.x == 0)
+            for (int c = threadIdx.x; c < CTR_COPIES; c += BS)
+                ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
+    }
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
     const int64_t n = P.n;
     const int K = P.K;
-    const bool reuse = !INIT && skin_cur >= 0 && !skin_rebuilds(ctr, skin_cur);
+    const bool reuse = !INIT && skin_cur >= 0 && (lean || !skin_rebuilds(ctr, skin_cur));
     uint64_t created = 0, deact = 0, entries = 0, found = 0;
     uint32_t overflow = 0, need = 0;
     // twice the displacement bound D (rounded up): a particle whose near-list gaps at the rebuild
@@ -1604,7 +1628,8 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
 
 // After a bond pass (rollback = 0): commit the rows it rebuilt.  After a failed step
 // (rollback = 1): keep the committed rows, clear the touching counts the skipped merges left, and
-// clear the failure flags (copy 0 of the flags; every copy of the overflow count).
+// clear the failure flags (copy 0 of the flags, the lean-step abort; every copy of the overflow
+// count).
 __global__ void __launch_bounds__(TPB)
 k_rows_finish(int64_t n, uint8_t* __restrict__ sel, uint32_t* __restrict__ tcnt,
               unsigned long long* __restrict__ ctr, int rollback) {
@@ -1616,7 +1641,7 @@ k_rows_finish(int64_t n, uint8_t* __restrict__ sel, uint32_t* __restrict__ tcnt,
     }
     if (rollback && blockIdx.x == 0) {
         for (int c = threadIdx.x; c < CTR_COPIES; c += TPB) ctr[c * CTR_STRIDE + CT_OVERFLOW] = 0ull;
-        if (threadIdx.x == 0) { ctr[CT_FAIL] = 0ull; ctr[CT_MAXROW] = 0ull; }
+        if (threadIdx.x == 0) { ctr[CT_FAIL] = 0ull; ctr[CT_MAXROW] = 0ull; ctr[CT_ABORT] = 0ull; }
     }
 }
 
@@ -1970,18 +1995,18 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
                                               s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
                                               d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag, skin_cur));
+                                              d.ctr, diag, skin_cur, 0));
     } else if (s->force_tpb == 64) {
         // contact-dense, as the force kernel's block choice: one-wave blocks for uneven rows
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), lds,
                                               s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
                                               d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag, skin_cur));
+                                              d.ctr, diag, skin_cur, 0));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
                                               s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
                                               d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag, skin_cur));
+                                              d.ctr, diag, skin_cur, 0));
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
@@ -2036,6 +2061,30 @@ static hipError_t force_impl(dem_scene* s, bool write_forces) {
 
 hipError_t run_force_integrate(dem_scene* s, bool write_forces) {
     return s->batched ? force_impl<true>(s, write_forces) : force_impl<false>(s, write_forces);
+}
+
+template <bool SEG>
+static hipError_t lean_impl(dem_scene* s) {
+    DevState& d = s->d;
+    const Params& P = s->P;
+    const int c = s->cur;
+    if (!P.skin || !s->dense) return hipErrorInvalidValue;
+    if (s->force_tpb == 64) {
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), 0,
+                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch,
+                                              d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0],
+                                              d.st[1], d.ctr, 0, c, 1));
+    } else {
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), grid_for(P.n, MERGE_TPB),
+                                              dim3(MERGE_TPB), 0, s->stream, P, d.pos[c], d.nbc, d.nb,
+                                              d.gmin, d.tcnt, d.touch, d.sel, d.rcnt[0], d.rcnt[1],
+                                              d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, 0, c, 1));
+    }
+    return force_impl<SEG>(s, false);
+}
+
+hipError_t run_lean_step(dem_scene* s) {
+    return s->batched ? lean_impl<true>(s) : lean_impl<false>(s);
 }
 
 hipError_t run_rows_finish(dem_scene* s, bool rollback) {
