@@ -162,6 +162,8 @@ typedef struct dem_counters {
     int64_t grid_cells;        /* cells of the engine's dense sort grid (0: reference grid used) */
     int64_t rebuilds;          /* cumulative steps that sorted and ran the broad phase (every step
                                   without near lists) */
+    int64_t replays;           /* cumulative lean steps (merge and force kernel only) that found
+                                  they had to rebuild, and were replayed as full steps */
 } dem_counters_t;
 
 typedef struct dem_scene dem_scene_t;

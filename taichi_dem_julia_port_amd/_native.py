@@ -105,7 +105,7 @@ class DemCounters(ctypes.Structure):
     _fields_ = [(name, ctypes.c_int64) for name in (
         "step", "n", "hash_table_size", "candidate_pairs", "contacts", "bonded", "hertz",
         "wall_contacts", "created", "deactivated", "bonds_broken", "bonds_broken_total",
-        "fatal_gap", "overflow", "examined", "grid_cells", "rebuilds")]
+        "fatal_gap", "overflow", "examined", "grid_cells", "rebuilds", "replays")]
 
 
 DEM_NUM_KERNELS = 8

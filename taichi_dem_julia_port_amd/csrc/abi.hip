@@ -1312,6 +1312,7 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
         const bool abort = s->h_ctr[CT_ABORT] && !s->h_ctr[CT_MAXROW];
         HIPCHK(s, run_rows_finish(s, true));
         if (abort) {
+            ++s->replays;
             // a lean step had to rebuild: the next chunk starts with a full step there, and is
             // as long as the interval since the previous such rebuild
             const int64_t gap = s->steps - s->last_rebuild_step;
@@ -1376,6 +1377,7 @@ int32_t dem_get_counters(dem_scene_t* s, dem_counters_t* out) {
     out->examined = s->diag_valid ? (int64_t)c[CT_EXAMINED] : -1;
     out->grid_cells = s->dense ? (int64_t)s->P.sort_keys : 0;
     out->rebuilds = s->P.skin ? (int64_t)c[CT_REBUILDS] : s->steps;
+    out->replays = s->replays;
     return DEM_OK;
 }
 

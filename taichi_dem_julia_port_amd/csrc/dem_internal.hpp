@@ -302,6 +302,7 @@ struct dem_scene {
     // after a chunk without a rebuild, the last interval between rebuilds after one
     int64_t lean_chunk = 64;
     int64_t last_rebuild_step = 0;
+    int64_t replays = 0;  // lean steps that had to rebuild (replayed as full steps)
     // the displacement bound D at the last counter read, its step and the rebuild count then: two
     // reads with no rebuild between them give D's growth per step, from which the next chunk
     // ends before D reaches the skin
