@@ -259,7 +259,8 @@ def run_serving(args):
     from taichi_dem_julia_port_amd import serving
     reqs = serving.make_stream(args.requests, rate=args.rate, seed=5)
     work = sum(r.n * r.steps for r in reqs)
-    srv = serving.Server(precision=args.precision, epoch_steps=args.epoch, poll_steps=args.poll)
+    srv = serving.Server(precision=args.precision, epoch_steps=args.epoch,
+                         poll_steps=args.poll or None)
     done, makespan = srv.run(reqs)
     out_b = serving.summarize(done, makespan)
     base, makespan0 = serving.serve_one_at_a_time(serving.make_stream(args.requests, rate=args.rate, seed=5),
@@ -273,8 +274,9 @@ def run_serving(args):
         "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": (f"{args.requests} requests, Poisson arrivals at {args.rate}/s, grains "
                                 f"log-uniform 1k-200k, half bonded cubes / half loose pours, steps "
-                                f"uniform 2k-50k, epochs of up to {args.epoch} steps ended early "
-                                f"by an arrival after a slice of {args.poll} steps"),
+                                f"uniform 2k-50k, epochs of up to {args.epoch} steps"
+                                + (f", ended early by an arrival after a slice of {args.poll} "
+                                   f"steps" if args.poll else "")),
                    "precision": args.precision, "particle_steps_requested": work},
         "batched": out_b, "repacks": srv.repacks,
         "repack_s": round(srv.repack_s, 3), "stepping_s": round(srv.step_s, 3),
@@ -331,8 +333,9 @@ def main():
     ap.add_argument("--requests", type=int, default=40)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--epoch", type=int, default=1000)
-    ap.add_argument("--poll", type=int, default=100,
-                    help="serving: steps per slice of an epoch (an arrival ends the epoch)")
+    ap.add_argument("--poll", type=int, default=0,
+                    help="serving: steps per slice of an epoch, after which an arrival ends the "
+                         "epoch (default: the whole epoch)")
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="default: mixed for the serving stream, fp64 otherwise")

@@ -1288,10 +1288,13 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
         const unsigned long long done0 = s->done_count, rebuilds0 = s->h_ctr[CT_REBUILDS];
         const bool lean = s->P.skin && s->lean_chunk > 0;
         const int64_t n = lean ? std::min(left, s->lean_chunk) : left;
-        int32_t rc = enqueue_steps(s, n, n == left, lean);
+        // per-kernel timing: this chunk's launches are kept only if none of its steps failed
+        int32_t rc = collect_timing(s);
         if (rc) return rc;
+        if ((rc = enqueue_steps(s, n, n == left, lean))) return rc;
         if ((rc = sync_counters(s))) return rc;
         s->done_count = s->h_ctr[CT_DONE];
+        if (s->h_ctr[CT_FAIL]) s->nev = 0;
         if (!s->h_ctr[CT_FAIL]) {
             left -= n;
             if (lean) {

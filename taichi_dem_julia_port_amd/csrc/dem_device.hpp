@@ -218,10 +218,9 @@ __device__ __forceinline__ uint32_t dense_cell(const DenseGrid& g, d3 p, int32_t
     const double r[3] = {p.x, p.y, p.z};
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        int32_t v = (int32_t)floor((r[a] - g.origin[a]) * g.inv_cell);
-        v = v < 0 ? 0 : v;
-        v = v >= g.dims[a] ? g.dims[a] - 1 : v;
-        c[a] = v;
+        // clamped to the grid; a non-finite coordinate goes to cell 0
+        const double f = floor((r[a] - g.origin[a]) * g.inv_cell);
+        c[a] = f >= 0.0 ? (f < (double)g.dims[a] ? (int32_t)f : g.dims[a] - 1) : 0;
     }
     return (uint32_t)c[0] + (uint32_t)g.dims[0] * ((uint32_t)c[1] + (uint32_t)g.dims[1] * (uint32_t)c[2]);
 }

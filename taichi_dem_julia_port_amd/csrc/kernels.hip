@@ -570,7 +570,7 @@ k_broad_dense(const Params P, const uint32_t* __restrict__ sidx, const uint32_t*
             const float ex = me32.x - o.x, ey = me32.y - o.y, ez = me32.z - o.z;
             const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
             const float thr = fmaf(o.w, fmult, c0);
-            return !(fmaf(-thr, thr, d2) > 0.0f);
+            return fmaf(-thr, thr, d2) <= 0.0f;  // false for a non-finite position
         };
         // one batch of candidates o[] at slots rlo + i + u of a range of rlen slots: survivors go
         // to the wave's queue, confirmed below by all lanes together; a survivor that finds the
@@ -1590,10 +1590,14 @@ k_force(const Params P, const double4* __restrict__ pos, const double4* __restri
             const double chk = S.skin * (1.0 - 1e-6) -
                                0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) + fabs((double)xr.z));
             const double d2 = ex * ex + ey * ey + ez * ez;
-            moved = !(chk > 0.0) || !(d2 < chk * chk);
+            // A grain whose position is not finite (the reference's exactly normal impacts make
+            // NaN grains) touches nothing in any step, so it never needs a rebuild.
+            const bool finite = isfinite(knew.x) && isfinite(knew.y) && isfinite(knew.z);
+            moved = finite && (!(chk > 0.0) || !(d2 < chk * chk));
             // the displacement bound for the next step: |x_new - x_rebuild| rounded up
-            disp = __double2float_ru((sqrt(d2) + 0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) +
-                                                            fabs((double)xr.z))) * (1.0 + 1e-12));
+            if (finite)
+                disp = __double2float_ru((sqrt(d2) + 0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) +
+                                                                fabs((double)xr.z))) * (1.0 + 1e-12));
         }
         // streaming stores: the next state is not read again before this launch ends
         store_nt(pos_n + p, make_double4(knew.x, knew.y, knew.z, X.w));

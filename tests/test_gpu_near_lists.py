@@ -33,19 +33,25 @@ def _Scene():
     return Scene
 
 
+def bits(x):
+    """The raw bits of an array (so NaN grains compare too)."""
+    x = np.ascontiguousarray(x)
+    return x.view(np.uint64) if x.dtype == np.float64 else x
+
+
 def same_state(a, b):
     ca, ba = a.read_contacts()
     cb, bb = b.read_contacts()
     assert np.array_equal(pair_keys(ca), pair_keys(cb)), "contact set differs"
     assert np.array_equal(ba, bb), "bonded flags differ"
     for f in CONTACT_FIELDS:
-        assert np.array_equal(ca[f], cb[f]), f"contact field {f}"
+        assert np.array_equal(bits(ca[f]), bits(cb[f])), f"contact field {f}"
     ga, gb = a.read_grains(), b.read_grains()
     for f in GRAIN_FIELDS:
-        assert np.array_equal(ga[f], gb[f]), f"grain field {f}"
+        assert np.array_equal(bits(ga[f]), bits(gb[f])), f"grain field {f}"
     Fa, Ta = a.read_forces()
     Fb, Tb = b.read_forces()
-    assert np.array_equal(Fa, Fb) and np.array_equal(Ta, Tb), "force sums differ"
+    assert np.array_equal(bits(Fa), bits(Fb)) and np.array_equal(bits(Ta), bits(Tb)), "force sums differ"
     na, nb = a.counters(), b.counters()
     for k in na:
         # near lists need cells 2 delta wider, and their broad phase examines more candidates
@@ -183,3 +189,35 @@ def test_near_list_overflow_widens_the_lists():
         assert np.count_nonzero(ca["i"] == 1) >= 30, "the shell must close in on the big grain"
         # the first step's lists overflowed; after the widening they hold the shell
         assert c["rebuilds"] < 30, c
+
+
+def test_nan_grains_do_not_force_rebuilds():
+    """Grains dropped straight onto the floor: an exactly normal impact makes the reference's
+    NaN (its slip ratio divides by a zero tangential force).  A NaN grain touches nothing, so it
+    must not make every step rebuild; the state, NaN grains included, is the full path's bit for
+    bit."""
+    n = 400
+    r = 0.5e-3
+    side = 20
+    ii, jj = np.meshgrid(np.arange(side), np.arange(side), indexing="ij")
+    k = np.stack([(ii.ravel() + 0.5) * 3 * r, (jj.ravel() + 0.5) * 3 * r,
+                  np.full(n, r + 2e-6)], axis=1)
+    V = np.full(n, 4.0 / 3.0 * math.pi * r ** 3)
+    v = np.zeros((n, 3))
+    v[:, 2] = -0.5
+    g = grains_in(np.arange(1, n + 1), V, scenes.DENSITY * V, k, v)
+    cfg = DEMConfig(domain_min=[-0.01, -0.01, -0.01], domain_max=[0.04, 0.04, 0.02],
+                    particle_elastic_modulus=1e7, pp_coefficient_friction=0.4,
+                    gravity=[0.0, 0.0, -9.81], dt=2e-7, max_coordinate_number=16,
+                    wall_normal=[0.0, 0.0, -1.0], wall_distance=0.0)
+    Scene = _Scene()
+    print("[case] 400 grains dropped straight onto the floor", flush=True)
+    with Scene(cfg, g, None) as near, Scene(cfg, g, None, near_lists=False) as full:
+        for s in (near, full):
+            s.step(3000)  # an impact lasts about 1000 steps
+        c = same_state(near, full)
+        nan = ~np.isfinite(near.read_grains()["k"]).all(axis=1)
+        print(f"[nan] {int(nan.sum())} grains not finite, {c['rebuilds']} rebuilds in {c['step']} "
+              f"steps", flush=True)
+        assert nan.any(), "no grain went non-finite: the case does not test what it names"
+        assert c["rebuilds"] < 20, c
