@@ -870,166 +870,221 @@ __device__ __forceinline__ bool touching_pair(const SceneP& S, const double4 mo,
     return !S.ref_check || ref_candidate(S.grid, mk3(xl.x, xl.y, xl.z), mk3(xh.x, xh.y, xh.z));
 }
 
+constexpr int MERGE_LDS = 16;
+
+// The arrays the merge reads and writes.
+struct MergeArgs {
+    const double4* __restrict__ pos;  // positions of the step (the current buffer)
+    const uint32_t* __restrict__ nbc; const uint32_t* __restrict__ nb; const uint32_t* __restrict__ gmin;
+    uint32_t* __restrict__ tcnt; uint32_t* __restrict__ touch;
+    uint8_t* __restrict__ sel;
+    uint32_t* __restrict__ rcnt0; uint32_t* __restrict__ rcnt1;
+    uint32_t* __restrict__ pq0; uint32_t* __restrict__ pq1;
+    double* __restrict__ st0; double* __restrict__ st1;
+};
+
+// What a wave of the merge counts.
+struct MergeTally {
+    uint64_t created = 0, deact = 0, entries = 0, found = 0;
+    uint32_t overflow = 0, need = 0;
+};
+
+// The merge of particle p: its new touching set (from the broad phase, or on a step that reuses
+// the near lists the near-list entries that pass the broad phase's test) merged into its row.
+// lt, lw: this thread's column of the block's LDS stage (stride BS).  twoD: twice the
+// displacement bound D (rounded up); a particle whose near-list gaps at the rebuild all exceed it
+// touches none of those partners now.
+// A particle's merge inputs, loaded first (independent loads): the touching (or near-list) count,
+// the smallest near-list gap, the row selector and both row counts.
+struct MergeIn {
+    uint32_t cnt_in, c0, c1;
+    float gm;
+    uint8_t sl;
+};
+__device__ __forceinline__ MergeIn load_merge_in(const MergeArgs& M, uint32_t p, bool reuse) {
+    MergeIn m;
+    m.cnt_in = reuse ? M.nbc[p] : M.tcnt[p];
+    m.gm = reuse ? __uint_as_float(M.gmin[p]) : 0.0f;
+    m.sl = M.sel[p];
+    m.c0 = M.rcnt0[p];
+    m.c1 = M.rcnt1[p];
+    return m;
+}
+
+template <bool INIT, bool SEG, int BS>
+__device__ __forceinline__ void merge_particle(const Params& P, const MergeArgs& M, uint32_t p,
+                                               const MergeIn& in, uint32_t* lt, uint32_t* lw,
+                                               bool reuse, float twoD, MergeTally& t) {
+    const int64_t n = P.n;
+    const int K = P.K;
+    // The committed buffer: after a failed step of the call a row may still be marked
+    // pending; this merge then overwrites the uncommitted buffer again, which the host drops.
+    const uint32_t cnt_in = in.cnt_in;
+    const float gm = in.gm;
+    const int b = in.sl & 1;
+    const uint32_t nold = b ? in.c1 : in.c0;
+    uint32_t nt = cnt_in;
+    bool t_lds = false;  // the touching partners are in lt already
+    if (reuse) {
+        // the first MERGE_LDS touching partners go to lt, later ones (when there are more) to
+        // the touching row, which then also takes a copy of the first ones
+        nt = 0;
+        const uint32_t nn = gm > twoD ? 0u : min(cnt_in, (uint32_t)P.KN);
+        if (nn) {
+            const SceneP& S = scene_of<SEG>(P, p);
+            const double4 mo = M.pos[p];
+            // entry m + 1's word and partner position are loaded while entry m is tested
+            uint32_t w = M.nb[p];
+            double4 o = M.pos[w & IDX_MASK];
+            for (uint32_t m = 0; m < nn; ++m) {
+                uint32_t wn = 0u;
+                double4 on = mo;
+                if (m + 1 < nn) {
+                    wn = M.nb[(int64_t)(m + 1) * n + p];
+                    on = M.pos[wn & IDX_MASK];
+                }
+                if (touching_pair(S, mo, o, (w & IS_I) != 0u)) {
+                    if (w & IS_I) ++t.found;
+                    if (nt < (uint32_t)MERGE_LDS) {
+                        lt[nt * BS] = w;
+                    } else {
+                        if (nt == (uint32_t)MERGE_LDS)
+                            for (uint32_t a = 0; a < (uint32_t)MERGE_LDS && (int)a < K; ++a)
+                                M.touch[(int64_t)a * n + p] = lt[a * BS];
+                        if ((int)nt < K) M.touch[(int64_t)nt * n + p] = w;
+                    }
+                    ++nt;
+                }
+                w = wn;
+                o = on;
+            }
+        }
+        t_lds = nt <= (uint32_t)MERGE_LDS;
+    }
+    t.need = max(t.need, nt);
+    if (nt | nold) {
+        const SceneP& S = scene_of<SEG>(P, p);
+        if (nt && !reuse) M.tcnt[p] = 0u;
+        uint32_t overflow = 0;
+        if ((int)nt > K) { overflow = 1; nt = K; }
+        const uint32_t* pqo = b ? M.pq1 : M.pq0;
+        uint64_t entries = 0;
+        if (nt <= (uint32_t)MERGE_LDS && nold <= (uint32_t)MERGE_LDS) {
+            for (uint32_t m = 0; m < (t_lds ? 0u : nt); m += 8) {
+                uint32_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = m + u < nt ? M.touch[(int64_t)(m + u) * n + p] : 0u;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) if (m + u < nt) lt[(m + u) * BS] = v[u];
+            }
+            for (uint32_t m = 0; m < nold; m += 8) {
+                uint32_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = m + u < nold ? pqo[(int64_t)(m + u) * n + p] : 0u;
+#pragma unroll
+                for (int u = 0; u < 8; ++u) if (m + u < nold) lw[(m + u) * BS] = v[u];
+            }
+            const uint32_t nu = sort_unique(
+                nt, [&](uint32_t m) { return lt[m * BS]; },
+                [&](uint32_t m, uint32_t v) { lt[m * BS] = v; });
+            merge_rows<INIT>(S, M.pos, p, n, K, nold, nu, b,
+                             [&](uint32_t m) { return lw[m * BS]; },
+                             [&](uint32_t m) { return lt[m * BS]; }, M.sel, M.rcnt0,
+                             M.rcnt1, M.pq0, M.pq1, M.st0, M.st1, t.created, t.deact, entries, overflow);
+        } else {
+            if (t_lds)
+                for (uint32_t a = 0; a < nt; ++a) M.touch[(int64_t)a * n + p] = lt[a * BS];
+            // long rows: the touching list is sorted in place (it is rebuilt every step)
+            const uint32_t nu = sort_unique(
+                nt, [&](uint32_t m) { return M.touch[(int64_t)m * n + p]; },
+                [&](uint32_t m, uint32_t v) { M.touch[(int64_t)m * n + p] = v; });
+            merge_rows<INIT>(S, M.pos, p, n, K, nold, nu, b,
+                             [&](uint32_t m) { return pqo[(int64_t)m * n + p]; },
+                             [&](uint32_t m) { return M.touch[(int64_t)m * n + p]; }, M.sel, M.rcnt0,
+                             M.rcnt1, M.pq0, M.pq1, M.st0, M.st1, t.created, t.deact, entries, overflow);
+        }
+        t.entries += entries;
+        if (overflow) {
+            t.overflow = 1;
+            t.need = max(t.need, (uint32_t)entries);
+        }
+    }
+}
+
+// The start of a lean step, decided by one thread for the block (so the block leaves or stays as
+// a whole): nothing after a failed step of the call; a step that has to rebuild fails the call
+// instead; otherwise the step runs and block 0 clears the next step's words.  Returns whether the
+// block runs the step.
+__device__ __forceinline__ bool lean_step_begins(unsigned long long* ctr, int skin_cur, int bs) {
+    __shared__ int s_go;
+    if (threadIdx.x == 0) {
+        int go = 1;
+        if (step_failed(ctr)) {
+            go = 0;
+        } else if (skin_rebuilds(ctr, skin_cur)) {
+            atomicOr(&ctr[CT_ABORT], 1ull);
+            atomicOr(&ctr[CT_FAIL], 1ull);
+            go = 0;
+        } else if (blockIdx.x == 0) {
+            ctr[CT_SKIN + 1 - skin_cur] = 0ull;
+        }
+        s_go = go;
+    }
+    __syncthreads();
+    if (!s_go) return false;
+    if (blockIdx.x == 0)
+        for (int c = threadIdx.x; c < CTR_COPIES; c += bs)
+            ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
+    return true;
+}
+
+// Twice the displacement bound D of the step of parity skin_cur (rounded up), the maximum over the
+// counter copies, for the whole block.
+__device__ __forceinline__ float twice_disp_bound(const unsigned long long* ctr, int skin_cur) {
+    __shared__ float s_disp;
+    if (threadIdx.x < 64) {
+        float v = 0.0f;
+        for (int c = threadIdx.x; c < CTR_COPIES; c += 64)
+            v = fmaxf(v, __uint_as_float((uint32_t)ctr[c * CTR_STRIDE + CT_DISP + skin_cur]));
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+        if (threadIdx.x == 0) s_disp = v;
+    }
+    __syncthreads();
+    return 2.0f * s_disp * (1.0f + 1e-6f);
+}
+
+__device__ __forceinline__ void merge_counters(unsigned long long* ctr, bool diag, bool reuse,
+                                               const MergeTally& t) {
+    if (diag) {
+        wave_add(ctr_at(ctr, CT_CREATED), t.created);
+        wave_add(ctr_at(ctr, CT_DEACT), t.deact);
+        wave_add(ctr_at(ctr, CT_ENTRIES), t.entries);
+        if (reuse) wave_add(ctr_at(ctr, CT_TOUCH), t.found);
+    }
+    // the row this particle needs: its merged length, or at least its touching count
+    if (t.overflow) fail_step(ctr, t.need);
+}
+
 // The merge kernel.  Rows of up to MERGE_LDS touching partners and old entries are staged in LDS
 // with batched independent loads; longer rows are merged in place in global memory.  On a step
 // that reuses the near lists the touching partners are the near-list entries that pass the
 // broad phase's test, found here (the broad phase did not run).
-constexpr int MERGE_LDS = 16;
-
 template <bool INIT, bool SEG, int BS>
 __global__ void __launch_bounds__(BS)
-k_merge(const Params P, const double4* __restrict__ pos, const uint32_t* __restrict__ nbc,
-        const uint32_t* __restrict__ nb, const uint32_t* __restrict__ gmin,
-        uint32_t* __restrict__ tcnt, uint32_t* __restrict__ touch,
-        uint8_t* __restrict__ sel, uint32_t* __restrict__ rcnt0, uint32_t* __restrict__ rcnt1,
-        uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
-        double* __restrict__ st1, unsigned long long* ctr, int diag, int skin_cur, int lean) {
+k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, int skin_cur, int lean) {
     __shared__ uint32_t lt[MERGE_LDS * BS];  // touching partners
     __shared__ uint32_t lw[MERGE_LDS * BS];  // old-row words
-    if (lean) {
-        // a lean step: nothing after a failed step of the call; a step that has to rebuild fails
-        // the call instead; otherwise this kernel starts the step (the next step's words).  One
-        // thread decides for the block, so the block leaves or stays as a whole.
-        __shared__ int s_go;
-        if (threadIdx.x == 0) {
-            int go = 1;
-            if (step_failed(ctr)) {
-                go = 0;
-            } else if (skin_rebuilds(ctr, skin_cur)) {
-                atomicOr(&ctr[CT_ABORT], 1ull);
-                atomicOr(&ctr[CT_FAIL], 1ull);
-                go = 0;
-            } else if (blockIdx.x == 0) {
-                ctr[CT_SKIN + 1 - skin_cur] = 0ull;
-            }
-            s_go = go;
-        }
-        __syncthreads();
-        if (!s_go) return;
-        if (blockIdx.x == 0)
-            for (int c = threadIdx.x; c < CTR_COPIES; c += BS)
-                ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
-    }
+    if (lean && !lean_step_begins(ctr, skin_cur, BS)) return;
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    const int64_t n = P.n;
-    const int K = P.K;
     const bool reuse = !INIT && skin_cur >= 0 && (lean || !skin_rebuilds(ctr, skin_cur));
-    uint64_t created = 0, deact = 0, entries = 0, found = 0;
-    uint32_t overflow = 0, need = 0;
-    // twice the displacement bound D (rounded up): a particle whose near-list gaps at the rebuild
-    // all exceed it touches none of those partners now
-    float twoD = 0.0f;
-    if (reuse) {
-        __shared__ float s_disp;
-        if (threadIdx.x < 64) {
-            float v = 0.0f;
-            for (int c = threadIdx.x; c < CTR_COPIES; c += 64)
-                v = fmaxf(v, __uint_as_float((uint32_t)ctr[c * CTR_STRIDE + CT_DISP + skin_cur]));
-            for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-            if (threadIdx.x == 0) s_disp = v;
-        }
-        __syncthreads();
-        twoD = 2.0f * s_disp * (1.0f + 1e-6f);
-    }
-    if (p64 < n) {
+    const float twoD = reuse ? twice_disp_bound(ctr, skin_cur) : 0.0f;
+    MergeTally t;
+    if (p64 < P.n) {
         const uint32_t p = (uint32_t)p64;
-        const int tid = threadIdx.x;
-        // independent loads first: the touching (or near-list) count, the row selector and counts.
-        // The committed buffer: after a failed step of the call a row may still be marked
-        // pending; this merge then overwrites the uncommitted buffer again, which the host drops.
-        const uint32_t cnt_in = reuse ? nbc[p] : tcnt[p];
-        const float gm = reuse ? __uint_as_float(gmin[p]) : 0.0f;
-        const int b = sel[p] & 1;
-        const uint32_t c0 = rcnt0[p], c1 = rcnt1[p];
-        const uint32_t nold = b ? c1 : c0;
-        uint32_t nt = cnt_in;
-        bool t_lds = false;  // the touching partners are in lt already
-        if (reuse) {
-            // the first MERGE_LDS touching partners go to lt, later ones (when there are more) to
-            // the touching row, which then also takes a copy of the first ones
-            nt = 0;
-            const uint32_t nn = gm > twoD ? 0u : min(cnt_in, (uint32_t)P.KN);
-            if (nn) {
-                const SceneP& S = scene_of<SEG>(P, p);
-                const double4 mo = pos[p];
-                // entry m + 1's word and partner position are loaded while entry m is tested
-                uint32_t w = nb[p];
-                double4 o = pos[w & IDX_MASK];
-                for (uint32_t m = 0; m < nn; ++m) {
-                    uint32_t wn = 0u;
-                    double4 on = mo;
-                    if (m + 1 < nn) {
-                        wn = nb[(int64_t)(m + 1) * n + p];
-                        on = pos[wn & IDX_MASK];
-                    }
-                    if (touching_pair(S, mo, o, (w & IS_I) != 0u)) {
-                        if (w & IS_I) ++found;
-                        if (nt < (uint32_t)MERGE_LDS) {
-                            lt[nt * BS + tid] = w;
-                        } else {
-                            if (nt == (uint32_t)MERGE_LDS)
-                                for (uint32_t a = 0; a < (uint32_t)MERGE_LDS && (int)a < K; ++a)
-                                    touch[(int64_t)a * n + p] = lt[a * BS + tid];
-                            if ((int)nt < K) touch[(int64_t)nt * n + p] = w;
-                        }
-                        ++nt;
-                    }
-                    w = wn;
-                    o = on;
-                }
-            }
-            t_lds = nt <= (uint32_t)MERGE_LDS;
-        }
-        need = nt;
-        if (nt | nold) {
-            const SceneP& S = scene_of<SEG>(P, p);
-            if (nt && !reuse) tcnt[p] = 0u;
-            if ((int)nt > K) { overflow = 1; nt = K; }
-            const uint32_t* pqo = b ? pq1 : pq0;
-            if (nt <= (uint32_t)MERGE_LDS && nold <= (uint32_t)MERGE_LDS) {
-                for (uint32_t m = 0; m < (t_lds ? 0u : nt); m += 8) {
-                    uint32_t v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) v[u] = m + u < nt ? touch[(int64_t)(m + u) * n + p] : 0u;
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) if (m + u < nt) lt[(m + u) * BS + tid] = v[u];
-                }
-                for (uint32_t m = 0; m < nold; m += 8) {
-                    uint32_t v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) v[u] = m + u < nold ? pqo[(int64_t)(m + u) * n + p] : 0u;
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) if (m + u < nold) lw[(m + u) * BS + tid] = v[u];
-                }
-                const uint32_t nu = sort_unique(
-                    nt, [&](uint32_t m) { return lt[m * BS + tid]; },
-                    [&](uint32_t m, uint32_t v) { lt[m * BS + tid] = v; });
-                merge_rows<INIT>(S, pos, p, n, K, nold, nu, b,
-                                 [&](uint32_t m) { return lw[m * BS + tid]; },
-                                 [&](uint32_t m) { return lt[m * BS + tid]; }, sel, rcnt0,
-                                 rcnt1, pq0, pq1, st0, st1, created, deact, entries, overflow);
-            } else {
-                if (t_lds)
-                    for (uint32_t a = 0; a < nt; ++a) touch[(int64_t)a * n + p] = lt[a * BS + tid];
-                // long rows: the touching list is sorted in place (it is rebuilt every step)
-                const uint32_t nu = sort_unique(
-                    nt, [&](uint32_t m) { return touch[(int64_t)m * n + p]; },
-                    [&](uint32_t m, uint32_t v) { touch[(int64_t)m * n + p] = v; });
-                merge_rows<INIT>(S, pos, p, n, K, nold, nu, b,
-                                 [&](uint32_t m) { return pqo[(int64_t)m * n + p]; },
-                                 [&](uint32_t m) { return touch[(int64_t)m * n + p]; }, sel, rcnt0,
-                                 rcnt1, pq0, pq1, st0, st1, created, deact, entries, overflow);
-            }
-        }
+        merge_particle<INIT, SEG, BS>(P, M, p, load_merge_in(M, p, reuse), lt + threadIdx.x,
+                                      lw + threadIdx.x, reuse, twoD, t);
     }
-    if (diag) {
-        wave_add(ctr_at(ctr, CT_CREATED), created);
-        wave_add(ctr_at(ctr, CT_DEACT), deact);
-        wave_add(ctr_at(ctr, CT_ENTRIES), entries);
-        if (reuse) wave_add(ctr_at(ctr, CT_TOUCH), found);
-    }
-    // the row this particle needs: its merged length, or at least its touching count
-    if (overflow) fail_step(ctr, (uint32_t)entries > need ? (uint32_t)entries : need);
+    merge_counters(ctr, diag != 0, reuse, t);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1396,6 +1451,304 @@ __device__ __forceinline__ Contrib eval_contact(const SceneP& S, double dt, uint
     }
 }
 
+// The walls particle at kp with radius r touches (the test of kernels.jl:331, |k . n - d| < r),
+// bit j for wall j.  The planes are read four at a time, so their loads wait once per group
+// rather than once per wall (a wait per wall cost the 1M pour's force kernel 7.5 us of 68).
+__device__ __forceinline__ uint32_t walls_touched(const SceneP& S, d3 kp, double r) {
+    uint32_t m = 0u;
+    const int nw = S.nwall;
+    for (int j0 = 0; j0 < nw; j0 += 4) {
+        WallP w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + u < nw) w[u] = S.walls[j0 + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (j0 + u < nw && fabs(dot(kp, mk3(w[u].n[0], w[u].n[1], w[u].n[2])) - w[u].d) < r)
+                m |= 1u << (j0 + u);
+    }
+    return m;
+}
+
+// What a wave of a force kernel counts for the step's counters and the near-list check.
+struct ForceTally {
+    uint64_t broken = 0, nwallc = 0, fatal = 0, active = 0, active_b = 0;
+    bool moved = false;  // a particle's new position is delta or more from its rebuild position
+    float disp = 0.0f;   // and its distance from there (an upper bound), the largest of the lane's
+};
+
+// The state arrays a force kernel reads and writes.
+struct ForceArgs {
+    const double4* __restrict__ pos; const double4* __restrict__ vel;
+    const double4* __restrict__ omg; const double4* __restrict__ quat;
+    double4* __restrict__ pos_n; double4* __restrict__ vel_n;
+    double4* __restrict__ omg_n; double4* __restrict__ quat_n;
+    double4* __restrict__ acc; double4* __restrict__ dwo;
+    double4* __restrict__ ftot; double4* __restrict__ ttot;
+    uint8_t* __restrict__ sel;
+    const uint32_t* __restrict__ rcnt0; const uint32_t* __restrict__ rcnt1;
+    uint32_t* __restrict__ pq0; uint32_t* __restrict__ pq1;
+    double* __restrict__ st0; double* __restrict__ st1;
+    uint32_t* __restrict__ wact; double4* __restrict__ wshear;
+    float4* __restrict__ xref;
+};
+
+// apply_body_force! (kernels.jl:428-443) and update! (kernels.jl:445-485) of particle p from the
+// summed contact and wall force F and moment T; the near-list displacement check; the next state.
+// Q and xr: the orientation and rebuild position, already loaded (FP64) or loaded here (MIXED).
+template <bool MIXED>
+__device__ __forceinline__ void integrate_particle(const SceneP& S, const ForceArgs& A, uint32_t p,
+                                                   const double4& X, const double4& Vv,
+                                                   const double4& W, double4 Q, float4 xr, d3 F,
+                                                   d3 T, int write_forces, int skin_cur,
+                                                   bool rebuilt, ForceTally& t) {
+    const double dt = S.dt;
+    const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
+    F = F + mk3(S.g[0] * Vv.w, S.g[1] * Vv.w, S.g[2] * Vv.w);
+    const double nd = -S.damping;
+    F = F + mk3(nd * fabs(F.x) * jsign(vp.x), nd * fabs(F.y) * jsign(vp.y), nd * fabs(F.z) * jsign(vp.z));
+    T = T + mk3(nd * fabs(T.x) * jsign(wp.x), nd * fabs(T.y) * jsign(wp.y), nd * fabs(T.z) * jsign(wp.z));
+    if (write_forces) {
+        A.ftot[p] = make_double4(F.x, F.y, F.z, 0.0);
+        A.ttot[p] = make_double4(T.x, T.y, T.z, 0.0);
+    }
+    // update! (kernels.jl:445-485)
+    if constexpr (MIXED) Q = A.quat[p];
+    const d3 av = F / Vv.w;
+    const d3 knew = (kp + vp * dt) + (0.5 * av) * (dt * dt);
+    const d3 vnew = vp + av * dt;
+    const m3 Rq = quat_matrix(Q);
+    const d3 Tl = mul(Rq, T);
+    const d3 wl = mul(Rq, wp);
+    m3 I;
+    I.a[0][0] = W.w; I.a[0][1] = 0.0; I.a[0][2] = 0.0;
+    I.a[1][0] = 0.0; I.a[1][1] = W.w; I.a[1][2] = 0.0;
+    I.a[2][0] = 0.0; I.a[2][1] = 0.0; I.a[2][2] = W.w;
+    const m3 Iinv = inverse(I);
+    const d3 dwl = mul(Iinv, Tl - cross(wl, mul(I, wl)));
+    const m3 Rqi = inverse(Rq);
+    const d3 dw = mul(Rqi, dwl);
+    const double q0 = Q.x, q1 = Q.y, q2 = Q.z, q3 = Q.w;
+    const double dq0 = -0.5 * (q1 * wp.x + q2 * wp.y + q3 * wp.z);
+    const double dq1 = 0.5 * (q0 * wp.x + q2 * wp.z - q3 * wp.y);
+    const double dq2 = 0.5 * (q0 * wp.y + q1 * wp.z + q3 * wp.x);
+    const double dq3 = 0.5 * (q0 * wp.z + q1 * wp.y - q2 * wp.x);
+    const double n0 = q0 + dq0, n1 = q1 + dq1, n2 = q2 + dq2, n3 = q3 + dq3;
+    const double qn = sqrt(n0 * n0 + n1 * n1 + n2 * n2 + n3 * n3);
+    const d3 wnew = wp + dw * dt;
+    if (skin_cur >= 0) {
+        // |x_new - x_rebuild| < delta, with x_rebuild rounded to FP32: the bound is reduced by
+        // that rounding (at most 2^-24 |x| per axis) and by 1e-6 relative for the FP64 sums
+        if (rebuilt) {
+            xr = make_float4((float)X.x, (float)X.y, (float)X.z, 0.f);
+            A.xref[p] = xr;
+        } else if (MIXED) {
+            xr = A.xref[p];
+        }
+        const double ex = knew.x - (double)xr.x, ey = knew.y - (double)xr.y,
+                     ez = knew.z - (double)xr.z;
+        const double er = 0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) + fabs((double)xr.z));
+        const double chk = S.skin * (1.0 - 1e-6) - er;
+        const double d2 = ex * ex + ey * ey + ez * ez;
+        // A grain whose position is not finite (the reference's exactly normal impacts make
+        // NaN grains) touches nothing in any step, so it never needs a rebuild.
+        const bool finite = isfinite(knew.x) && isfinite(knew.y) && isfinite(knew.z);
+        t.moved |= finite && (!(chk > 0.0) || !(d2 < chk * chk));
+        // the displacement bound for the next step, at least sqrt(d2) + er: FP32 from values
+        // rounded up, each later rounding (at most an ulp, 2^-23) covered by a factor 1 + 2^-20
+        if (finite) {
+            const float up = 1.0f + 0x1p-20f;
+            const float b = (sqrtf(__double2float_ru(d2)) * up + __double2float_ru(er)) * up;
+            t.disp = fmaxf(t.disp, b);
+        }
+    }
+    // streaming stores: the next state is not read again before this launch ends
+    store_nt(A.pos_n + p, make_double4(knew.x, knew.y, knew.z, X.w));
+    store_nt(A.vel_n + p, make_double4(vnew.x, vnew.y, vnew.z, Vv.w));
+    store_nt(A.omg_n + p, make_double4(wnew.x, wnew.y, wnew.z, W.w));
+    store_nt(A.quat_n + p, make_double4(n0 / qn, n1 / qn, n2 / qn, n3 / qn));
+    if (write_forces) {  // a and dw are outputs of update!; the step never reads them back
+        A.acc[p] = make_double4(av.x, av.y, av.z, 0.0);
+        A.dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
+    }
+}
+
+// A particle's own state as the force kernels load it first: position, velocity, spin; in FP64
+// also the orientation, the rebuild position and the wall flags (one memory round trip less per
+// wave); mixed precision loads those after the row, where they need no registers during the
+// contact loop.
+struct Own {
+    double4 X, Vv, W, Q;
+    float4 xr;    // the position at the last rebuild
+    uint32_t wa;  // the walls this particle touched last step
+};
+template <bool MIXED>
+__device__ __forceinline__ Own load_own(const SceneP& S, const ForceArgs& A, uint32_t p,
+                                        int skin_cur, bool rebuilt) {
+    Own o;
+    o.X = A.pos[p]; o.Vv = A.vel[p]; o.W = A.omg[p];
+    o.Q = make_double4(0.0, 0.0, 0.0, 0.0);
+    o.xr = make_float4(0.f, 0.f, 0.f, 0.f);
+    o.wa = 0u;
+    if constexpr (!MIXED) {
+        o.Q = A.quat[p];
+        // loaded whether or not the step rebuilt: no state load waits for the step's flag word
+        if (skin_cur >= 0) o.xr = A.xref[p];
+        if (S.nwall) o.wa = A.wact[p];
+    }
+    return o;
+}
+
+// Contacts (kernels.jl:122-315), walls (kernels.jl:317-426), body force and update! of particle p,
+// whose own state `own` is loaded.  failed: a row of this step or an earlier one overflowed, so
+// nothing changes (tested before the first store, so the flag's load hides behind the state
+// loads); rebuilt: near lists in use (skin_cur >= 0) and this step rebuilt them.
+template <bool MIXED, bool SEG>
+__device__ __forceinline__ void force_particle(const Params& P, const ForceArgs& A, uint32_t p,
+                                               const Own& own, int write_forces, int skin_cur,
+                                               bool failed, bool rebuilt,
+                                               unsigned long long* ctr, ForceTally& t) {
+    const int64_t n = P.n;
+    const SceneP& S = scene_of<SEG>(P, p);
+    const double dt = S.dt;
+    const double4 X = own.X, Vv = own.Vv, W = own.W;
+    const double4 Q = own.Q;
+    const float4 xr = own.xr;
+    uint32_t wa = own.wa;
+    const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
+    // the walls touched now, tested before the row so the planes' loads overlap the row's
+    const uint32_t wtouch = walls_touched(S, kp, X.w);
+    d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
+    CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
+    // commit a row the merge rebuilt this step (it sits in the other buffer)
+    // both row counts with the selector (independent loads), the committed one chosen after
+    const uint8_t sl = A.sel[p];
+    const uint32_t c0 = A.rcnt0[p], c1 = A.rcnt1[p];
+    const int b = (sl & ROW_PENDING) ? ((sl & 1) ^ 1) : sl;
+    if (failed) return;
+    if (sl & ROW_PENDING) A.sel[p] = (uint8_t)b;
+    uint32_t* pq = b ? A.pq1 : A.pq0;
+    double* st = b ? A.st1 : A.st0;
+    const uint32_t cnt = b ? c1 : c0;
+    const int64_t cs = (int64_t)P.K * n;
+    const double inv_r = cnt ? 1.0 / X.w : 0.0, inv_m = cnt ? 1.0 / Vv.w : 0.0;
+    // software pipeline over the row: entry m+1's partner state and entry m+2's word are
+    // loaded while entry m is evaluated
+    uint32_t w_cur = 0u, w_nxt = 0u;
+    double4 Xq_c = make_double4(0.0, 0.0, 0.0, 0.0), Vq_c = Xq_c, Wq_c = Xq_c;
+    if (cnt > 0) {
+        w_cur = pq[p];
+        if (cnt > 1) w_nxt = pq[n + p];
+        const uint32_t q0 = w_cur & IDX_MASK;
+        Xq_c = A.pos[q0]; Vq_c = A.vel[q0]; Wq_c = A.omg[q0];
+    }
+    for (uint32_t m = 0; m < cnt; ++m) {
+        const int64_t e = (int64_t)m * n + p;
+        const uint32_t word = w_cur;
+        const double4 Xq = Xq_c, Vq = Vq_c, Wq = Wq_c;
+        if (m + 1 < cnt) {
+            const uint32_t qn = w_nxt & IDX_MASK;
+            Xq_c = A.pos[qn]; Vq_c = A.vel[qn]; Wq_c = A.omg[qn];
+            w_cur = w_nxt;
+            if (m + 2 < cnt) w_nxt = pq[(int64_t)(m + 2) * n + p];
+        }
+        const bool plo = (word & IS_I) != 0u;  // p is the contact's i (the lower file index)
+        // canonical orientation: i is the lower file index, j the higher
+        const d3 ki = plo ? kp : mk3(Xq.x, Xq.y, Xq.z);
+        const d3 kj = plo ? mk3(Xq.x, Xq.y, Xq.z) : kp;
+        const d3 vi = plo ? vp : mk3(Vq.x, Vq.y, Vq.z);
+        const d3 vj = plo ? mk3(Vq.x, Vq.y, Vq.z) : vp;
+        const d3 wi = plo ? wp : mk3(Wq.x, Wq.y, Wq.z);
+        const d3 wj = plo ? mk3(Wq.x, Wq.y, Wq.z) : wp;
+        const double ri = plo ? X.w : Xq.w, rj = plo ? Xq.w : X.w;
+        const double mi = plo ? Vv.w : Vq.w, mj = plo ? Vq.w : Vv.w;
+        const Contrib o = eval_contact<MIXED>(S, dt, word, plo, ki, kj, vi, vj, wi, wj, ri, rj, mi,
+                                              mj, inv_r, inv_m, st, cs, e, write_forces);
+        if (o.broke) {
+            pq[e] = word | BROKEN;  // both copies take the same decision
+            if (plo) {
+                ++t.broken;
+                if (SEG) atomicAdd(&P.seg_broken[P.sid[p]], 1ull);
+            }
+            continue;
+        }
+        if (o.kind == CK_FP32) {
+            if (plo) { FH.add(tof(o.F)); TH.add(tof(o.Ti)); }
+            else { FH.add(tof(-o.F)); TH.add(tof(o.Tj)); }
+        } else {
+            if (plo) { F = F + o.F; T = T + o.Ti; }
+            else { F = F + (-o.F); T = T + o.Tj; }
+        }
+        if (plo) {
+            ++t.active;
+            if (o.bonded) ++t.active_b;
+            if (o.fatal) ++t.fatal;
+        }
+    }
+    if (MIXED) {
+        F = F + FH.value();
+        T = T + TH.value();
+    }
+    // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
+    if (MIXED && S.nwall) wa = A.wact[p];
+    for (uint32_t wm = wtouch; wm; wm &= wm - 1u) {  // in wall order
+        const int j = __ffs(wm) - 1;
+        const WallP& wl = S.walls[j];
+        const d3 nrm = mk3(wl.n[0], wl.n[1], wl.n[2]);
+        {
+            ++t.nwallc;
+            const m3 R = contact_frame(nrm, S.tol);
+            const double L = dot(kp, nrm) - wl.d;
+            const double gap = fabs(L) - X.w;
+            const double dn = fabs(gap);
+            const d3 kip = (((-L) * nrm) / fabs(L)) * (fabs(L) + dn / 2.0);
+            const d3 vci = cross(wp, kip) + vp;
+            const d3 vc = mul(R, -vci);
+            const int mj = wl.mid - 1;
+            const SurfaceP& sf = S.surf[0][mj];
+            d3 s0 = mk3(0.0, 0.0, 0.0);
+            if (wa & (1u << j)) {
+                const double4 ws = A.wshear[(int64_t)j * n + p];
+                s0 = mk3(ws.x, ws.y, ws.z);
+            }
+            const HMOut h = hertz_mindlin(gap, dn, vc, S.ysm[mj], S.gsm[mj], X.w, Vv.w, sf, dt, s0);
+            if (h.fatal) ++t.fatal;
+            const m3 Ri = inverse(R);
+            const d3 Fg = mul(Ri, -h.F);
+            F = F + Fg;
+            T = T + cross(kip, Fg);
+            A.wshear[(int64_t)j * n + p] = make_double4(h.s.x, h.s.y, h.s.z, 0.0);
+        }
+    }
+    if (S.nwall && wtouch != wa) A.wact[p] = wtouch;
+    integrate_particle<MIXED>(S, A, p, X, Vv, W, Q, xr, F, T, write_forces, skin_cur, rebuilt, t);
+}
+
+// A force kernel's counters, per wave (every lane calls it).  done: this launch ends the step
+// (counts it, and its rebuild).
+__device__ __forceinline__ void force_counters(unsigned long long* ctr, int skin_cur, bool rebuilt,
+                                               int write_forces, bool done, ForceTally& t) {
+    if (done && blockIdx.x == 0 && threadIdx.x == 0) {
+        atomicAdd(&ctr[CT_DONE], 1ull);
+        if (rebuilt) atomicAdd(&ctr[CT_REBUILDS], 1ull);
+    }
+    if (skin_cur >= 0) {
+        if (__ballot(t.moved) != 0ull && (threadIdx.x & 63) == 0) skin_request(ctr, skin_cur);
+        float disp = t.disp;
+        for (int o = 32; o > 0; o >>= 1) disp = fmaxf(disp, __shfl_xor(disp, o, 64));
+        if ((threadIdx.x & 63) == 0 && disp > 0.0f)
+            atomicMax(ctr_at(ctr, CT_DISP + 1 - skin_cur), (unsigned long long)__float_as_uint(disp));
+    }
+    wave_add(ctr_at(ctr, CT_BROKEN), t.broken);
+    wave_add(ctr_at(ctr, CT_BROKEN_TOTAL), t.broken);
+    if (write_forces) {
+        wave_add(ctr_at(ctr, CT_WALL), t.nwallc);
+        wave_add(ctr_at(ctr, CT_FATAL), t.fatal);
+        wave_add(ctr_at(ctr, CT_ACTIVE), t.active);
+        wave_add(ctr_at(ctr, CT_ACTIVE_BOND), t.active_b);
+    }
+}
+
 // The single-scene mixed kernel carries both contact laws in two precisions; left alone the
 // compiler gives it more than 256 registers and one wave per SIMD, so it is held to two waves per
 // SIMD. The other variants fit in 256 and keep the schedule the compiler picks without the bound.
@@ -1404,230 +1757,20 @@ __device__ __forceinline__ Contrib eval_contact(const SceneP& S, double dt, uint
 // blocks stream the state 1 % faster.  The host picks by the last counted row entries per particle.
 template <bool MIXED, bool SEG, int BS>
 __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MIXED && !SEG ? 2 : 1)))
-k_force(const Params P, const double4* __restrict__ pos, const double4* __restrict__ vel,
-        const double4* __restrict__ omg, const double4* __restrict__ quat,
-        double4* __restrict__ pos_n, double4* __restrict__ vel_n, double4* __restrict__ omg_n,
-        double4* __restrict__ quat_n, double4* __restrict__ acc, double4* __restrict__ dwo,
-        double4* __restrict__ ftot, double4* __restrict__ ttot, uint8_t* __restrict__ sel,
-        const uint32_t* __restrict__ rcnt0, const uint32_t* __restrict__ rcnt1,
-        uint32_t* __restrict__ pq0, uint32_t* __restrict__ pq1, double* __restrict__ st0,
-        double* __restrict__ st1, uint32_t* __restrict__ wact, double4* __restrict__ wshear,
-        unsigned long long* ctr, int write_forces, float4* __restrict__ xref, int skin_cur) {
-    // a row of this step or an earlier one overflowed: change nothing.  Loaded first and tested
-    // before the first store, so the load's latency hides behind the state loads.
+k_force(const Params P, const ForceArgs A, unsigned long long* ctr, int write_forces, int skin_cur) {
     const bool failed = step_failed(ctr);
     // near lists in use (skin_cur >= 0): did this step rebuild them?
     const bool rebuilt = skin_cur >= 0 && skin_rebuilds(ctr, skin_cur);
-    bool moved = false;  // this particle's new position is delta or more from its rebuild position
-    float disp = 0.0f;   // and its distance from there (an upper bound)
     // XCD-contiguous blocks: partners are spatial neighbours, so their state is in this XCD's L2
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    const int64_t n = P.n;
-    uint64_t broken = 0, nwallc = 0, fatal = 0, active = 0, active_b = 0;
-    if (p64 < n) {
+    ForceTally t;
+    if (p64 < P.n) {
         const uint32_t p = (uint32_t)p64;
-        const SceneP& S = scene_of<SEG>(P, p);
-        const double dt = S.dt;
-        const double4 X = pos[p], Vv = vel[p], W = omg[p];
-        // the orientation: with the state in FP64 it is loaded with the rest (one memory round
-        // trip less per wave); mixed precision loads it after the row, where it needs no registers
-        // during the contact loop
-        double4 Q = make_double4(0.0, 0.0, 0.0, 0.0);
-        float4 xr = make_float4(0.f, 0.f, 0.f, 0.f);  // the position at the last rebuild
-        if constexpr (!MIXED) {
-            Q = quat[p];
-            if (skin_cur >= 0 && !rebuilt) xr = xref[p];
-        }
-        uint32_t wa = 0u;  // the walls this particle touched last step
-        if (!MIXED && S.nwall) wa = wact[p];
-        const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
-        d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
-        CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
-        // commit a row the merge rebuilt this step (it sits in the other buffer)
-        const uint8_t sl = sel[p];
-        const int b = (sl & ROW_PENDING) ? ((sl & 1) ^ 1) : sl;
-        if (failed) return;
-        if (sl & ROW_PENDING) sel[p] = (uint8_t)b;
-        uint32_t* pq = b ? pq1 : pq0;
-        double* st = b ? st1 : st0;
-        const uint32_t cnt = (b ? rcnt1 : rcnt0)[p];
-        const int64_t cs = (int64_t)P.K * n;
-        const double inv_r = cnt ? 1.0 / X.w : 0.0, inv_m = cnt ? 1.0 / Vv.w : 0.0;
-        // software pipeline over the row: entry m+1's partner state and entry m+2's word are
-        // loaded while entry m is evaluated
-        uint32_t w_cur = 0u, w_nxt = 0u;
-        double4 Xq_c = make_double4(0.0, 0.0, 0.0, 0.0), Vq_c = Xq_c, Wq_c = Xq_c;
-        if (cnt > 0) {
-            w_cur = pq[p];
-            if (cnt > 1) w_nxt = pq[n + p];
-            const uint32_t q0 = w_cur & IDX_MASK;
-            Xq_c = pos[q0]; Vq_c = vel[q0]; Wq_c = omg[q0];
-        }
-        for (uint32_t m = 0; m < cnt; ++m) {
-            const int64_t e = (int64_t)m * n + p;
-            const uint32_t word = w_cur;
-            const double4 Xq = Xq_c, Vq = Vq_c, Wq = Wq_c;
-            if (m + 1 < cnt) {
-                const uint32_t qn = w_nxt & IDX_MASK;
-                Xq_c = pos[qn]; Vq_c = vel[qn]; Wq_c = omg[qn];
-                w_cur = w_nxt;
-                if (m + 2 < cnt) w_nxt = pq[(int64_t)(m + 2) * n + p];
-            }
-            const bool plo = (word & IS_I) != 0u;  // p is the contact's i (the lower file index)
-            // canonical orientation: i is the lower file index, j the higher
-            const d3 ki = plo ? kp : mk3(Xq.x, Xq.y, Xq.z);
-            const d3 kj = plo ? mk3(Xq.x, Xq.y, Xq.z) : kp;
-            const d3 vi = plo ? vp : mk3(Vq.x, Vq.y, Vq.z);
-            const d3 vj = plo ? mk3(Vq.x, Vq.y, Vq.z) : vp;
-            const d3 wi = plo ? wp : mk3(Wq.x, Wq.y, Wq.z);
-            const d3 wj = plo ? mk3(Wq.x, Wq.y, Wq.z) : wp;
-            const double ri = plo ? X.w : Xq.w, rj = plo ? Xq.w : X.w;
-            const double mi = plo ? Vv.w : Vq.w, mj = plo ? Vq.w : Vv.w;
-            const Contrib o = eval_contact<MIXED>(S, dt, word, plo, ki, kj, vi, vj, wi, wj, ri, rj, mi,
-                                                  mj, inv_r, inv_m, st, cs, e, write_forces);
-            if (o.broke) {
-                pq[e] = word | BROKEN;  // both copies take the same decision
-                if (plo) {
-                    ++broken;
-                    if (SEG) atomicAdd(&P.seg_broken[P.sid[p]], 1ull);
-                }
-                continue;
-            }
-            if (o.kind == CK_FP32) {
-                if (plo) { FH.add(tof(o.F)); TH.add(tof(o.Ti)); }
-                else { FH.add(tof(-o.F)); TH.add(tof(o.Tj)); }
-            } else {
-                if (plo) { F = F + o.F; T = T + o.Ti; }
-                else { F = F + (-o.F); T = T + o.Tj; }
-            }
-            if (plo) {
-                ++active;
-                if (o.bonded) ++active_b;
-                if (o.fatal) ++fatal;
-            }
-        }
-        if (MIXED) {
-            F = F + FH.value();
-            T = T + TH.value();
-        }
-        // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
-        if (MIXED && S.nwall) wa = wact[p];
-        uint32_t wa_new = 0u;
-        for (int j = 0; j < S.nwall; ++j) {
-            const WallP& wl = S.walls[j];
-            const d3 nrm = mk3(wl.n[0], wl.n[1], wl.n[2]);
-            if (fabs(dot(kp, nrm) - wl.d) < X.w) {
-                ++nwallc;
-                const m3 R = contact_frame(nrm, S.tol);
-                const double L = dot(kp, nrm) - wl.d;
-                const double gap = fabs(L) - X.w;
-                const double dn = fabs(gap);
-                const d3 kip = (((-L) * nrm) / fabs(L)) * (fabs(L) + dn / 2.0);
-                const d3 vci = cross(wp, kip) + vp;
-                const d3 vc = mul(R, -vci);
-                const int mj = wl.mid - 1;
-                const SurfaceP& sf = S.surf[0][mj];
-                d3 s0 = mk3(0.0, 0.0, 0.0);
-                if (wa & (1u << j)) {
-                    const double4 ws = wshear[(int64_t)j * n + p];
-                    s0 = mk3(ws.x, ws.y, ws.z);
-                }
-                const HMOut h = hertz_mindlin(gap, dn, vc, S.ysm[mj], S.gsm[mj], X.w, Vv.w, sf, dt, s0);
-                if (h.fatal) ++fatal;
-                const m3 Ri = inverse(R);
-                const d3 Fg = mul(Ri, -h.F);
-                F = F + Fg;
-                T = T + cross(kip, Fg);
-                wshear[(int64_t)j * n + p] = make_double4(h.s.x, h.s.y, h.s.z, 0.0);
-                wa_new |= 1u << j;
-            }
-        }
-        if (S.nwall && wa_new != wa) wact[p] = wa_new;
-        // apply_body_force! (kernels.jl:428-443)
-        F = F + mk3(S.g[0] * Vv.w, S.g[1] * Vv.w, S.g[2] * Vv.w);
-        const double nd = -S.damping;
-        F = F + mk3(nd * fabs(F.x) * jsign(vp.x), nd * fabs(F.y) * jsign(vp.y), nd * fabs(F.z) * jsign(vp.z));
-        T = T + mk3(nd * fabs(T.x) * jsign(wp.x), nd * fabs(T.y) * jsign(wp.y), nd * fabs(T.z) * jsign(wp.z));
-        if (write_forces) {
-            ftot[p] = make_double4(F.x, F.y, F.z, 0.0);
-            ttot[p] = make_double4(T.x, T.y, T.z, 0.0);
-        }
-        // update! (kernels.jl:445-485)
-        if constexpr (MIXED) Q = quat[p];
-        const d3 av = F / Vv.w;
-        const d3 knew = (kp + vp * dt) + (0.5 * av) * (dt * dt);
-        const d3 vnew = vp + av * dt;
-        const m3 Rq = quat_matrix(Q);
-        const d3 Tl = mul(Rq, T);
-        const d3 wl = mul(Rq, wp);
-        m3 I;
-        I.a[0][0] = W.w; I.a[0][1] = 0.0; I.a[0][2] = 0.0;
-        I.a[1][0] = 0.0; I.a[1][1] = W.w; I.a[1][2] = 0.0;
-        I.a[2][0] = 0.0; I.a[2][1] = 0.0; I.a[2][2] = W.w;
-        const m3 Iinv = inverse(I);
-        const d3 dwl = mul(Iinv, Tl - cross(wl, mul(I, wl)));
-        const m3 Rqi = inverse(Rq);
-        const d3 dw = mul(Rqi, dwl);
-        const double q0 = Q.x, q1 = Q.y, q2 = Q.z, q3 = Q.w;
-        const double dq0 = -0.5 * (q1 * wp.x + q2 * wp.y + q3 * wp.z);
-        const double dq1 = 0.5 * (q0 * wp.x + q2 * wp.z - q3 * wp.y);
-        const double dq2 = 0.5 * (q0 * wp.y + q1 * wp.z + q3 * wp.x);
-        const double dq3 = 0.5 * (q0 * wp.z + q1 * wp.y - q2 * wp.x);
-        const double n0 = q0 + dq0, n1 = q1 + dq1, n2 = q2 + dq2, n3 = q3 + dq3;
-        const double qn = sqrt(n0 * n0 + n1 * n1 + n2 * n2 + n3 * n3);
-        const d3 wnew = wp + dw * dt;
-        if (skin_cur >= 0) {
-            // |x_new - x_rebuild| < delta, with x_rebuild rounded to FP32: the bound is reduced by
-            // that rounding (at most 2^-24 |x| per axis) and by 1e-6 relative for the FP64 sums
-            if (rebuilt) {
-                xr = make_float4((float)X.x, (float)X.y, (float)X.z, 0.f);
-                xref[p] = xr;
-            } else if (MIXED) {
-                xr = xref[p];
-            }
-            const double ex = knew.x - (double)xr.x, ey = knew.y - (double)xr.y,
-                         ez = knew.z - (double)xr.z;
-            const double chk = S.skin * (1.0 - 1e-6) -
-                               0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) + fabs((double)xr.z));
-            const double d2 = ex * ex + ey * ey + ez * ez;
-            // A grain whose position is not finite (the reference's exactly normal impacts make
-            // NaN grains) touches nothing in any step, so it never needs a rebuild.
-            const bool finite = isfinite(knew.x) && isfinite(knew.y) && isfinite(knew.z);
-            moved = finite && (!(chk > 0.0) || !(d2 < chk * chk));
-            // the displacement bound for the next step: |x_new - x_rebuild| rounded up
-            if (finite)
-                disp = __double2float_ru((sqrt(d2) + 0x1p-22 * (fabs((double)xr.x) + fabs((double)xr.y) +
-                                                                fabs((double)xr.z))) * (1.0 + 1e-12));
-        }
-        // streaming stores: the next state is not read again before this launch ends
-        store_nt(pos_n + p, make_double4(knew.x, knew.y, knew.z, X.w));
-        store_nt(vel_n + p, make_double4(vnew.x, vnew.y, vnew.z, Vv.w));
-        store_nt(omg_n + p, make_double4(wnew.x, wnew.y, wnew.z, W.w));
-        store_nt(quat_n + p, make_double4(n0 / qn, n1 / qn, n2 / qn, n3 / qn));
-        if (write_forces) {  // a and dw are outputs of update!; the step never reads them back
-            acc[p] = make_double4(av.x, av.y, av.z, 0.0);
-            dwo[p] = make_double4(dw.x, dw.y, dw.z, 0.0);
-        }
+        const Own own = load_own<MIXED>(scene_of<SEG>(P, p), A, p, skin_cur, rebuilt);
+        force_particle<MIXED, SEG>(P, A, p, own, write_forces, skin_cur, failed, rebuilt, ctr, t);
     }
     if (failed) return;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        atomicAdd(&ctr[CT_DONE], 1ull);
-        if (rebuilt) atomicAdd(&ctr[CT_REBUILDS], 1ull);
-    }
-    if (skin_cur >= 0) {
-        if (__ballot(moved) != 0ull && (threadIdx.x & 63) == 0) skin_request(ctr, skin_cur);
-        for (int o = 32; o > 0; o >>= 1) disp = fmaxf(disp, __shfl_xor(disp, o, 64));
-        if ((threadIdx.x & 63) == 0 && disp > 0.0f)
-            atomicMax(ctr_at(ctr, CT_DISP + 1 - skin_cur), (unsigned long long)__float_as_uint(disp));
-    }
-    wave_add(ctr_at(ctr, CT_BROKEN), broken);
-    wave_add(ctr_at(ctr, CT_BROKEN_TOTAL), broken);
-    if (write_forces) {
-        wave_add(ctr_at(ctr, CT_WALL), nwallc);
-        wave_add(ctr_at(ctr, CT_FATAL), fatal);
-        wave_add(ctr_at(ctr, CT_ACTIVE), active);
-        wave_add(ctr_at(ctr, CT_ACTIVE_BOND), active_b);
-    }
+    force_counters(ctr, skin_cur, rebuilt, write_forces, true, t);
 }
 
 // After a bond pass (rollback = 0): commit the rows it rebuilt.  After a failed step
@@ -1929,6 +2072,27 @@ static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) 
                       : sort_impl<false>(s, reset_counters, dense);
 }
 
+static MergeArgs merge_args(dem_scene* s) {
+    DevState& d = s->d;
+    MergeArgs M;
+    M.pos = d.pos[s->cur]; M.nbc = d.nbc; M.nb = d.nb; M.gmin = d.gmin; M.tcnt = d.tcnt;
+    M.touch = d.touch; M.sel = d.sel; M.rcnt0 = d.rcnt[0]; M.rcnt1 = d.rcnt[1];
+    M.pq0 = d.pq[0]; M.pq1 = d.pq[1]; M.st0 = d.st[0]; M.st1 = d.st[1];
+    return M;
+}
+
+static ForceArgs force_args(dem_scene* s) {
+    DevState& d = s->d;
+    const int c = s->cur, nx = 1 - s->cur;
+    ForceArgs A;
+    A.pos = d.pos[c]; A.vel = d.vel[c]; A.omg = d.omg[c]; A.quat = d.quat[c];
+    A.pos_n = d.pos[nx]; A.vel_n = d.vel[nx]; A.omg_n = d.omg[nx]; A.quat_n = d.quat[nx];
+    A.acc = d.acc; A.dwo = d.dw; A.ftot = d.ftot; A.ttot = d.ttot; A.sel = d.sel;
+    A.rcnt0 = d.rcnt[0]; A.rcnt1 = d.rcnt[1]; A.pq0 = d.pq[0]; A.pq1 = d.pq[1];
+    A.st0 = d.st[0]; A.st1 = d.st[1]; A.wact = d.wact; A.wshear = d.wshear; A.xref = d.xref;
+    return A;
+}
+
 template <bool SEG>
 static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
     // per-step counters are only read after the last step of a call; the diagnostic passes (the
@@ -1997,20 +2161,14 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
     const dim3 gm = grid_for(P.n, MERGE_TPB);
     if (init) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
-                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
-                                              d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag, skin_cur, 0));
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0));
     } else if (s->force_tpb == 64) {
         // contact-dense, as the force kernel's block choice: one-wave blocks for uneven rows
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), lds,
-                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
-                                              d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag, skin_cur, 0));
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
-                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch, d.sel,
-                                              d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0], d.st[1],
-                                              d.ctr, diag, skin_cur, 0));
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0));
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
@@ -2031,25 +2189,16 @@ hipError_t run_contact_detection(dem_scene* s, bool init, bool last) {
 
 template <bool SEG, int BS>
 static hipError_t launch_force(dem_scene* s, bool write_forces) {
-    DevState& d = s->d;
     const Params& P = s->P;
-    const int c = s->cur, nx = 1 - s->cur;
-    const int skin = (P.skin && s->dense) ? c : -1;
+    const int skin = (P.skin && s->dense) ? s->cur : -1;
     const dim3 g = grid_for(P.n, BS);
+    const ForceArgs A = force_args(s);
     if (s->precision == DEM_PRECISION_MIXED) {
         LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<true, SEG, BS>), g, dim3(BS), 0, s->stream, P,
-                                              d.pos[c], d.vel[c], d.omg[c], d.quat[c], d.pos[nx],
-                                              d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
-                                              d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
-                                              d.st[0], d.st[1], d.wact, d.wshear, d.ctr,
-                                              write_forces ? 1 : 0, d.xref, skin));
+                                              A, s->d.ctr, write_forces ? 1 : 0, skin));
     } else {
         LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<false, SEG, BS>), g, dim3(BS), 0, s->stream, P,
-                                              d.pos[c], d.vel[c], d.omg[c], d.quat[c], d.pos[nx],
-                                              d.vel[nx], d.omg[nx], d.quat[nx], d.acc, d.dw, d.ftot,
-                                              d.ttot, d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1],
-                                              d.st[0], d.st[1], d.wact, d.wshear, d.ctr,
-                                              write_forces ? 1 : 0, d.xref, skin));
+                                              A, s->d.ctr, write_forces ? 1 : 0, skin));
     }
     return hipSuccess;
 }
@@ -2075,14 +2224,10 @@ static hipError_t lean_impl(dem_scene* s) {
     if (!P.skin || !s->dense) return hipErrorInvalidValue;
     if (s->force_tpb == 64) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), 0,
-                                              s->stream, P, d.pos[c], d.nbc, d.nb, d.gmin, d.tcnt, d.touch,
-                                              d.sel, d.rcnt[0], d.rcnt[1], d.pq[0], d.pq[1], d.st[0],
-                                              d.st[1], d.ctr, 0, c, 1));
+                                              s->stream, P, merge_args(s), d.ctr, 0, c, 1));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), grid_for(P.n, MERGE_TPB),
-                                              dim3(MERGE_TPB), 0, s->stream, P, d.pos[c], d.nbc, d.nb,
-                                              d.gmin, d.tcnt, d.touch, d.sel, d.rcnt[0], d.rcnt[1],
-                                              d.pq[0], d.pq[1], d.st[0], d.st[1], d.ctr, 0, c, 1));
+                                              dim3(MERGE_TPB), 0, s->stream, P, merge_args(s), d.ctr, 0, c, 1));
     }
     return force_impl<SEG>(s, false);
 }
