@@ -896,15 +896,19 @@ struct MergeTally {
 // touches none of those partners now.
 // A particle's merge inputs, loaded first (independent loads): the touching (or near-list) count,
 // the smallest near-list gap, the row selector and both row counts.
+// `near`: near lists in use, so the step may reuse them (then the near-list count and smallest gap
+// are the inputs) or rebuild them (then the touching count is).
 struct MergeIn {
-    uint32_t cnt_in, c0, c1;
+    uint32_t tc, nc, c0, c1;
     float gm;
     uint8_t sl;
+    __device__ __forceinline__ uint32_t cnt_in(bool reuse) const { return reuse ? nc : tc; }
 };
-__device__ __forceinline__ MergeIn load_merge_in(const MergeArgs& M, uint32_t p, bool reuse) {
+__device__ __forceinline__ MergeIn load_merge_in(const MergeArgs& M, uint32_t p, bool near, bool lean) {
     MergeIn m;
-    m.cnt_in = reuse ? M.nbc[p] : M.tcnt[p];
-    m.gm = reuse ? __uint_as_float(M.gmin[p]) : 0.0f;
+    m.tc = lean ? 0u : M.tcnt[p];
+    m.nc = near ? M.nbc[p] : 0u;
+    m.gm = near ? __uint_as_float(M.gmin[p]) : 0.0f;
     m.sl = M.sel[p];
     m.c0 = M.rcnt0[p];
     m.c1 = M.rcnt1[p];
@@ -919,8 +923,8 @@ __device__ __forceinline__ void merge_particle(const Params& P, const MergeArgs&
     const int K = P.K;
     // The committed buffer: after a failed step of the call a row may still be marked
     // pending; this merge then overwrites the uncommitted buffer again, which the host drops.
-    const uint32_t cnt_in = in.cnt_in;
-    const float gm = in.gm;
+    const uint32_t cnt_in = in.cnt_in(reuse);
+    const float gm = reuse ? in.gm : 0.0f;
     const int b = in.sl & 1;
     const uint32_t nold = b ? in.c1 : in.c0;
     uint32_t nt = cnt_in;
@@ -1011,46 +1015,49 @@ __device__ __forceinline__ void merge_particle(const Params& P, const MergeArgs&
     }
 }
 
-// The start of a lean step, decided by one thread for the block (so the block leaves or stays as
-// a whole): nothing after a failed step of the call; a step that has to rebuild fails the call
-// instead; otherwise the step runs and block 0 clears the next step's words.  Returns whether the
-// block runs the step.
-__device__ __forceinline__ bool lean_step_begins(unsigned long long* ctr, int skin_cur, int bs) {
-    __shared__ int s_go;
-    if (threadIdx.x == 0) {
-        int go = 1;
-        if (step_failed(ctr)) {
-            go = 0;
-        } else if (skin_rebuilds(ctr, skin_cur)) {
-            atomicOr(&ctr[CT_ABORT], 1ull);
-            atomicOr(&ctr[CT_FAIL], 1ull);
-            go = 0;
-        } else if (blockIdx.x == 0) {
-            ctr[CT_SKIN + 1 - skin_cur] = 0ull;
-        }
-        s_go = go;
-    }
-    __syncthreads();
-    if (!s_go) return false;
-    if (blockIdx.x == 0)
-        for (int c = threadIdx.x; c < CTR_COPIES; c += bs)
-            ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
-    return true;
-}
-
-// Twice the displacement bound D of the step of parity skin_cur (rounded up), the maximum over the
-// counter copies, for the whole block.
-__device__ __forceinline__ float twice_disp_bound(const unsigned long long* ctr, int skin_cur) {
+// The merge's view of the step, decided for the whole block (so a block leaves or stays as a
+// whole), after the block's particle loads are issued: on a lean step nothing after a failed step
+// of the call, a step that has to rebuild fails the call instead, otherwise the step runs and
+// block 0 clears the next step's words; whether the step reuses the near lists; and twice the
+// displacement bound D of the step (rounded up, the maximum over the counter copies).
+struct MergeStep { bool go, reuse; float twoD; };
+__device__ __forceinline__ MergeStep merge_step_begins(unsigned long long* ctr, int skin_cur,
+                                                       bool near, bool lean, int bs) {
+    __shared__ int s_go, s_reuse;
     __shared__ float s_disp;
     if (threadIdx.x < 64) {
         float v = 0.0f;
-        for (int c = threadIdx.x; c < CTR_COPIES; c += 64)
-            v = fmaxf(v, __uint_as_float((uint32_t)ctr[c * CTR_STRIDE + CT_DISP + skin_cur]));
+        if (near)
+            for (int c = threadIdx.x; c < CTR_COPIES; c += 64)
+                v = fmaxf(v, __uint_as_float((uint32_t)ctr[c * CTR_STRIDE + CT_DISP + skin_cur]));
         for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-        if (threadIdx.x == 0) s_disp = v;
+        if (threadIdx.x == 0) {
+            int go = 1;
+            if (lean) {
+                if (step_failed(ctr)) {
+                    go = 0;
+                } else if (skin_rebuilds(ctr, skin_cur)) {
+                    atomicOr(&ctr[CT_ABORT], 1ull);
+                    atomicOr(&ctr[CT_FAIL], 1ull);
+                    go = 0;
+                } else if (blockIdx.x == 0) {
+                    ctr[CT_SKIN + 1 - skin_cur] = 0ull;
+                }
+            }
+            s_go = go;
+            s_reuse = near && (lean || !skin_rebuilds(ctr, skin_cur));
+            s_disp = v;
+        }
     }
     __syncthreads();
-    return 2.0f * s_disp * (1.0f + 1e-6f);
+    MergeStep m;
+    m.go = s_go != 0;
+    m.reuse = s_reuse != 0;
+    m.twoD = 2.0f * s_disp * (1.0f + 1e-6f);
+    if (m.go && lean && blockIdx.x == 0)
+        for (int c = threadIdx.x; c < CTR_COPIES; c += bs)
+            ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
+    return m;
 }
 
 __device__ __forceinline__ void merge_counters(unsigned long long* ctr, bool diag, bool reuse,
@@ -1074,17 +1081,17 @@ __global__ void __launch_bounds__(BS)
 k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, int skin_cur, int lean) {
     __shared__ uint32_t lt[MERGE_LDS * BS];  // touching partners
     __shared__ uint32_t lw[MERGE_LDS * BS];  // old-row words
-    if (lean && !lean_step_begins(ctr, skin_cur, BS)) return;
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
-    const bool reuse = !INIT && skin_cur >= 0 && (lean || !skin_rebuilds(ctr, skin_cur));
-    const float twoD = reuse ? twice_disp_bound(ctr, skin_cur) : 0.0f;
+    const bool in = p64 < P.n;
+    const uint32_t p = in ? (uint32_t)p64 : 0u;
+    const bool near = !INIT && skin_cur >= 0;
+    // the particle's loads first: the block's look at the step's words waits for them together
+    const MergeIn mi = load_merge_in(M, p, near, lean != 0);
+    const MergeStep st = merge_step_begins(ctr, skin_cur, near, lean != 0, BS);
+    if (!st.go) return;
     MergeTally t;
-    if (p64 < P.n) {
-        const uint32_t p = (uint32_t)p64;
-        merge_particle<INIT, SEG, BS>(P, M, p, load_merge_in(M, p, reuse), lt + threadIdx.x,
-                                      lw + threadIdx.x, reuse, twoD, t);
-    }
-    merge_counters(ctr, diag != 0, reuse, t);
+    if (in) merge_particle<INIT, SEG, BS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse, st.twoD, t);
+    merge_counters(ctr, diag != 0, st.reuse, t);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1457,13 +1464,13 @@ __device__ __forceinline__ Contrib eval_contact(const SceneP& S, double dt, uint
 __device__ __forceinline__ uint32_t walls_touched(const SceneP& S, d3 kp, double r) {
     uint32_t m = 0u;
     const int nw = S.nwall;
-    for (int j0 = 0; j0 < nw; j0 += 4) {
-        WallP w[4];
+    for (int j0 = 0; j0 < nw; j0 += 8) {
+        WallP w[8];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 8; ++u)
             if (j0 + u < nw) w[u] = S.walls[j0 + u];
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < 8; ++u)
             if (j0 + u < nw && fabs(dot(kp, mk3(w[u].n[0], w[u].n[1], w[u].n[2])) - w[u].d) < r)
                 m |= 1u << (j0 + u);
     }
