@@ -934,7 +934,37 @@ __device__ __forceinline__ void merge_particle(const Params& P, const MergeArgs&
         // the touching row, which then also takes a copy of the first ones
         nt = 0;
         const uint32_t nn = gm > twoD ? 0u : min(cnt_in, (uint32_t)P.KN);
-        if (nn) {
+        auto take = [&](uint32_t w) {  // a touching near-list entry, in list order
+            if (w & IS_I) ++t.found;
+            if (nt < (uint32_t)MERGE_LDS) {
+                lt[nt * BS] = w;
+            } else {
+                if (nt == (uint32_t)MERGE_LDS)
+                    for (uint32_t a = 0; a < (uint32_t)MERGE_LDS && (int)a < K; ++a)
+                        M.touch[(int64_t)a * n + p] = lt[a * BS];
+                if ((int)nt < K) M.touch[(int64_t)nt * n + p] = w;
+            }
+            ++nt;
+        };
+        if constexpr (BS == 64) {
+          if (nn) {
+            // contact-dense states (one-wave blocks): long lists, four entries at a time -- four
+            // word loads, then four partner gathers, then four tests
+            const SceneP& S = scene_of<SEG>(P, p);
+            const double4 mo = M.pos[p];
+            for (uint32_t m0 = 0; m0 < nn; m0 += 4) {
+                uint32_t w4[4];
+                double4 o4[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) w4[u] = m0 + u < nn ? M.nb[(int64_t)(m0 + u) * n + p] : 0u;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) o4[u] = m0 + u < nn ? M.pos[w4[u] & IDX_MASK] : mo;
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (m0 + u < nn && touching_pair(S, mo, o4[u], (w4[u] & IS_I) != 0u)) take(w4[u]);
+            }
+          }
+        } else if (nn) {
             const SceneP& S = scene_of<SEG>(P, p);
             const double4 mo = M.pos[p];
             // entry m + 1's word and partner position are loaded while entry m is tested
