@@ -1532,8 +1532,8 @@ struct ForceArgs {
 
 // apply_body_force! (kernels.jl:428-443) and update! (kernels.jl:445-485) of particle p from the
 // summed contact and wall force F and moment T; the near-list displacement check; the next state.
-// Q and xr: the orientation and rebuild position, already loaded (FP64) or loaded here (MIXED).
-template <bool MIXED>
+// Q and xr: the orientation and rebuild position, already loaded, or loaded here (LATE).
+template <bool LATE>
 __device__ __forceinline__ void integrate_particle(const SceneP& S, const ForceArgs& A, uint32_t p,
                                                    const double4& X, const double4& Vv,
                                                    const double4& W, double4 Q, float4 xr, d3 F,
@@ -1550,7 +1550,7 @@ __device__ __forceinline__ void integrate_particle(const SceneP& S, const ForceA
         A.ttot[p] = make_double4(T.x, T.y, T.z, 0.0);
     }
     // update! (kernels.jl:445-485)
-    if constexpr (MIXED) Q = A.quat[p];
+    if constexpr (LATE) Q = A.quat[p];
     const d3 av = F / Vv.w;
     const d3 knew = (kp + vp * dt) + (0.5 * av) * (dt * dt);
     const d3 vnew = vp + av * dt;
@@ -1579,7 +1579,7 @@ __device__ __forceinline__ void integrate_particle(const SceneP& S, const ForceA
         if (rebuilt) {
             xr = make_float4((float)X.x, (float)X.y, (float)X.z, 0.f);
             A.xref[p] = xr;
-        } else if (MIXED) {
+        } else if (LATE) {
             xr = A.xref[p];
         }
         const double ex = knew.x - (double)xr.x, ey = knew.y - (double)xr.y,
@@ -1610,16 +1610,17 @@ __device__ __forceinline__ void integrate_particle(const SceneP& S, const ForceA
     }
 }
 
-// A particle's own state as the force kernels load it first: position, velocity, spin; in FP64
-// also the orientation, the rebuild position and the wall flags (one memory round trip less per
-// wave); mixed precision loads those after the row, where they need no registers during the
-// contact loop.
+// A particle's own state as the force kernels load it first: position, velocity, spin, and unless
+// LATE also the orientation, the rebuild position and the wall flags (one memory round trip less
+// per wave).  LATE (mixed precision on contact-dense states) loads those after the row, where
+// they hold no registers during the contact loop: the mixed kernel is at its register limit, and
+// with them live across the loop it spills (packed 1M pour: 249 -> 267 us).
 struct Own {
     double4 X, Vv, W, Q;
     float4 xr;    // the position at the last rebuild
     uint32_t wa;  // the walls this particle touched last step
 };
-template <bool MIXED>
+template <bool LATE>
 __device__ __forceinline__ Own load_own(const SceneP& S, const ForceArgs& A, uint32_t p,
                                         int skin_cur, bool rebuilt) {
     Own o;
@@ -1627,7 +1628,7 @@ __device__ __forceinline__ Own load_own(const SceneP& S, const ForceArgs& A, uin
     o.Q = make_double4(0.0, 0.0, 0.0, 0.0);
     o.xr = make_float4(0.f, 0.f, 0.f, 0.f);
     o.wa = 0u;
-    if constexpr (!MIXED) {
+    if constexpr (!LATE) {
         o.Q = A.quat[p];
         // loaded whether or not the step rebuilt: no state load waits for the step's flag word
         if (skin_cur >= 0) o.xr = A.xref[p];
@@ -1640,7 +1641,7 @@ __device__ __forceinline__ Own load_own(const SceneP& S, const ForceArgs& A, uin
 // whose own state `own` is loaded.  failed: a row of this step or an earlier one overflowed, so
 // nothing changes (tested before the first store, so the flag's load hides behind the state
 // loads); rebuilt: near lists in use (skin_cur >= 0) and this step rebuilt them.
-template <bool MIXED, bool SEG>
+template <bool MIXED, bool SEG, bool LATE>
 __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs& A, uint32_t p,
                                                const Own& own, int write_forces, int skin_cur,
                                                bool failed, bool rebuilt,
@@ -1727,7 +1728,7 @@ __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs&
         T = T + TH.value();
     }
     // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
-    if (MIXED && S.nwall) wa = A.wact[p];
+    if (LATE && S.nwall) wa = A.wact[p];
     for (uint32_t wm = wtouch; wm; wm &= wm - 1u) {  // in wall order
         const int j = __ffs(wm) - 1;
         const WallP& wl = S.walls[j];
@@ -1758,7 +1759,7 @@ __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs&
         }
     }
     if (S.nwall && wtouch != wa) A.wact[p] = wtouch;
-    integrate_particle<MIXED>(S, A, p, X, Vv, W, Q, xr, F, T, write_forces, skin_cur, rebuilt, t);
+    integrate_particle<LATE>(S, A, p, X, Vv, W, Q, xr, F, T, write_forces, skin_cur, rebuilt, t);
 }
 
 // A force kernel's counters, per wave (every lane calls it).  done: this launch ends the step
@@ -1786,14 +1787,15 @@ __device__ __forceinline__ void force_counters(unsigned long long* ctr, int skin
     }
 }
 
-// The single-scene mixed kernel carries both contact laws in two precisions; left alone the
-// compiler gives it more than 256 registers and one wave per SIMD, so it is held to two waves per
-// SIMD. The other variants fit in 256 and keep the schedule the compiler picks without the bound.
+// The mixed kernel carries both contact laws in two precisions; left alone the compiler gives the
+// single-scene variants and the batched sparse-state variant more than 256 registers and one wave
+// per SIMD, so they are held to two waves per SIMD.  The other variants fit in 256 and keep the
+// schedule the compiler picks without the bound.
 // BS threads a block.  With contacts, rows differ in length and a one-wave block frees its slot as
 // soon as its wave is done (packed 1M pour: 269 us at 256 threads, 247 at 64); without, 256-thread
 // blocks stream the state 1 % faster.  The host picks by the last counted row entries per particle.
 template <bool MIXED, bool SEG, int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MIXED && !SEG ? 2 : 1)))
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MIXED && (!SEG || BS == 256) ? 2 : 1)))
 k_force(const Params P, const ForceArgs A, unsigned long long* ctr, int write_forces, int skin_cur) {
     const bool failed = step_failed(ctr);
     // near lists in use (skin_cur >= 0): did this step rebuild them?
@@ -1803,8 +1805,10 @@ k_force(const Params P, const ForceArgs A, unsigned long long* ctr, int write_fo
     ForceTally t;
     if (p64 < P.n) {
         const uint32_t p = (uint32_t)p64;
-        const Own own = load_own<MIXED>(scene_of<SEG>(P, p), A, p, skin_cur, rebuilt);
-        force_particle<MIXED, SEG>(P, A, p, own, write_forces, skin_cur, failed, rebuilt, ctr, t);
+        // mixed precision on contact-dense states (one-wave blocks) loads part of the state late
+        constexpr bool LATE = MIXED && BS == 64;
+        const Own own = load_own<LATE>(scene_of<SEG>(P, p), A, p, skin_cur, rebuilt);
+        force_particle<MIXED, SEG, LATE>(P, A, p, own, write_forces, skin_cur, failed, rebuilt, ctr, t);
     }
     if (failed) return;
     force_counters(ctr, skin_cur, rebuilt, write_forces, true, t);
