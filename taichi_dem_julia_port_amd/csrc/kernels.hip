@@ -1655,7 +1655,8 @@ __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs&
     uint32_t wa = own.wa;
     const d3 kp = mk3(X.x, X.y, X.z), vp = mk3(Vv.x, Vv.y, Vv.z), wp = mk3(W.x, W.y, W.z);
     // the walls touched now, tested before the row so the planes' loads overlap the row's
-    const uint32_t wtouch = walls_touched(S, kp, X.w);
+    // (after it with LATE, where no register may be held across the contact loop)
+    const uint32_t wtouch_early = LATE ? 0u : walls_touched(S, kp, X.w);
     d3 F = mk3(0.0, 0.0, 0.0), T = mk3(0.0, 0.0, 0.0);
     CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
     // commit a row the merge rebuilt this step (it sits in the other buffer)
@@ -1729,6 +1730,7 @@ __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs&
     }
     // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
     if (LATE && S.nwall) wa = A.wact[p];
+    const uint32_t wtouch = LATE ? walls_touched(S, kp, X.w) : wtouch_early;
     for (uint32_t wm = wtouch; wm; wm &= wm - 1u) {  // in wall order
         const int j = __ffs(wm) - 1;
         const WallP& wl = S.walls[j];
