@@ -946,10 +946,12 @@ __device__ __forceinline__ void merge_particle(const Params& P, const MergeArgs&
             }
             ++nt;
         };
-        if constexpr (BS == 64) {
+        if constexpr (BS == 64 && !SEG) {
           if (nn) {
-            // contact-dense states (one-wave blocks): long lists, four entries at a time -- four
-            // word loads, then four partner gathers, then four tests
+            // contact-dense single scenes (one-wave blocks): long lists, four entries at a time --
+            // four word loads, then four partner gathers, then four tests (packed 1M pour merge
+            // 97 -> 90 us; batches keep the one-deep walk, whose registers allow five waves per
+            // SIMD instead of three: 256-cube batch merge 101 against 112 us)
             const SceneP& S = scene_of<SEG>(P, p);
             const double4 mo = M.pos[p];
             for (uint32_t m0 = 0; m0 < nn; m0 += 4) {

@@ -81,7 +81,7 @@ def _oracle_solve(cfg_path, out_dir, max_steps, no_bond, parallel=False):
 
 
 @pytest.mark.parametrize("k, max_steps, no_bond, parallel", [
-    (1, 5000, False, False), (2, 5000, False, False), (2, 3000, True, False), (3, 20000, False, True)])
+    (1, 5000, False, False), (2, 5000, False, False), (2, 3000, True, False), (3, 10000, False, True)])
 def test_native_driver_output_passes_the_reference_comparers_against_the_oracle(
         tmp_path, k, max_steps, no_bond, parallel):
     print(f"[case] demhip_solve cfg{k} max_steps={max_steps} no_bond={no_bond} vs oracle", flush=True)
@@ -95,7 +95,7 @@ def test_native_driver_output_passes_the_reference_comparers_against_the_oracle(
     _oracle_solve(cfg, str(b), max_steps, no_bond, parallel)
     pa, pb = read_p4p(str(a / "output.p4p")), read_p4p(str(b / "output.p4p"))
     ca, cb = read_p4c(str(a / "output.p4c")), read_p4c(str(b / "output.p4c"))
-    assert len(pa) >= 3 and len(ca) == len(pa)
+    assert len(pa) >= 2 and len(ca) == len(pa)
     if not no_bond:
         assert sum(len(c) for _, c in ca) > 0, "the run must hold bonds"
     compare_p4p(pa, pb)
