@@ -130,10 +130,11 @@ def run_engine(cfg, g, w, bonded, precision, steps, warmup, graph=True, near_lis
     if bonded:
         s.initialize()
     s.step(warmup)
-    # capture the step graph of both buffer parities (a call of 3 steps replays one two-step
-    # graph and flips the parity), so that no capture falls into a timed call
-    s.step(3)
-    s.step(3)
+    # capture the step graphs of both buffer parities (a call of 19 steps replays one
+    # sixteen-step and one two-step graph and flips the parity; one of 3 steps replays a
+    # two-step graph), so that no capture falls into a timed call
+    for n in (19, 19, 3, 3):
+        s.step(n)
     # per-kernel durations: K steps with a HIP event pair around every launch on the scene's
     # stream (events keep these steps out of the captured graph)
     s.set_timing(True)

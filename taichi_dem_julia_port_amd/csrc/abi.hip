@@ -194,14 +194,15 @@ hipError_t to_device(const dem_scene* s, void* dst, const void* src, size_t byte
 double grain_radius(double V) { return cbrt(V / (4.0 / 3.0 * M_PI)); }
 
 void drop_graphs(dem_scene* s) {
-    for (int b = 0; b < 2; ++b) {
-        if (s->graph_exec[b]) { hipGraphExecDestroy(s->graph_exec[b]); s->graph_exec[b] = nullptr; }
-        if (s->graph[b]) { hipGraphDestroy(s->graph[b]); s->graph[b] = nullptr; }
-        s->graph_force_tpb[b] = 0;
-        if (s->graph_lean_exec[b]) { hipGraphExecDestroy(s->graph_lean_exec[b]); s->graph_lean_exec[b] = nullptr; }
-        if (s->graph_lean[b]) { hipGraphDestroy(s->graph_lean[b]); s->graph_lean[b] = nullptr; }
-        s->graph_lean_tpb[b] = 0;
-    }
+    for (int l = 0; l < 2; ++l)
+        for (int g = 0; g < 2; ++g)
+            for (int b = 0; b < 2; ++b) {
+                hipGraphExec_t& ge = s->graph_exec[l][g][b];
+                hipGraph_t& gr = s->graph[l][g][b];
+                if (ge) { hipGraphExecDestroy(ge); ge = nullptr; }
+                if (gr) { hipGraphDestroy(gr); gr = nullptr; }
+                s->graph_tpb[l][g][b] = 0;
+            }
 }
 
 void free_scene(dem_scene* s) {
@@ -1149,17 +1150,22 @@ int32_t dem_initialize(dem_scene_t* s) {
     return DEM_OK;
 }
 
-// Capture two steps starting from the current buffer parity as that parity's graph: plain steps
-// (no output writes, no diagnostics), or lean steps.
-static int32_t capture_graph(dem_scene* s, bool lean) {
+// Steps per long graph.  Between two graph launches the GPU idles about 8 us (1M pour, lean
+// steps of 73 us), so long call bodies replay graphs of this many steps, the rest two-step ones.
+constexpr int64_t GRAPH_LONG_STEPS = 16;
+
+// Capture the steps of one graph starting from the current buffer parity as that parity's graph:
+// plain steps (no output writes, no diagnostics) or lean steps, GRAPH_LONG_STEPS or two of them.
+static int32_t capture_graph(dem_scene* s, bool lean, bool lng) {
     const int start = s->cur;
-    hipGraph_t* gr = lean ? s->graph_lean : s->graph;
-    hipGraphExec_t* ge = lean ? s->graph_lean_exec : s->graph_exec;
-    if (ge[start]) { hipGraphExecDestroy(ge[start]); ge[start] = nullptr; }
-    if (gr[start]) { hipGraphDestroy(gr[start]); gr[start] = nullptr; }
+    hipGraph_t& gr = s->graph[lean][lng][start];
+    hipGraphExec_t& ge = s->graph_exec[lean][lng][start];
+    if (ge) { hipGraphExecDestroy(ge); ge = nullptr; }
+    if (gr) { hipGraphDestroy(gr); gr = nullptr; }
     HIPCHK(s, hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipSuccess;
-    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+    const int64_t nst = lng ? GRAPH_LONG_STEPS : 2;
+    for (int64_t k = 0; k < nst && e == hipSuccess; ++k) {
         if (lean) {
             e = run_lean_step(s);
         } else {
@@ -1172,9 +1178,26 @@ static int32_t capture_graph(dem_scene* s, bool lean) {
     s->cur = start;  // capture only recorded the launches
     if (e != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "graph capture", e); }
     if (e2 != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "hipStreamEndCapture", e2); }
-    gr[start] = g;
-    HIPCHK(s, hipGraphInstantiate(&ge[start], g, nullptr, nullptr, 0));
-    (lean ? s->graph_lean_tpb : s->graph_force_tpb)[start] = s->force_tpb;
+    gr = g;
+    HIPCHK(s, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+    s->graph_tpb[lean][lng][start] = s->force_tpb;
+    return DEM_OK;
+}
+
+// Replay `count` graphs of plain or lean steps, long or two-step ones, from the current buffer
+// parity (where they also end), capturing that parity's graph first if it is missing or was
+// captured with another force block size.
+static int32_t replay_graphs(dem_scene* s, bool lean, bool lng, int64_t count) {
+    if (count <= 0) return DEM_OK;
+    const int b = s->cur;
+    if (!s->graph_exec[lean][lng][b] || s->graph_tpb[lean][lng][b] != s->force_tpb) {
+        int32_t rc = capture_graph(s, lean, lng);
+        if (rc) return rc;
+    }
+    for (int64_t k = 0; k < count; ++k) {
+        HIPCHK(s, hipGraphLaunch(s->graph_exec[lean][lng][b], s->stream));
+        s->steps += lng ? GRAPH_LONG_STEPS : 2;
+    }
     return DEM_OK;
 }
 
@@ -1198,9 +1221,9 @@ static hipError_t full_step(dem_scene* s, bool last) {
 
 // Enqueue n steps; call_end: the last of them ends the dem_step call.  With lean steps: a full
 // first step when it has to rebuild (the counters read last hold its flag word), lean steps
-// (pairs replayed from the lean graph of the current buffer parity), and a full last step when
-// it ends the call.  Otherwise full steps: pairs replayed from the graph, the call's last step
-// eager.  Graphs are not used while timing individual kernels.
+// (replayed from the long, then the two-step lean graphs of the current buffer parity, an odd
+// one eager), and a full last step when it ends the call.  Otherwise full steps replayed the
+// same way, the call's last step eager.  Graphs are not used while timing individual kernels.
 static int32_t enqueue_steps(dem_scene* s, int64_t nsteps, bool call_end, bool lean) {
     int64_t t = 0;
     bool rebuild = false;
@@ -1218,31 +1241,28 @@ static int32_t enqueue_steps(dem_scene* s, int64_t nsteps, bool call_end, bool l
             HIPCHK(s, full_step(s, false));
             ++t;
         }
-        if (graphs && body_end - t >= 2) {
-            const int b = s->cur;  // a graph of two steps ends at the parity it starts from
-            if (!s->graph_lean_exec[b] || s->graph_lean_tpb[b] != s->force_tpb) {
-                int32_t rc = capture_graph(s, true);
-                if (rc) return rc;
-            }
-            for (; t + 2 <= body_end; t += 2) {
-                HIPCHK(s, hipGraphLaunch(s->graph_lean_exec[b], s->stream));
-                s->steps += 2;
-            }
+        if (graphs) {
+            const int64_t nlong = (body_end - t) / GRAPH_LONG_STEPS;
+            int32_t rc = replay_graphs(s, true, true, nlong);
+            if (rc) return rc;
+            t += nlong * GRAPH_LONG_STEPS;
+            const int64_t npair = (body_end - t) / 2;
+            if ((rc = replay_graphs(s, true, false, npair))) return rc;
+            t += 2 * npair;
         }
         for (; t < body_end; ++t) {
             HIPCHK(s, run_lean_step(s));
             s->steps++;
         }
-    } else if (graphs && nsteps >= 3) {
-        const int b = s->cur;
-        if (!s->graph_exec[b] || s->graph_force_tpb[b] != s->force_tpb) {
-            int32_t rc = capture_graph(s, false);
-            if (rc) return rc;
-        }
-        for (; t + 2 < nsteps; t += 2) {
-            HIPCHK(s, hipGraphLaunch(s->graph_exec[b], s->stream));
-            s->steps += 2;
-        }
+    } else if (graphs) {
+        // the call's last step stays eager (it writes the outputs)
+        const int64_t nlong = (nsteps - 1) / GRAPH_LONG_STEPS;
+        int32_t rc = replay_graphs(s, false, true, nlong);
+        if (rc) return rc;
+        t += nlong * GRAPH_LONG_STEPS;
+        const int64_t npair = (nsteps - 1 - t) / 2;
+        if ((rc = replay_graphs(s, false, false, npair))) return rc;
+        t += 2 * npair;
     }
     for (; t < nsteps; ++t) HIPCHK(s, full_step(s, call_end && t == nsteps - 1));
     return DEM_OK;

@@ -290,14 +290,14 @@ struct dem_scene {
     unsigned long long* h_ctr = nullptr;      // pinned, summed over the copies
     unsigned long long* h_ctr_raw = nullptr;  // pinned, CTR_WORDS
     std::string err;
-    // two steps captured as one HIP graph (the particle buffers alternate, so two steps restore
-    // the buffer parity), one graph for each buffer parity cur it starts from; replayed for the
-    // body of every dem_step call.  The lean graphs hold two lean steps.
-    hipGraph_t graph[2] = {nullptr, nullptr};
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
-    hipGraph_t graph_lean[2] = {nullptr, nullptr};
-    hipGraphExec_t graph_lean_exec[2] = {nullptr, nullptr};
-    int graph_lean_tpb[2] = {0, 0};
+    // steps captured as HIP graphs and replayed for the body of every dem_step call: the
+    // particle buffers alternate, so an even number of steps restores the buffer parity.  One
+    // graph for each buffer parity cur it starts from, of plain or lean steps, GRAPH_LONG_STEPS
+    // long (long bodies) or two steps long (the rest), indexed [lean][long][parity]; each holds
+    // the force kernel's block size it was captured with.
+    hipGraph_t graph[2][2][2] = {};
+    hipGraphExec_t graph_exec[2][2][2] = {};
+    int graph_tpb[2][2][2] = {};
     // lean steps run in chunks of at most lean_chunk steps between host checks: twice as long
     // after a chunk without a rebuild, the last interval between rebuilds after one
     int64_t lean_chunk = 64;
@@ -313,7 +313,6 @@ struct dem_scene {
     // threads per block of the force kernel: 64 when the last counted step had at least one row
     // entry per four particles, else 256; each graph holds the value it was captured with
     int force_tpb = 256;
-    int graph_force_tpb[2] = {0, 0};
     // diagnostic passes on the last step of a call (candidate pairs of the reference's cp_list,
     // candidates examined by the broad phase): off unless dem_set_diagnostics turns them on
     bool diagnostics = false;

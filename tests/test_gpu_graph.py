@@ -1,4 +1,5 @@
-"""The captured two-step HIP graph replays to exactly the state of eager launches.
+"""The captured HIP graphs (sixteen-step and two-step ones) replay to exactly the state of eager
+launches, with lean steps between near-list rebuilds and with the sort and broad phase on every step.
 
 The FP64 and mixed engines are deterministic (contact rows are sorted by partner and every
 particle sums its own row in that order), so graph replays must match eager stepping bit for bit
@@ -18,22 +19,25 @@ def _scene(*a, **k):
     return Scene(*a, **k)
 
 
+@pytest.mark.parametrize("near_lists", [True, False])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("case", ["lattice", "packed"])
-def test_graph_replay_matches_eager(case, precision):
-    print(f"[case] graph {case} {precision}", flush=True)
+def test_graph_replay_matches_eager(case, precision, near_lists):
+    print(f"[case] graph {case} {precision} near_lists={near_lists}", flush=True)
     if case == "lattice":
         cfg, g, w = scenes.bonded_cube(6, 6, 5, speed=10.0, dt=1e-6, base_gap=2e-5)
         bonded = True
     else:
         cfg, g, w = packed_sample(3000)
         bonded = False
-    with _scene(cfg, g, w, precision=precision) as a, \
-            _scene(cfg, g, w, precision=precision, graph=False) as b:
+    with _scene(cfg, g, w, precision=precision, near_lists=near_lists) as a, \
+            _scene(cfg, g, w, precision=precision, graph=False, near_lists=near_lists) as b:
         if bonded:
             a.initialize()
             b.initialize()
-        for n in (5, 7, 4, 9):  # odd and even lengths: replays plus eager tail steps
+        # odd and even lengths: two-step replays plus eager tail steps, then calls long enough
+        # for sixteen-step replays from both buffer parities
+        for n in (5, 7, 4, 9, 37, 18, 35):
             a.step(n)
             b.step(n)
             ga, gb = a.read_grains(), b.read_grains()
@@ -45,4 +49,4 @@ def test_graph_replay_matches_eager(case, precision):
             assert np.array_equal(ba, bb)
             for f in ("F", "s", "ti", "tj"):
                 assert np.array_equal(ca[f].view(np.uint64), cb[f].view(np.uint64)), f
-        assert a.counters()["step"] == 25
+        assert a.counters()["step"] == 115
