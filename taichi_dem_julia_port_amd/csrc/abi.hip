@@ -1222,8 +1222,10 @@ static hipError_t full_step(dem_scene* s, bool last) {
 // Enqueue n steps; call_end: the last of them ends the dem_step call.  With lean steps: a full
 // first step when it has to rebuild (the counters read last hold its flag word), lean steps
 // (replayed from the long, then the two-step lean graphs of the current buffer parity, an odd
-// one eager), and a full last step when it ends the call.  Otherwise full steps replayed the
-// same way, the call's last step eager.  Graphs are not used while timing individual kernels.
+// one eager), and when it ends the call an eager last step that counts and writes the outputs:
+// lean, or full when it is also the rebuilding first step or runs the diagnostic passes.
+// Otherwise full steps replayed the same way, the call's last step eager.  Graphs are not used
+// while timing individual kernels.
 static int32_t enqueue_steps(dem_scene* s, int64_t nsteps, bool call_end, bool lean) {
     int64_t t = 0;
     bool rebuild = false;
@@ -1264,7 +1266,16 @@ static int32_t enqueue_steps(dem_scene* s, int64_t nsteps, bool call_end, bool l
         if ((rc = replay_graphs(s, false, false, npair))) return rc;
         t += 2 * npair;
     }
-    for (; t < nsteps; ++t) HIPCHK(s, full_step(s, call_end && t == nsteps - 1));
+    for (; t < nsteps; ++t) {
+        const bool last = call_end && t == nsteps - 1;
+        if (lean && last && !s->diagnostics && !(rebuild && t == 0)) {
+            // a lean last step: if it has to rebuild it fails, and the host replays it as a full one
+            HIPCHK(s, run_lean_step(s, true));
+            s->steps++;
+        } else {
+            HIPCHK(s, full_step(s, last));
+        }
+    }
     return DEM_OK;
 }
 

@@ -339,8 +339,9 @@ enum KernelId { K_HASH = 0, K_SCAN, K_SCATTER, K_BROAD, K_MERGE, K_FORCE, K_INIT
 // passes when the scene has them on)
 hipError_t run_contact_detection(dem_scene* s, bool init, bool last);
 hipError_t run_force_integrate(dem_scene* s, bool write_forces);
-// one lean step: the merge over the near lists (or the abort) and the force kernel
-hipError_t run_lean_step(dem_scene* s);
+// one lean step: the merge over the near lists (or the abort) and the force kernel; last: the
+// last step of a call, which counts the per-step counters and writes the outputs
+hipError_t run_lean_step(dem_scene* s, bool last = false);
 // after a bond pass or a failed step: commit the pending rows (rollback = false), or drop them,
 // clear the touching counts and the failure flags (rollback = true)
 hipError_t run_rows_finish(dem_scene* s, bool rollback);

@@ -2262,24 +2262,37 @@ hipError_t run_force_integrate(dem_scene* s, bool write_forces) {
     return s->batched ? force_impl<true>(s, write_forces) : force_impl<false>(s, write_forces);
 }
 
+// The per-step counters (those below CT_NSTEP), cleared before a lean step that ends a dem_step
+// call as the first kernel of a full step clears them: the counters read after the call are
+// that step's.
+__global__ void __launch_bounds__(TPB) k_clear_step_counters(unsigned long long* ctr) {
+    for (int i = threadIdx.x; i < CTR_COPIES * CT_NSTEP; i += TPB)
+        ctr[(i / CT_NSTEP) * CTR_STRIDE + i % CT_NSTEP] = 0ull;
+}
+
 template <bool SEG>
-static hipError_t lean_impl(dem_scene* s) {
+static hipError_t lean_impl(dem_scene* s, bool last) {
     DevState& d = s->d;
     const Params& P = s->P;
     const int c = s->cur;
     if (!P.skin || !s->dense) return hipErrorInvalidValue;
+    const int diag = last ? 1 : 0;
+    if (last)
+        LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_clear_step_counters, dim3(1), dim3(TPB), 0, s->stream, d.ctr));
     if (s->force_tpb == 64) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), 0,
-                                              s->stream, P, merge_args(s), d.ctr, 0, c, 1));
+                                              s->stream, P, merge_args(s), d.ctr, diag, c, 1));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), grid_for(P.n, MERGE_TPB),
-                                              dim3(MERGE_TPB), 0, s->stream, P, merge_args(s), d.ctr, 0, c, 1));
+                                              dim3(MERGE_TPB), 0, s->stream, P, merge_args(s), d.ctr, diag,
+                                              c, 1));
     }
-    return force_impl<SEG>(s, false);
+    return force_impl<SEG>(s, last);
 }
 
-hipError_t run_lean_step(dem_scene* s) {
-    return s->batched ? lean_impl<true>(s) : lean_impl<false>(s);
+hipError_t run_lean_step(dem_scene* s, bool last) {
+    if (last) s->diag_valid = false;
+    return s->batched ? lean_impl<true>(s, last) : lean_impl<false>(s, last);
 }
 
 hipError_t run_rows_finish(dem_scene* s, bool rollback) {
