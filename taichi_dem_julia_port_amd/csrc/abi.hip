@@ -576,7 +576,7 @@ int32_t allocate_scene(dem_scene* s, const dem_grain_in_t* grows, int64_t gbase,
         c.take(&d.bstart, nkeys + 1); c.take(&d.sidxb[0], n); c.take(&d.sidxb[1], n); c.take(&d.skey, n);
         c.take(&d.spos, n); c.take(&d.spos32, n); c.take(&d.eid, n); c.take(&d.sfid, n);
         c.take(&d.tcnt, n); c.take(&d.sel, n);
-        c.take(&d.nbc, n); c.take(&d.xref, n); c.take(&d.gmin, n);
+        c.take(&d.nbc, n); c.take(&d.xref, n); c.take(&d.gmin, n); c.take(&d.gskip, (n + 63) / 64);
         c.take(&d.ctr, CTR_WORDS);
         c.take(&d.scan.status, max_tiles); c.take(&d.scan.counters, 2);
         c.take(&d.scan.tagg, max_tiles * TAGG_COPIES);
@@ -1564,6 +1564,8 @@ int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_
     // on the scene's stream, ahead of k_write_rows and the next merge
     if (e == hipSuccess) e = hipMemsetAsync(d.rcnt[1], 0, n * sizeof(uint32_t), s->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d.sel, 0, n, s->stream);
+    // the merges' per-group values assumed the old rows: none may be left out until renewed
+    if (e == hipSuccess && d.gskip) e = hipMemsetAsync(d.gskip, 0, ((n + 63) / 64) * sizeof(uint32_t), s->stream);
     if (e == hipSuccess) e = run_write_contacts(s, ddst, dword, dstate, (int64_t)ents.size());
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     hipFree(ddst); hipFree(dword); hipFree(dstate);

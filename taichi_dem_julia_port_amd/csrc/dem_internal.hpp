@@ -224,6 +224,10 @@ struct DevState {
     uint32_t* nb = nullptr;      // [KN][n]
     float4* xref = nullptr;      // [n]
     uint32_t* gmin = nullptr;    // [n] FP32 bits: the smallest near-list gap at the rebuild (>= 0)
+    // [ceil(n / 64)] FP32 bits per group of 64 particles (a wave's): while none of them has a row
+    // entry or a touching partner, the smallest of their gmin, else 0; written by every merge
+    // wave that runs, so a lean merge leaves out a group whose value exceeds 2 D
+    uint32_t* gskip = nullptr;
     // contact rows, double-buffered per particle (sel[p])
     uint8_t* sel = nullptr;
     uint32_t* rcnt[2] = {nullptr, nullptr};

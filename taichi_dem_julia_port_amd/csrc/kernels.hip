@@ -881,6 +881,7 @@ struct MergeArgs {
     uint32_t* __restrict__ rcnt0; uint32_t* __restrict__ rcnt1;
     uint32_t* __restrict__ pq0; uint32_t* __restrict__ pq1;
     double* __restrict__ st0; double* __restrict__ st1;
+    uint32_t* __restrict__ gskip;
 };
 
 // What a wave of the merge counts.
@@ -916,7 +917,7 @@ __device__ __forceinline__ MergeIn load_merge_in(const MergeArgs& M, uint32_t p,
 }
 
 template <bool INIT, bool SEG, int BS>
-__device__ __forceinline__ void merge_particle(const Params& P, const MergeArgs& M, uint32_t p,
+__device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs& M, uint32_t p,
                                                const MergeIn& in, uint32_t* lt, uint32_t* lw,
                                                bool reuse, float twoD, MergeTally& t) {
     const int64_t n = P.n;
@@ -1045,6 +1046,10 @@ __device__ __forceinline__ void merge_particle(const Params& P, const MergeArgs&
             t.need = max(t.need, (uint32_t)entries);
         }
     }
+    // no row and no touching partner: nothing changed (only the variant that keeps group values
+    // asks; computing it costs the others registers)
+    if constexpr (!INIT && !SEG && BS == MERGE_TPB) return (nt | nold) == 0u;
+    else return false;
 }
 
 // The merge's view of the step, decided for the whole block (so a block leaves or stays as a
@@ -1107,7 +1112,11 @@ __device__ __forceinline__ void merge_counters(unsigned long long* ctr, bool dia
 // The merge kernel.  Rows of up to MERGE_LDS touching partners and old entries are staged in LDS
 // with batched independent loads; longer rows are merged in place in global memory.  On a step
 // that reuses the near lists the touching partners are the near-list entries that pass the
-// broad phase's test, found here (the broad phase did not run).
+// broad phase's test, found here (the broad phase did not run).  On a lean step of a sparse
+// single scene (blocks of MERGE_TPB threads) a wave whose group of 64 particles has no row entry
+// and no near-list partner within 2 D (gskip) leaves before loading anything of its particles:
+// their merges would change nothing.  The other single-scene variants set their groups' values
+// to 0 (never left out), which keeps them valid when the block size changes.
 template <bool INIT, bool SEG, int BS>
 __global__ void __launch_bounds__(BS)
 k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, int skin_cur, int lean) {
@@ -1117,13 +1126,43 @@ k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, in
     const bool in = p64 < P.n;
     const uint32_t p = in ? (uint32_t)p64 : 0u;
     const bool near = !INIT && skin_cur >= 0;
-    // the particle's loads first: the block's look at the step's words waits for them together
-    const MergeIn mi = load_merge_in(M, p, near, lean != 0);
-    const MergeStep st = merge_step_begins(ctr, skin_cur, near, lean != 0, BS);
-    if (!st.go) return;
-    MergeTally t;
-    if (in) merge_particle<INIT, SEG, BS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse, st.twoD, t);
-    merge_counters(ctr, diag != 0, st.reuse, t);
+    constexpr bool GROUPS = !INIT && !SEG && BS == MERGE_TPB;
+    if constexpr (!GROUPS) {
+        // the particle's loads first: the block's look at the step's words waits for them together
+        const MergeIn mi = load_merge_in(M, p, near, lean != 0);
+        const MergeStep st = merge_step_begins(ctr, skin_cur, near, lean != 0, BS);
+        if (!st.go) return;
+        MergeTally t;
+        if (in) merge_particle<INIT, SEG, BS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse, st.twoD, t);
+        merge_counters(ctr, diag != 0, st.reuse, t);
+        if constexpr (!SEG)
+            if ((threadIdx.x & 63) == 0 && p64 < P.n) M.gskip[p64 >> 6] = 0u;
+    } else {
+        const int64_t g = p64 >> 6;       // the wave's group
+        const bool gin = (g << 6) < P.n;  // it holds particles
+        MergeIn mi;
+        MergeStep st;
+        if (lean) {
+            // the group's value first, the particle's loads only if the wave stays
+            const float gs = gin ? __uint_as_float(M.gskip[g]) : __builtin_inff();
+            st = merge_step_begins(ctr, skin_cur, near, true, BS);
+            if (!st.go || gs > st.twoD) return;
+            mi = load_merge_in(M, p, near, true);
+        } else {
+            mi = load_merge_in(M, p, near, false);
+            st = merge_step_begins(ctr, skin_cur, near, false, BS);
+            if (!st.go) return;
+        }
+        MergeTally t;
+        bool idle = true;
+        if (in) idle = merge_particle<INIT, SEG, BS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse,
+                                                     st.twoD, t);
+        merge_counters(ctr, diag != 0, st.reuse, t);
+        // the group's value for the next lean steps (gmin stays until the next rebuild)
+        float v = !in ? __builtin_inff() : (idle && near && mi.gm > 0.0f) ? mi.gm : 0.0f;
+        for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0 && gin) M.gskip[g] = __float_as_uint(v);
+    }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2121,7 +2160,7 @@ static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) 
 static MergeArgs merge_args(dem_scene* s) {
     DevState& d = s->d;
     MergeArgs M;
-    M.pos = d.pos[s->cur]; M.nbc = d.nbc; M.nb = d.nb; M.gmin = d.gmin; M.tcnt = d.tcnt;
+    M.pos = d.pos[s->cur]; M.nbc = d.nbc; M.nb = d.nb; M.gmin = d.gmin; M.gskip = d.gskip; M.tcnt = d.tcnt;
     M.touch = d.touch; M.sel = d.sel; M.rcnt0 = d.rcnt[0]; M.rcnt1 = d.rcnt[1];
     M.pq0 = d.pq[0]; M.pq1 = d.pq[1]; M.st0 = d.st[0]; M.st1 = d.st[1];
     return M;

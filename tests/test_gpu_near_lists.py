@@ -5,8 +5,8 @@ r_i + r_j + 2 delta, and later steps test only those lists until some particle h
 the same scene created with near_lists=False sorts and runs the broad phase on every step.  After
 every call both are bit for bit in the same state: grain k, v, a, q, omega, d omega, the contact
 and bonded sets with every contact field, the per-particle force and moment sums and the step
-counters.  The cases cover graph replays and eager steps, FP64 and mixed precision, a bonded
-impact, a batch, a host write between calls, the diagnostic passes, near lists that overflow and
+counters.  The cases cover graph replays and eager steps, FP64 and mixed precision, sparse and
+contact-heavy states, a bonded impact, a batch, a host write between calls, the diagnostic passes, near lists that overflow and
 are widened, and grains fast enough that lean steps (the merge and force kernel alone) keep
 finding they must rebuild and are replayed.  Each case moves far enough that the lists are
 rebuilt during the run.
@@ -72,8 +72,17 @@ def fast(n=3000):
     return packed_sample(n, vmax=30.0)
 
 
+def sparse(n=3000):
+    """The pour sample at its own radii (few contacts) with speeds up to 2 m/s: contacts form and
+    end during lean steps, while the lean merges leave out the groups of 64 grains with no row
+    and no near-list partner within twice the displacement bound."""
+    return packed_sample(n, scale=1.0, vmax=2.0)
+
+
 CASES = {
     "fast_fp64": dict(build=fast, precision="fp64", graph=True, bonds=False, replays=True),
+    "sparse_fp64": dict(build=sparse, precision="fp64", graph=True, bonds=False),
+    "sparse_mixed_eager": dict(build=sparse, precision="mixed", graph=False, bonds=False),
     "packed_fp64_graph": dict(build=shaken, precision="fp64", graph=True, bonds=False),
     "packed_fp64_eager": dict(build=shaken, precision="fp64", graph=False, bonds=False),
     "packed_mixed": dict(build=shaken, precision="mixed", graph=True, bonds=False),
