@@ -25,7 +25,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> float:
+def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str, free: bool = False) -> float:
     """Compulsory HBM bytes of one launch of ``kernel`` in this engine's design (each array the
     kernel must touch, read or written once):
       hash            previous order 4 B + position 32 B read; bucket, rank 8 B + FP32 position
@@ -41,7 +41,8 @@ def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> floa
                       per Hertz contact 2 rows x (partner 4 + spring 24 read + 24 written),
                       per bond 2 rows x (partner 4 + F,ti,tj 72 read + 72 written), per wall
                       contact 32 B (contact points and local forces are written only on a call's
-                      last step, outside the timed replays)"""
+                      last step, outside the timed replays); in the contact-free mode of a single
+                      scene with no row entry (free) the row counts are not read"""
     n = c["n"]
     cells = c.get("grid_cells") or c["hash_table_size"]
     Ch, Cb, W = c["hertz"], c["bonded"], c["wall_contacts"]
@@ -49,7 +50,7 @@ def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str) -> floa
     if kernel == "broad_phase":
         return n * 24.0 + 4.0 * cells + touching * (64 + 4 + 16)
     if kernel == "force_integrate":
-        return n * (128 + 128 + 5 + 4) + 2 * Ch * (4 + 48) + 2 * Cb * (4 + 144) + 32.0 * W
+        return n * (128 + 128 + (0 if free else 5) + 4) + 2 * Ch * (4 + 48) + 2 * Cb * (4 + 144) + 32.0 * W
     if kernel == "hash":
         return n * 60.0
     if kernel == "scatter":
@@ -294,7 +295,9 @@ def summarize(r, workload, precision, nwall, n, steps):
     total_kernel_ms = sum(ms for ms, _ in kt.values())
     dom = max(((k, ms) for k, (ms, l) in kt.items() if l), key=lambda kv: kv[1])[0]
     c = r["counters"]
-    alg = _algorithmic_bytes(dom, c, nwall, precision)
+    # a single scene whose state has no contact runs the contact-free force kernel
+    free = workload != "batch256" and c["contacts"] == 0
+    alg = _algorithmic_bytes(dom, c, nwall, precision, free)
     achieved = alg / (step_ms[dom] * 1e-3) / 1e9 if step_ms[dom] > 0 else 0.0
     traffic, tsrc = measured_traffic(workload, precision, n)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
@@ -312,7 +315,7 @@ def summarize(r, workload, precision, nwall, n, steps):
             _reference_equivalent_bytes(c) / (step_ms[dom] * 1e-3) / 1e9, 1)
     per_kernel = {}
     for k, ms in step_ms.items():
-        b = _algorithmic_bytes(k, c, nwall, precision)
+        b = _algorithmic_bytes(k, c, nwall, precision, free)
         if b > 0 and ms > 0:
             per_kernel[k] = round(b / (ms * 1e-3) / 1e9, 1)
     return {"value": n * steps / r["elapsed"],

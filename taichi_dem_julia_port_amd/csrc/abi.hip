@@ -237,10 +237,14 @@ int32_t sync_counters(dem_scene* s) {
 }
 
 // The force kernel's block size for the next steps, from the row entries counted on the last
-// counted step (sync_counters must have run).
+// counted step (sync_counters must have run), and whether a single scene runs contact-free.
 void choose_force_blocks(dem_scene* s) {
     s->force_tpb = 4 * s->h_ctr[CT_ENTRIES] >= (unsigned long long)s->n ? 64 : 256;
+    s->force_free = !s->batched && s->h_ctr[CT_ENTRIES] == 0;
 }
+
+// What a captured graph's force and merge launches depend on besides the buffer parity.
+int force_key(const dem_scene* s) { return s->force_tpb + (s->force_free ? 1 : 0); }
 
 }  // namespace
 
@@ -866,6 +870,7 @@ int32_t dem_batch_adopt(dem_scene_t* dst, int32_t dst_scene, dem_scene_t* src, i
     hipFree(ovf);
     if (e != hipSuccess) return hip_fail(dst, "dem_batch_adopt", e);
     dst->skin_dirty = true;  // the destination scene's particles were replaced
+    dst->force_free = false;  // and may have rows now
     if (o) return fail(dst, DEM_ERR_CAPACITY, "destination has fewer contact slots than needed");
     // cumulative bond breaks of the scene travel with it
     unsigned long long nb = 0;
@@ -1180,7 +1185,7 @@ static int32_t capture_graph(dem_scene* s, bool lean, bool lng) {
     if (e2 != hipSuccess) { if (g) hipGraphDestroy(g); return hip_fail(s, "hipStreamEndCapture", e2); }
     gr = g;
     HIPCHK(s, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
-    s->graph_tpb[lean][lng][start] = s->force_tpb;
+    s->graph_tpb[lean][lng][start] = force_key(s);
     return DEM_OK;
 }
 
@@ -1190,7 +1195,7 @@ static int32_t capture_graph(dem_scene* s, bool lean, bool lng) {
 static int32_t replay_graphs(dem_scene* s, bool lean, bool lng, int64_t count) {
     if (count <= 0) return DEM_OK;
     const int b = s->cur;
-    if (!s->graph_exec[lean][lng][b] || s->graph_tpb[lean][lng][b] != s->force_tpb) {
+    if (!s->graph_exec[lean][lng][b] || s->graph_tpb[lean][lng][b] != force_key(s)) {
         int32_t rc = capture_graph(s, lean, lng);
         if (rc) return rc;
     }
@@ -1343,17 +1348,18 @@ int32_t dem_step(dem_scene_t* s, int64_t nsteps) {
         s->cur = (done & 1) ? 1 - cur0 : cur0;
         left -= done;
         s->skin_dirty = true;  // the failed steps' broad phases may have rewritten the near lists
-        const bool abort = s->h_ctr[CT_ABORT] && !s->h_ctr[CT_MAXROW];
+        const unsigned long long aborted = s->h_ctr[CT_MAXROW] ? 0ull : s->h_ctr[CT_ABORT];
         HIPCHK(s, run_rows_finish(s, true));
-        if (abort) {
+        if (aborted & 2ull) s->force_free = false;  // contacts appeared: the full force kernel
+        if (aborted & 1ull) {
             ++s->replays;
             // a lean step had to rebuild: the next chunk starts with a full step there, and is
             // as long as the interval since the previous such rebuild
             const int64_t gap = s->steps - s->last_rebuild_step;
             s->last_rebuild_step = s->steps;
             s->lean_chunk = gap < LEAN_CHUNK_MIN ? 0 : std::min(gap, LEAN_CHUNK_MAX);
-            continue;
         }
+        if (aborted) continue;
         // a row overflowed: widen the rows
         if (s->K >= MAX_SLOTS) {
             HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -1564,8 +1570,10 @@ int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_
     // on the scene's stream, ahead of k_write_rows and the next merge
     if (e == hipSuccess) e = hipMemsetAsync(d.rcnt[1], 0, n * sizeof(uint32_t), s->stream);
     if (e == hipSuccess) e = hipMemsetAsync(d.sel, 0, n, s->stream);
-    // the merges' per-group values assumed the old rows: none may be left out until renewed
+    // the merges' per-group values and the contact-free mode assumed the old rows: no group is left
+    // out until its value is renewed, and the force kernel reads rows again
     if (e == hipSuccess && d.gskip) e = hipMemsetAsync(d.gskip, 0, ((n + 63) / 64) * sizeof(uint32_t), s->stream);
+    s->force_free = false;
     if (e == hipSuccess) e = run_write_contacts(s, ddst, dword, dstate, (int64_t)ents.size());
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     hipFree(ddst); hipFree(dword); hipFree(dstate);

@@ -317,6 +317,11 @@ struct dem_scene {
     // threads per block of the force kernel: 64 when the last counted step had at least one row
     // entry per four particles, else 256; each graph holds the value it was captured with
     int force_tpb = 256;
+    // contact-free mode (single scenes): the last counted step left no row entry, so the force
+    // kernel runs a variant that reads and walks no rows; a merge that would make a row entry
+    // fails its step instead (CT_ABORT bit 2) and the host replays it with this mode off.  The
+    // graphs hold the mode they were captured in with the block size (force_key).
+    bool force_free = false;
     // diagnostic passes on the last step of a call (candidate pairs of the reference's cp_list,
     // candidates examined by the broad phase): off unless dem_set_diagnostics turns them on
     bool diagnostics = false;

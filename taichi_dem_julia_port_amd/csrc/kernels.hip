@@ -888,6 +888,7 @@ struct MergeArgs {
 struct MergeTally {
     uint64_t created = 0, deact = 0, entries = 0, found = 0;
     uint32_t overflow = 0, need = 0;
+    uint32_t contact = 0;  // contact-free mode: a particle would get a row entry
 };
 
 // The merge of particle p: its new touching set (from the broad phase, or on a step that reuses
@@ -919,7 +920,7 @@ __device__ __forceinline__ MergeIn load_merge_in(const MergeArgs& M, uint32_t p,
 template <bool INIT, bool SEG, int BS>
 __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs& M, uint32_t p,
                                                const MergeIn& in, uint32_t* lt, uint32_t* lw,
-                                               bool reuse, float twoD, MergeTally& t) {
+                                               bool reuse, float twoD, MergeTally& t, bool free = false) {
     const int64_t n = P.n;
     const int K = P.K;
     // The committed buffer: after a failed step of the call a row may still be marked
@@ -1002,6 +1003,10 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
     if (nt | nold) {
         const SceneP& S = scene_of<SEG>(P, p);
         if (nt && !reuse) M.tcnt[p] = 0u;
+        if (free) {  // contact-free mode: the step fails, to be replayed with the full force kernel
+            t.contact = 1;
+            return false;
+        }
         uint32_t overflow = 0;
         if ((int)nt > K) { overflow = 1; nt = K; }
         const uint32_t* pqo = b ? M.pq1 : M.pq0;
@@ -1098,7 +1103,11 @@ __device__ __forceinline__ MergeStep merge_step_begins(unsigned long long* ctr, 
 }
 
 __device__ __forceinline__ void merge_counters(unsigned long long* ctr, bool diag, bool reuse,
-                                               const MergeTally& t) {
+                                               const MergeTally& t, bool free = false) {
+    if (free && __ballot(t.contact != 0u) != 0ull && (threadIdx.x & 63) == 0) {
+        atomicOr(&ctr[CT_ABORT], 2ull);
+        atomicOr(&ctr[CT_FAIL], 1ull);
+    }
     if (diag) {
         wave_add(ctr_at(ctr, CT_CREATED), t.created);
         wave_add(ctr_at(ctr, CT_DEACT), t.deact);
@@ -1116,10 +1125,14 @@ __device__ __forceinline__ void merge_counters(unsigned long long* ctr, bool dia
 // single scene (blocks of MERGE_TPB threads) a wave whose group of 64 particles has no row entry
 // and no near-list partner within 2 D (gskip) leaves before loading anything of its particles:
 // their merges would change nothing.  The other single-scene variants set their groups' values
-// to 0 (never left out), which keeps them valid when the block size changes.
+// to 0 (never left out), which keeps them valid when the block size changes.  free: the scene is
+// in contact-free mode (the sparse single-scene variant only): no particle has a row, the force
+// kernel evaluates none, and a particle that would get one fails the step instead (CT_ABORT bit
+// 2), which the host replays with the full force kernel.
 template <bool INIT, bool SEG, int BS>
 __global__ void __launch_bounds__(BS)
-k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, int skin_cur, int lean) {
+k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, int skin_cur, int lean,
+        int free) {
     __shared__ uint32_t lt[MERGE_LDS * BS];  // touching partners
     __shared__ uint32_t lw[MERGE_LDS * BS];  // old-row words
     const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * BS + threadIdx.x;
@@ -1156,8 +1169,8 @@ k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, in
         MergeTally t;
         bool idle = true;
         if (in) idle = merge_particle<INIT, SEG, BS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse,
-                                                     st.twoD, t);
-        merge_counters(ctr, diag != 0, st.reuse, t);
+                                                     st.twoD, t, free != 0);
+        merge_counters(ctr, diag != 0, st.reuse, t, free != 0);
         // the group's value for the next lean steps (gmin stays until the next rebuild)
         float v = !in ? __builtin_inff() : (idle && near && mi.gm > 0.0f) ? mi.gm : 0.0f;
         for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
@@ -1681,8 +1694,10 @@ __device__ __forceinline__ Own load_own(const SceneP& S, const ForceArgs& A, uin
 // Contacts (kernels.jl:122-315), walls (kernels.jl:317-426), body force and update! of particle p,
 // whose own state `own` is loaded.  failed: a row of this step or an earlier one overflowed, so
 // nothing changes (tested before the first store, so the flag's load hides behind the state
-// loads); rebuilt: near lists in use (skin_cur >= 0) and this step rebuilt them.
-template <bool MIXED, bool SEG, bool LATE, bool WALLS_LATE>
+// loads); rebuilt: near lists in use (skin_cur >= 0) and this step rebuilt them.  FREE: the
+// contact-free mode, in which no particle has a row (the merge fails a step that would give one a
+// row), so the row is neither read nor walked and the kernel needs few registers.
+template <bool MIXED, bool SEG, bool LATE, bool WALLS_LATE, bool FREE = false>
 __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs& A, uint32_t p,
                                                const Own& own, int write_forces, int skin_cur,
                                                bool failed, bool rebuilt,
@@ -1703,14 +1718,14 @@ __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs&
     CSum3 FH, TH;  // mixed precision: compensated FP32 sums of the Hertz contributions
     // commit a row the merge rebuilt this step (it sits in the other buffer)
     // both row counts with the selector (independent loads), the committed one chosen after
-    const uint8_t sl = A.sel[p];
-    const uint32_t c0 = A.rcnt0[p], c1 = A.rcnt1[p];
+    const uint8_t sl = FREE ? 0 : A.sel[p];
+    const uint32_t c0 = FREE ? 0u : A.rcnt0[p], c1 = FREE ? 0u : A.rcnt1[p];
     const int b = (sl & ROW_PENDING) ? ((sl & 1) ^ 1) : sl;
     if (failed) return;
     if (sl & ROW_PENDING) A.sel[p] = (uint8_t)b;
     uint32_t* pq = b ? A.pq1 : A.pq0;
     double* st = b ? A.st1 : A.st0;
-    const uint32_t cnt = b ? c1 : c0;
+    const uint32_t cnt = FREE ? 0u : (b ? c1 : c0);
     const int64_t cs = (int64_t)P.K * n;
     const double inv_r = cnt ? 1.0 / X.w : 0.0, inv_m = cnt ? 1.0 / Vv.w : 0.0;
     // software pipeline over the row: entry m+1's partner state and entry m+2's word are
@@ -1838,9 +1853,12 @@ __device__ __forceinline__ void force_counters(unsigned long long* ctr, int skin
 // BS threads a block.  With contacts, rows differ in length and a one-wave block frees its slot as
 // soon as its wave is done (packed 1M pour: 269 us at 256 threads, 247 at 64); without, 256-thread
 // blocks stream the state 1 % faster.  The host picks by the last counted row entries per particle.
-template <bool MIXED, bool SEG, int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(MIXED && (!SEG || BS == 256) ? 2 : 1)))
+// FREE (single scenes, 256 threads): the contact-free mode, no row read or walked.
+template <bool MIXED, bool SEG, int BS, bool FREE = false>
+__global__ void __launch_bounds__(BS)
+__attribute__((amdgpu_waves_per_eu(FREE ? 3 : (MIXED && (!SEG || BS == 256) ? 2 : 1))))
 k_force(const Params P, const ForceArgs A, unsigned long long* ctr, int write_forces, int skin_cur) {
+    static_assert(!FREE || (!SEG && BS == 256), "contact-free mode: sparse single scenes");
     const bool failed = step_failed(ctr);
     // near lists in use (skin_cur >= 0): did this step rebuild them?
     const bool rebuilt = skin_cur >= 0 && skin_rebuilds(ctr, skin_cur);
@@ -1852,7 +1870,8 @@ k_force(const Params P, const ForceArgs A, unsigned long long* ctr, int write_fo
         // mixed precision on contact-dense states (one-wave blocks) loads part of the state late
         constexpr bool LATE = MIXED && BS == 64;
         const Own own = load_own<LATE>(scene_of<SEG>(P, p), A, p, skin_cur, rebuilt);
-        force_particle<MIXED, SEG, LATE, BS == 64>(P, A, p, own, write_forces, skin_cur, failed, rebuilt, ctr, t);
+        force_particle<MIXED, SEG, LATE, BS == 64, FREE>(P, A, p, own, write_forces, skin_cur, failed, rebuilt, ctr,
+                                                         t);
     }
     if (failed) return;
     force_counters(ctr, skin_cur, rebuilt, write_forces, true, t);
@@ -2246,14 +2265,15 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
     const dim3 gm = grid_for(P.n, MERGE_TPB);
     if (init) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
-                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0));
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0, 0));
     } else if (s->force_tpb == 64) {
         // contact-dense, as the force kernel's block choice: one-wave blocks for uneven rows
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), lds,
-                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0));
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0, 0));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
-                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0));
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0,
+                                              s->force_free ? 1 : 0));
     }
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
@@ -2272,17 +2292,17 @@ hipError_t run_contact_detection(dem_scene* s, bool init, bool last) {
     return s->batched ? detect_impl<true>(s, init, last) : detect_impl<false>(s, init, last);
 }
 
-template <bool SEG, int BS>
+template <bool SEG, int BS, bool FREE = false>
 static hipError_t launch_force(dem_scene* s, bool write_forces) {
     const Params& P = s->P;
     const int skin = (P.skin && s->dense) ? s->cur : -1;
     const dim3 g = grid_for(P.n, BS);
     const ForceArgs A = force_args(s);
     if (s->precision == DEM_PRECISION_MIXED) {
-        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<true, SEG, BS>), g, dim3(BS), 0, s->stream, P,
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<true, SEG, BS, FREE>), g, dim3(BS), 0, s->stream, P,
                                               A, s->d.ctr, write_forces ? 1 : 0, skin));
     } else {
-        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<false, SEG, BS>), g, dim3(BS), 0, s->stream, P,
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_force<false, SEG, BS, FREE>), g, dim3(BS), 0, s->stream, P,
                                               A, s->d.ctr, write_forces ? 1 : 0, skin));
     }
     return hipSuccess;
@@ -2290,8 +2310,14 @@ static hipError_t launch_force(dem_scene* s, bool write_forces) {
 
 template <bool SEG>
 static hipError_t force_impl(dem_scene* s, bool write_forces) {
-    const hipError_t e = s->force_tpb == 64 ? launch_force<SEG, 64>(s, write_forces)
-                                            : launch_force<SEG, 256>(s, write_forces);
+    hipError_t e;
+    if constexpr (!SEG) {
+        if (s->force_free) e = launch_force<false, 256, true>(s, write_forces);
+        else e = s->force_tpb == 64 ? launch_force<false, 64>(s, write_forces)
+                                    : launch_force<false, 256>(s, write_forces);
+    } else {
+        e = s->force_tpb == 64 ? launch_force<true, 64>(s, write_forces) : launch_force<true, 256>(s, write_forces);
+    }
     if (e != hipSuccess) return e;
     s->cur = 1 - s->cur;
     return hipSuccess;
@@ -2320,11 +2346,11 @@ static hipError_t lean_impl(dem_scene* s, bool last) {
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_clear_step_counters, dim3(1), dim3(TPB), 0, s->stream, d.ctr));
     if (s->force_tpb == 64) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), 0,
-                                              s->stream, P, merge_args(s), d.ctr, diag, c, 1));
+                                              s->stream, P, merge_args(s), d.ctr, diag, c, 1, 0));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), grid_for(P.n, MERGE_TPB),
                                               dim3(MERGE_TPB), 0, s->stream, P, merge_args(s), d.ctr, diag,
-                                              c, 1));
+                                              c, 1, s->force_free ? 1 : 0));
     }
     return force_impl<SEG>(s, last);
 }
