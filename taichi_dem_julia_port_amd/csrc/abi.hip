@@ -1574,6 +1574,7 @@ int32_t dem_write_contacts(dem_scene_t* s, const dem_contact_t* in, const uint8_
     // out until its value is renewed, and the force kernel reads rows again
     if (e == hipSuccess && d.gskip) e = hipMemsetAsync(d.gskip, 0, ((n + 63) / 64) * sizeof(uint32_t), s->stream);
     s->force_free = false;
+    s->skin_dirty = true;  // the near lists' bond marks are those of the old rows: rebuild them
     if (e == hipSuccess) e = run_write_contacts(s, ddst, dword, dstate, (int64_t)ents.size());
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     hipFree(ddst); hipFree(dword); hipFree(dstate);

@@ -18,6 +18,15 @@ constexpr uint32_t IS_I = 1u << 29;         // the row's particle is the contact
                                             // file index): the canonical orientation
 constexpr uint32_t BONDED = 1u << 30;       // entry is an EBPM bond
 constexpr uint32_t BROKEN = 1u << 31;       // bond broke this step; dropped by the next merge
+// A near-list entry is partner | IS_I, with NB_BOND when the partner was the particle's intact
+// bond at the rebuild: the rebuilding merge sets it, the force kernel clears it when the bond
+// breaks, and lean merges leave such entries untested (a bond persists whether or not its grains
+// touch, so testing it changes no row).
+constexpr uint32_t NB_BOND = 1u << 30;
+// Set in a particle's near-list count (nbc) by the force kernel when one of its bonds breaks: the
+// next lean merge clears the particle's NB_BOND marks (its lists are then all tested until the
+// next rebuild marks them again), since a broken bond's pair may touch as a Hertz contact.
+constexpr uint32_t NB_BROKE = 1u << 31;
 
 // Row-entry state components (each an array of contact_slots * n doubles per buffer).
 enum StateComp { S_KX = 0, S_KY, S_KZ, S_FX, S_FY, S_FZ, S_TIX, S_TIY, S_TIZ,

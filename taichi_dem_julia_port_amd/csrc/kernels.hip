@@ -875,7 +875,7 @@ constexpr int MERGE_LDS = 16;
 // The arrays the merge reads and writes.
 struct MergeArgs {
     const double4* __restrict__ pos;  // positions of the step (the current buffer)
-    const uint32_t* __restrict__ nbc; const uint32_t* __restrict__ nb; const uint32_t* __restrict__ gmin;
+    uint32_t* __restrict__ nbc; uint32_t* __restrict__ nb; const uint32_t* __restrict__ gmin;
     uint32_t* __restrict__ tcnt; uint32_t* __restrict__ touch;
     uint8_t* __restrict__ sel;
     uint32_t* __restrict__ rcnt0; uint32_t* __restrict__ rcnt1;
@@ -901,10 +901,10 @@ struct MergeTally {
 // `near`: near lists in use, so the step may reuse them (then the near-list count and smallest gap
 // are the inputs) or rebuild them (then the touching count is).
 struct MergeIn {
-    uint32_t tc, nc, c0, c1;
+    uint32_t tc, nc, c0, c1;  // nc: the near-list count, with NB_BROKE
     float gm;
     uint8_t sl;
-    __device__ __forceinline__ uint32_t cnt_in(bool reuse) const { return reuse ? nc : tc; }
+    __device__ __forceinline__ uint32_t cnt_in(bool reuse) const { return reuse ? (nc & ~NB_BROKE) : tc; }
 };
 __device__ __forceinline__ MergeIn load_merge_in(const MergeArgs& M, uint32_t p, bool near, bool lean) {
     MergeIn m;
@@ -917,7 +917,7 @@ __device__ __forceinline__ MergeIn load_merge_in(const MergeArgs& M, uint32_t p,
     return m;
 }
 
-template <bool INIT, bool SEG, int BS>
+template <bool INIT, bool SEG, int BS, bool BONDS = false>
 __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs& M, uint32_t p,
                                                const MergeIn& in, uint32_t* lt, uint32_t* lw,
                                                bool reuse, float twoD, MergeTally& t, bool free = false) {
@@ -931,6 +931,33 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
     const uint32_t nold = b ? in.c1 : in.c0;
     uint32_t nt = cnt_in;
     bool t_lds = false;  // the touching partners are in lt already
+    // On a lean step the old row (when it fits the LDS stage) is staged before the near lists are
+    // walked.  Entries marked as intact bonds are left untested, unless a bond of this particle
+    // has broken since the rebuild (a BROKEN entry now, or NB_BROKE set by an earlier merge that
+    // saw one: then its lists are tested in full until the next rebuild) or its row is long.
+    const uint32_t* pqo = b ? M.pq1 : M.pq0;
+    bool lw_staged = false, broke = (in.nc & NB_BROKE) != 0u;
+    if (BONDS && reuse && nold && nold <= (uint32_t)MERGE_LDS) {
+        uint32_t any = 0u;
+        for (uint32_t m = 0; m < nold; m += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = m + u < nold ? pqo[(int64_t)(m + u) * n + p] : 0u;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) if (m + u < nold) { lw[(m + u) * BS] = v[u]; any |= v[u]; }
+        }
+        lw_staged = true;
+        if ((any & BROKEN) && !broke) {
+            broke = true;
+            M.nbc[p] = in.nc | NB_BROKE;
+        }
+    } else if (BONDS && reuse && nold > (uint32_t)MERGE_LDS && !broke) {
+        // a long row is not searched for BROKEN entries: its lists are tested in full until the
+        // next rebuild
+        broke = true;
+        M.nbc[p] = in.nc | NB_BROKE;
+    }
+    const uint32_t bond_mask = (!BONDS || broke || nold > (uint32_t)MERGE_LDS) ? 0u : NB_BOND;
     if (reuse) {
         // the first MERGE_LDS touching partners go to lt, later ones (when there are more) to
         // the touching row, which then also takes a copy of the first ones
@@ -962,26 +989,30 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
 #pragma unroll
                 for (int u = 0; u < 4; ++u) w4[u] = m0 + u < nn ? M.nb[(int64_t)(m0 + u) * n + p] : 0u;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) o4[u] = m0 + u < nn ? M.pos[w4[u] & IDX_MASK] : mo;
+                for (int u = 0; u < 4; ++u)
+                    o4[u] = (m0 + u < nn && !(w4[u] & bond_mask)) ? M.pos[w4[u] & IDX_MASK] : mo;
 #pragma unroll
                 for (int u = 0; u < 4; ++u)
-                    if (m0 + u < nn && touching_pair(S, mo, o4[u], (w4[u] & IS_I) != 0u)) take(w4[u]);
+                    if (m0 + u < nn && !(w4[u] & bond_mask) && touching_pair(S, mo, o4[u], (w4[u] & IS_I) != 0u))
+                        take(w4[u] & ~NB_BOND);
             }
           }
         } else if (nn) {
             const SceneP& S = scene_of<SEG>(P, p);
             const double4 mo = M.pos[p];
-            // entry m + 1's word and partner position are loaded while entry m is tested
+            // entry m + 1's word and partner position are loaded while entry m is tested (no
+            // position for an intact bond's entry, which is not tested)
             uint32_t w = M.nb[p];
-            double4 o = M.pos[w & IDX_MASK];
+            double4 o = (w & bond_mask) ? mo : M.pos[w & IDX_MASK];
             for (uint32_t m = 0; m < nn; ++m) {
                 uint32_t wn = 0u;
                 double4 on = mo;
                 if (m + 1 < nn) {
                     wn = M.nb[(int64_t)(m + 1) * n + p];
-                    on = M.pos[wn & IDX_MASK];
+                    if (!(wn & bond_mask)) on = M.pos[wn & IDX_MASK];
                 }
-                if (touching_pair(S, mo, o, (w & IS_I) != 0u)) {
+                if (!(w & bond_mask) && touching_pair(S, mo, o, (w & IS_I) != 0u)) {
+                    w &= ~NB_BOND;
                     if (w & IS_I) ++t.found;
                     if (nt < (uint32_t)MERGE_LDS) {
                         lt[nt * BS] = w;
@@ -1009,7 +1040,6 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
         }
         uint32_t overflow = 0;
         if ((int)nt > K) { overflow = 1; nt = K; }
-        const uint32_t* pqo = b ? M.pq1 : M.pq0;
         uint64_t entries = 0;
         if (nt <= (uint32_t)MERGE_LDS && nold <= (uint32_t)MERGE_LDS) {
             for (uint32_t m = 0; m < (t_lds ? 0u : nt); m += 8) {
@@ -1019,7 +1049,7 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
 #pragma unroll
                 for (int u = 0; u < 8; ++u) if (m + u < nt) lt[(m + u) * BS] = v[u];
             }
-            for (uint32_t m = 0; m < nold; m += 8) {
+            for (uint32_t m = 0; m < (lw_staged ? 0u : nold); m += 8) {
                 uint32_t v[8];
 #pragma unroll
                 for (int u = 0; u < 8; ++u) v[u] = m + u < nold ? pqo[(int64_t)(m + u) * n + p] : 0u;
@@ -1129,8 +1159,14 @@ __device__ __forceinline__ void merge_counters(unsigned long long* ctr, bool dia
 // in contact-free mode (the sparse single-scene variant only): no particle has a row, the force
 // kernel evaluates none, and a particle that would get one fails the step instead (CT_ABORT bit
 // 2), which the host replays with the full force kernel.
-template <bool INIT, bool SEG, int BS>
+// BONDS: the scene's bond pass ran, so lean steps stage the old row first and leave the
+// near-list entries of intact bonds untested.  Those variants are held to five waves per SIMD
+// (four for the single-scene one-wave blocks, whose walk loads four near-list entries at a
+// time): the stage and the marks need a few registers more, and the merges are bound by their
+// chains of dependent loads.
+template <bool INIT, bool SEG, int BS, bool BONDS = false>
 __global__ void __launch_bounds__(BS)
+__attribute__((amdgpu_waves_per_eu(!BONDS ? 1 : (SEG || BS != 64 ? 5 : 4))))
 k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, int skin_cur, int lean,
         int free) {
     __shared__ uint32_t lt[MERGE_LDS * BS];  // touching partners
@@ -1146,7 +1182,7 @@ k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, in
         const MergeStep st = merge_step_begins(ctr, skin_cur, near, lean != 0, BS);
         if (!st.go) return;
         MergeTally t;
-        if (in) merge_particle<INIT, SEG, BS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse, st.twoD, t);
+        if (in) merge_particle<INIT, SEG, BS, BONDS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse, st.twoD, t);
         merge_counters(ctr, diag != 0, st.reuse, t);
         if constexpr (!SEG)
             if ((threadIdx.x & 63) == 0 && p64 < P.n) M.gskip[p64 >> 6] = 0u;
@@ -1168,13 +1204,48 @@ k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, in
         }
         MergeTally t;
         bool idle = true;
-        if (in) idle = merge_particle<INIT, SEG, BS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse,
+        if (in) idle = merge_particle<INIT, SEG, BS, BONDS>(P, M, p, mi, lt + threadIdx.x, lw + threadIdx.x, st.reuse,
                                                      st.twoD, t, free != 0);
         merge_counters(ctr, diag != 0, st.reuse, t, free != 0);
         // the group's value for the next lean steps (gmin stays until the next rebuild)
         float v = !in ? __builtin_inff() : (idle && near && mi.gm > 0.0f) ? mi.gm : 0.0f;
         for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
         if ((threadIdx.x & 63) == 0 && gin) M.gskip[g] = __float_as_uint(v);
+    }
+}
+
+// A rebuilding step's near lists, after its merge: mark each entry whose partner is an intact
+// bond in the particle's new row (NB_BOND), so that lean merges leave it untested.  Rows of more
+// than MERGE_LDS entries are not marked (all their entries are tested).  Launched on every full
+// step with near lists; it leaves at once unless the step rebuilt them.
+template <bool SEG>
+__global__ void __launch_bounds__(TPB)
+k_mark_bonds(const Params P, const MergeArgs M, const unsigned long long* ctr, int skin_cur) {
+    __shared__ uint32_t rw[MERGE_LDS * TPB];  // the new row, one column per thread
+    if (!skin_rebuilds(ctr, skin_cur)) return;
+    const int64_t p64 = (int64_t)xcd_map<SEG>(blockIdx.x, gridDim.x) * TPB + threadIdx.x;
+    if (p64 >= P.n) return;
+    const uint32_t p = (uint32_t)p64;
+    const int64_t n = P.n;
+    const uint32_t nc = min(M.nbc[p] & ~NB_BROKE, (uint32_t)P.KN);
+    const uint8_t sl = M.sel[p];
+    const int b = (sl & ROW_PENDING) ? ((sl & 1) ^ 1) : (sl & 1);  // the row the force kernel commits
+    const uint32_t cnt = b ? M.rcnt1[p] : M.rcnt0[p];
+    if (!nc || !cnt || cnt > (uint32_t)MERGE_LDS) return;
+    const uint32_t* pq = b ? M.pq1 : M.pq0;
+    uint32_t* r = rw + threadIdx.x;
+    for (uint32_t m = 0; m < cnt; ++m) r[m * TPB] = pq[(int64_t)m * n + p];
+    for (uint32_t m = 0; m < nc; ++m) {
+        const int64_t e = (int64_t)m * n + p;
+        const uint32_t w = M.nb[e];
+        const uint32_t q = w & IDX_MASK;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            const uint32_t eo = r[k * TPB];
+            if ((eo & IDX_MASK) == q) {
+                if ((eo & BONDED) && !(eo & BROKEN)) M.nb[e] = w | NB_BOND;
+                break;
+            }
+        }
     }
 }
 
@@ -2266,15 +2337,27 @@ static hipError_t detect_impl(dem_scene* s, bool init, bool last) {
     if (init) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<true, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
                                               s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0, 0));
+    } else if (s->force_tpb == 64 && s->initialized) {
+        // contact-dense, as the force kernel's block choice: one-wave blocks for uneven rows; a
+        // scene whose bond pass ran keeps its near-list bond marks current (a step that does not
+        // rebuild but walks the lists sees bonds that broke)
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64, true>), grid_for(P.n, 64), dim3(64), lds,
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0, 0));
     } else if (s->force_tpb == 64) {
-        // contact-dense, as the force kernel's block choice: one-wave blocks for uneven rows
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), lds,
                                               s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0, 0));
+    } else if (s->initialized) {
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB, true>), gm, dim3(MERGE_TPB), lds,
+                                              s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0,
+                                              s->force_free ? 1 : 0));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), gm, dim3(MERGE_TPB), lds,
                                               s->stream, P, merge_args(s), d.ctr, diag, skin_cur, 0,
                                               s->force_free ? 1 : 0));
     }
+    if (skin_cur >= 0 && s->initialized)  // a rebuilding step marks the near lists' intact bonds
+        LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_mark_bonds<SEG>), g, dim3(TPB), 0, s->stream, P, merge_args(s),
+                                              d.ctr, skin_cur));
     if (count_candidates && dense)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL((k_examined_count<SEG>), g, dim3(TPB), 0, s->stream, P,
                                               d.sidx, d.skey, d.spos32, d.bstart, d.ctr));
@@ -2344,13 +2427,23 @@ static hipError_t lean_impl(dem_scene* s, bool last) {
     const int diag = last ? 1 : 0;
     if (last)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_clear_step_counters, dim3(1), dim3(TPB), 0, s->stream, d.ctr));
-    if (s->force_tpb == 64) {
+    // a scene whose bond pass ran leaves the near-list entries of intact bonds untested
+    const bool bonds = s->initialized;
+    const int fr = s->force_free ? 1 : 0;
+    if (s->force_tpb == 64 && bonds) {
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64, true>), grid_for(P.n, 64), dim3(64), 0,
+                                              s->stream, P, merge_args(s), d.ctr, diag, c, 1, 0));
+    } else if (s->force_tpb == 64) {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, 64>), grid_for(P.n, 64), dim3(64), 0,
                                               s->stream, P, merge_args(s), d.ctr, diag, c, 1, 0));
+    } else if (bonds) {
+        LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB, true>), grid_for(P.n, MERGE_TPB),
+                                              dim3(MERGE_TPB), 0, s->stream, P, merge_args(s), d.ctr, diag,
+                                              c, 1, fr));
     } else {
         LAUNCH(s, K_MERGE, hipLaunchKernelGGL((k_merge<false, SEG, MERGE_TPB>), grid_for(P.n, MERGE_TPB),
                                               dim3(MERGE_TPB), 0, s->stream, P, merge_args(s), d.ctr, diag,
-                                              c, 1, s->force_free ? 1 : 0));
+                                              c, 1, fr));
     }
     return force_impl<SEG>(s, last);
 }
