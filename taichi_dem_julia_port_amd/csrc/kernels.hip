@@ -876,6 +876,7 @@ constexpr int MERGE_LDS = 16;
 struct MergeArgs {
     const double4* __restrict__ pos;  // positions of the step (the current buffer)
     uint32_t* __restrict__ nbc; uint32_t* __restrict__ nb; const uint32_t* __restrict__ gmin;
+    uint32_t* __restrict__ gmin_w;  // the same array, written by k_mark_bonds
     uint32_t* __restrict__ tcnt; uint32_t* __restrict__ touch;
     uint8_t* __restrict__ sel;
     uint32_t* __restrict__ rcnt0; uint32_t* __restrict__ rcnt1;
@@ -962,7 +963,9 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
         // the first MERGE_LDS touching partners go to lt, later ones (when there are more) to
         // the touching row, which then also takes a copy of the first ones
         nt = 0;
-        const uint32_t nn = gm > twoD ? 0u : min(cnt_in, (uint32_t)P.KN);
+        // gmin skips the walk; with a broken bond (bond_mask 0 on a bonded scene's lean step) it
+        // may be the smallest gap of the unmarked entries only
+        const uint32_t nn = (gm > twoD && !(BONDS && broke)) ? 0u : min(cnt_in, (uint32_t)P.KN);
         auto take = [&](uint32_t w) {  // a touching near-list entry, in list order
             if (w & IS_I) ++t.found;
             if (nt < (uint32_t)MERGE_LDS) {
@@ -1215,9 +1218,11 @@ k_merge(const Params P, const MergeArgs M, unsigned long long* ctr, int diag, in
 }
 
 // A rebuilding step's near lists, after its merge: mark each entry whose partner is an intact
-// bond in the particle's new row (NB_BOND), so that lean merges leave it untested.  Rows of more
-// than MERGE_LDS entries are not marked (all their entries are tested).  Launched on every full
-// step with near lists; it leaves at once unless the step rebuilt them.
+// bond in the particle's new row (NB_BOND), so that lean merges leave it untested, and make the
+// particle's smallest near-list gap (gmin) that of its unmarked entries, with the broad phase's
+// rounding (a lean merge whose particle has a broken bond walks its list whatever gmin says).
+// Rows of more than MERGE_LDS entries are not marked (all their entries are tested).  Launched
+// on every full step with near lists; it leaves at once unless the step rebuilt them.
 template <bool SEG>
 __global__ void __launch_bounds__(TPB)
 k_mark_bonds(const Params P, const MergeArgs M, const unsigned long long* ctr, int skin_cur) {
@@ -1235,18 +1240,32 @@ k_mark_bonds(const Params P, const MergeArgs M, const unsigned long long* ctr, i
     const uint32_t* pq = b ? M.pq1 : M.pq0;
     uint32_t* r = rw + threadIdx.x;
     for (uint32_t m = 0; m < cnt; ++m) r[m * TPB] = pq[(int64_t)m * n + p];
+    uint32_t marked = 0u, gb = 0x7f800000u;  // +inf
+    const double4 mo = M.pos[p];
     for (uint32_t m = 0; m < nc; ++m) {
         const int64_t e = (int64_t)m * n + p;
         const uint32_t w = M.nb[e];
         const uint32_t q = w & IDX_MASK;
+        bool bond = false;
         for (uint32_t k = 0; k < cnt; ++k) {
             const uint32_t eo = r[k * TPB];
             if ((eo & IDX_MASK) == q) {
-                if ((eo & BONDED) && !(eo & BROKEN)) M.nb[e] = w | NB_BOND;
+                bond = (eo & BONDED) && !(eo & BROKEN);
                 break;
             }
         }
+        if (bond) {
+            M.nb[e] = w | NB_BOND;
+            ++marked;
+        } else {  // the pair's gap, rounded down, as the broad phase takes it
+            const double4 ov = M.pos[q];
+            const double ex = mo.x - ov.x, ey = mo.y - ov.y, ez = mo.z - ov.z;
+            const double dd = sqrt(ex * ex + ey * ey + ez * ez), rr = mo.w + ov.w;
+            const double gap = (dd - rr) - 4e-16 * (dd + rr);
+            gb = min(gb, __float_as_uint(gap > 0.0 ? __double2float_rd(gap) : 0.0f));
+        }
     }
+    if (marked) M.gmin_w[p] = gb;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2250,7 +2269,8 @@ static hipError_t sort_particles(dem_scene* s, bool reset_counters, bool dense) 
 static MergeArgs merge_args(dem_scene* s) {
     DevState& d = s->d;
     MergeArgs M;
-    M.pos = d.pos[s->cur]; M.nbc = d.nbc; M.nb = d.nb; M.gmin = d.gmin; M.gskip = d.gskip; M.tcnt = d.tcnt;
+    M.pos = d.pos[s->cur]; M.nbc = d.nbc; M.nb = d.nb; M.gmin = d.gmin; M.gmin_w = d.gmin; M.gskip = d.gskip;
+    M.tcnt = d.tcnt;
     M.touch = d.touch; M.sel = d.sel; M.rcnt0 = d.rcnt[0]; M.rcnt1 = d.rcnt[1];
     M.pq0 = d.pq[0]; M.pq1 = d.pq[1]; M.st0 = d.st[0]; M.st1 = d.st[1];
     return M;
