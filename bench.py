@@ -25,7 +25,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str, free: bool = False) -> float:
+def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str, free: bool = False,
+                       near: bool = True) -> float:
     """Compulsory HBM bytes of one launch of ``kernel`` in this engine's design (each array the
     kernel must touch, read or written once):
       hash            previous order 4 B + position 32 B read; bucket, rank 8 B + FP32 position
@@ -37,7 +38,9 @@ def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str, free: b
                       cell; per touching pair both FP64 positions 64 B, the partner's index 4 B and
                       the two row appends (4 B counter + 4 B entry each)
       merge           touch count 4 B + row count and selector 5 B per grain
-      force_integrate grain state read 128 B + written 128 B, row counts 5 B, wall flags 4 B;
+      force_integrate grain state read 128 B + written 128 B, row counts 5 B, wall flags 4 B, with
+                      near lists the FP32 rebuild position 16 B (read for the displacement check,
+                      written on a rebuilding step);
                       per Hertz contact 2 rows x (partner 4 + spring 24 read + 24 written),
                       per bond 2 rows x (partner 4 + F,ti,tj 72 read + 72 written), per wall
                       contact 32 B (contact points and local forces are written only on a call's
@@ -50,7 +53,8 @@ def _algorithmic_bytes(kernel: str, c: dict, nwall: int, precision: str, free: b
     if kernel == "broad_phase":
         return n * 24.0 + 4.0 * cells + touching * (64 + 4 + 16)
     if kernel == "force_integrate":
-        return n * (128 + 128 + (0 if free else 5) + 4) + 2 * Ch * (4 + 48) + 2 * Cb * (4 + 144) + 32.0 * W
+        return (n * (128 + 128 + (0 if free else 5) + 4 + (16 if near else 0)) + 2 * Ch * (4 + 48)
+                + 2 * Cb * (4 + 144) + 32.0 * W)
     if kernel == "hash":
         return n * 60.0
     if kernel == "scatter":
@@ -288,7 +292,7 @@ def run_serving(args):
     print(json.dumps(out), flush=True)
 
 
-def summarize(r, workload, precision, nwall, n, steps):
+def summarize(r, workload, precision, nwall, n, steps, near=True):
     """Throughput, per-kernel durations and the dominant kernel's roofline of one engine run."""
     kt = r["kernels"]
     step_ms = {k: (ms / max(l, 1)) for k, (ms, l) in kt.items() if l}
@@ -297,7 +301,7 @@ def summarize(r, workload, precision, nwall, n, steps):
     c = r["counters"]
     # a single scene whose state has no contact runs the contact-free force kernel
     free = workload != "batch256" and c["contacts"] == 0
-    alg = _algorithmic_bytes(dom, c, nwall, precision, free)
+    alg = _algorithmic_bytes(dom, c, nwall, precision, free, near)
     achieved = alg / (step_ms[dom] * 1e-3) / 1e9 if step_ms[dom] > 0 else 0.0
     traffic, tsrc = measured_traffic(workload, precision, n)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
@@ -315,7 +319,7 @@ def summarize(r, workload, precision, nwall, n, steps):
             _reference_equivalent_bytes(c) / (step_ms[dom] * 1e-3) / 1e9, 1)
     per_kernel = {}
     for k, ms in step_ms.items():
-        b = _algorithmic_bytes(k, c, nwall, precision, free)
+        b = _algorithmic_bytes(k, c, nwall, precision, free, near)
         if b > 0 and ms > 0:
             per_kernel[k] = round(b / (ms * 1e-3) / 1e9, 1)
     return {"value": n * steps / r["elapsed"],
@@ -391,7 +395,7 @@ def main():
     near = not args.no_near_lists
     r = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup,
                    graph=not args.no_graph, near_lists=near)
-    m = summarize(r, args.workload, args.precision, len(w), n, args.steps)
+    m = summarize(r, args.workload, args.precision, len(w), n, args.steps, near)
     c = r["counters"]
     out = {
         "metric": "particle-steps/s (1M grains, FP32 vs FP64)",
@@ -426,7 +430,7 @@ def main():
         # near lists costs about this much
         r0 = run_engine(cfg, g, w, bonded, args.precision, args.steps, args.warmup,
                         graph=not args.no_graph, near_lists=False)
-        m0 = summarize(r0, args.workload, args.precision, len(w), n, args.steps)
+        m0 = summarize(r0, args.workload, args.precision, len(w), n, args.steps, near=False)
         out["every_step_rebuild"] = {"value": round(m0["value"], 1), "ms_per_step": m0["ms_per_step"],
                                      "kernel_ms_per_launch": m0["kernel_ms_per_launch"],
                                      "near_list_speedup": round(m["value"] / m0["value"], 3)}
@@ -440,7 +444,7 @@ def main():
         other = "mixed" if args.precision == "fp64" else "fp64"
         r2 = run_engine(cfg, g, w, bonded, other, args.steps, args.warmup,
                         graph=not args.no_graph, near_lists=near)
-        m2 = summarize(r2, args.workload, other, len(w), n, args.steps)
+        m2 = summarize(r2, args.workload, other, len(w), n, args.steps, near)
         out[f"value_{other}"] = round(m2["value"], 1)
         out[f"ms_per_step_{other}"] = m2["ms_per_step"]
         out[f"{other}_speedup_over_{args.precision}"] = round(m2["value"] / m["value"], 3)
@@ -456,7 +460,7 @@ def main():
         for prec in ("fp64", "mixed"):
             rp = run_engine(cfg_p, g_p, w_p, False, prec, args.steps, args.warmup,
                             graph=not args.no_graph, near_lists=near)
-            mp = summarize(rp, "pour_packed", prec, len(w_p), len(g_p), args.steps)
+            mp = summarize(rp, "pour_packed", prec, len(w_p), len(g_p), args.steps, near)
             heavy[prec] = {"value": round(mp["value"], 1), "ms_per_step": mp["ms_per_step"],
                            "kernel_ms_per_launch": mp["kernel_ms_per_launch"],
                            "roofline": mp["roofline"],
