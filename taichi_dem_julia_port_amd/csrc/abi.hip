@@ -1248,6 +1248,9 @@ static int32_t enqueue_steps(dem_scene* s, int64_t nsteps, bool call_end, bool l
             HIPCHK(s, full_step(s, false));
             ++t;
         }
+        // the body's lean steps of a single scene in the contact-free mode are one launch each,
+        // whose run starts by clearing the next step's words
+        if (s->force_free && !s->batched && t < body_end) HIPCHK(s, run_lean_free_prologue(s));
         if (graphs) {
             const int64_t nlong = (body_end - t) / GRAPH_LONG_STEPS;
             int32_t rc = replay_graphs(s, true, true, nlong);

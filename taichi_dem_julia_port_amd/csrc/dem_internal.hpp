@@ -56,8 +56,12 @@ enum CounterIdx {
                            // call did nothing (copy 0; CT_FAIL is set with it)
     CT_SKIN = 20,          // CT_SKIN + c: the step of buffer parity c rebuilds (copy 0; two words)
     CT_NCOUNTERS = 22,     // the host sums copies of the counters below this index
-    CT_DISP = 22           // CT_DISP + c: FP32 bits of a bound on every particle's displacement since
+    CT_DISP = 22,          // CT_DISP + c: FP32 bits of a bound on every particle's displacement since
                            // the last rebuild, for the step of parity c (the maximum over all copies)
+    CT_TICKET = 24,        // a one-kernel lean step: its blocks done so far, per class of blocks
+                           // (copy c: blocks b % CTR_COPIES == c), and the classes done (copy 0,
+                           // CT_TICKET + 2)
+    CT_MDONE = 25          // the same for its merge blocks (classes done: copy 0, CT_MDONE + 2)
 };
 
 // Counters are kept in CTR_COPIES copies, 256 B apart, and a block adds to copy blockIdx % CTR_COPIES:
@@ -360,6 +364,7 @@ hipError_t run_force_integrate(dem_scene* s, bool write_forces);
 // one lean step: the merge over the near lists (or the abort) and the force kernel; last: the
 // last step of a call, which counts the per-step counters and writes the outputs
 hipError_t run_lean_step(dem_scene* s, bool last = false);
+hipError_t run_lean_free_prologue(dem_scene* s);
 // after a bond pass or a failed step: commit the pending rows (rollback = false), or drop them,
 // clear the touching counts and the failure flags (rollback = true)
 hipError_t run_rows_finish(dem_scene* s, bool rollback);

@@ -1086,7 +1086,7 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
     }
     // no row and no touching partner: nothing changed (only the variant that keeps group values
     // asks; computing it costs the others registers)
-    if constexpr (!INIT && !SEG && BS == MERGE_TPB) return (nt | nold) == 0u;
+    if constexpr (!INIT && !SEG && BS >= MERGE_TPB) return (nt | nold) == 0u;  // sparse single-scene merges
     else return false;
 }
 
@@ -1095,9 +1095,12 @@ __device__ __forceinline__ bool merge_particle(const Params& P, const MergeArgs&
 // of the call, a step that has to rebuild fails the call instead, otherwise the step runs and
 // block 0 clears the next step's words; whether the step reuses the near lists; and twice the
 // displacement bound D of the step (rounded up, the maximum over the counter copies).
+// clear_next false (the one-kernel lean step): the next step's words are left alone, since force
+// blocks of the same launch write them.
 struct MergeStep { bool go, reuse; float twoD; };
 __device__ __forceinline__ MergeStep merge_step_begins(unsigned long long* ctr, int skin_cur,
-                                                       bool near, bool lean, int bs) {
+                                                       bool near, bool lean, int bs,
+                                                       bool clear_next = true) {
     __shared__ int s_go, s_reuse;
     __shared__ float s_disp;
     if (threadIdx.x < 64) {
@@ -1115,7 +1118,7 @@ __device__ __forceinline__ MergeStep merge_step_begins(unsigned long long* ctr, 
                     atomicOr(&ctr[CT_ABORT], 1ull);
                     atomicOr(&ctr[CT_FAIL], 1ull);
                     go = 0;
-                } else if (blockIdx.x == 0) {
+                } else if (blockIdx.x == 0 && clear_next) {
                     ctr[CT_SKIN + 1 - skin_cur] = 0ull;
                 }
             }
@@ -1129,7 +1132,7 @@ __device__ __forceinline__ MergeStep merge_step_begins(unsigned long long* ctr, 
     m.go = s_go != 0;
     m.reuse = s_reuse != 0;
     m.twoD = 2.0f * s_disp * (1.0f + 1e-6f);
-    if (m.go && lean && blockIdx.x == 0)
+    if (m.go && lean && blockIdx.x == 0 && clear_next)
         for (int c = threadIdx.x; c < CTR_COPIES; c += bs)
             ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
     return m;
@@ -1781,17 +1784,54 @@ __device__ __forceinline__ Own load_own(const SceneP& S, const ForceArgs& A, uin
     return o;
 }
 
+// The one-kernel lean step (k_lean_free): every merge block of the step is done and the step has
+// not failed, so a force wave may write wall history.  The wait is bounded: merge blocks precede
+// the force blocks in the grid and finish without waiting on anything, and on time-out the step
+// fails (CT_ABORT bit 2), which the host replays with separate launches.
+// The flags and counts are device-scope atomics throughout (a merge wave's flag atomics have
+// completed before its block is counted done), so no fence is needed; an agent-scope fence here
+// writes back and invalidates the XCD's L2, which cost the step 18 times its time.
+constexpr int GATE_SPIN = 4096;
+__device__ __forceinline__ bool merge_gate(unsigned long long* ctr, uint32_t nmb) {
+    const uint32_t ncls = nmb < (uint32_t)CTR_COPIES ? nmb : (uint32_t)CTR_COPIES;
+    for (int i = 0; i < GATE_SPIN; ++i) {
+        if (__hip_atomic_load(&ctr[CT_MDONE + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) >= ncls)
+            return __hip_atomic_load(&ctr[CT_FAIL], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull;
+        __builtin_amdgcn_s_sleep(4);
+    }
+    atomicOr(&ctr[CT_ABORT], 2ull);
+    atomicOr(&ctr[CT_FAIL], 1ull);
+    return false;
+}
+
+// Block b of nblocks is done: counted in class b % CTR_COPIES (that copy's word idx), since atomics
+// on one address serialise (one count per block of a million-particle launch cost the step four
+// times its time); the block that completes its class resets the class's word and counts the
+// class in copy 0's word idx + 2.  True for the block that completes the last class.
+__device__ __forceinline__ bool count_done(unsigned long long* ctr, int idx, uint32_t b, uint32_t nblocks) {
+    const uint32_t c = b % (uint32_t)CTR_COPIES;
+    const uint32_t ncls = nblocks < (uint32_t)CTR_COPIES ? nblocks : (uint32_t)CTR_COPIES;
+    const uint32_t in_cls = (nblocks - c + CTR_COPIES - 1) / CTR_COPIES;
+    unsigned long long* w = ctr + (size_t)c * CTR_STRIDE + idx;
+    if (__hip_atomic_fetch_add(w, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != in_cls - 1) return false;
+    __hip_atomic_store(w, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_fetch_add(&ctr[idx + 2], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == ncls - 1;
+}
+
 // Contacts (kernels.jl:122-315), walls (kernels.jl:317-426), body force and update! of particle p,
 // whose own state `own` is loaded.  failed: a row of this step or an earlier one overflowed, so
 // nothing changes (tested before the first store, so the flag's load hides behind the state
 // loads); rebuilt: near lists in use (skin_cur >= 0) and this step rebuilt them.  FREE: the
 // contact-free mode, in which no particle has a row (the merge fails a step that would give one a
-// row), so the row is neither read nor walked and the kernel needs few registers.
-template <bool MIXED, bool SEG, bool LATE, bool WALLS_LATE, bool FREE = false>
+// row), so the row is neither read nor walked and the kernel needs few registers.  GATE: the force
+// blocks of a one-kernel lean step, whose wave writes wall history only once the step's gate_n
+// merge blocks are done and the step has not failed.
+template <bool MIXED, bool SEG, bool LATE, bool WALLS_LATE, bool FREE = false, bool GATE = false>
 __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs& A, uint32_t p,
                                                const Own& own, int write_forces, int skin_cur,
                                                bool failed, bool rebuilt,
-                                               unsigned long long* ctr, ForceTally& t) {
+                                               unsigned long long* ctr, ForceTally& t,
+                                               uint32_t gate_n = 0u) {
     const int64_t n = P.n;
     const SceneP& S = scene_of<SEG>(P, p);
     const double dt = S.dt;
@@ -1878,6 +1918,8 @@ __device__ __forceinline__ void force_particle(const Params& P, const ForceArgs&
     // walls (kernels.jl:317-426): two-sided planes, shear history reset when not touching
     if (LATE && S.nwall) wa = A.wact[p];
     const uint32_t wtouch = WALLS_LATE ? walls_touched(S, kp, X.w) : wtouch_early;
+    if constexpr (GATE)  // wall history is written in place (a failed step must leave it)
+        if (__ballot(wtouch != 0u || wa != 0u) != 0ull && !merge_gate(ctr, gate_n)) return;
     for (uint32_t wm = wtouch; wm; wm &= wm - 1u) {  // in wall order
         const int j = __ffs(wm) - 1;
         const WallP& wl = S.walls[j];
@@ -1965,6 +2007,110 @@ k_force(const Params P, const ForceArgs A, unsigned long long* ctr, int write_fo
     }
     if (failed) return;
     force_counters(ctr, skin_cur, rebuilt, write_forces, true, t);
+}
+
+// A lean step of a single scene in the contact-free mode, as one launch.  The contact-free force
+// kernel reads nothing the merge writes (no row), so the first nmb blocks test the near lists as
+// the lean merge does (a wave per 64-grain group, groups left out by their gskip value) while the
+// other blocks run the force kernel: the merge's launch and chains of dependent loads overlap the
+// state streaming instead of preceding it.  What the order of two launches gave is kept:
+// * a merge that finds a contact fails the step (CT_ABORT bit 2 with CT_FAIL) after force blocks
+//   may have written the next state; that buffer is not the step's input, so the host's replay
+//   from the step starts from its state unchanged.  Wall history is written in place, so waves
+//   with a wall contact pass merge_gate first;
+// * the step is counted (CT_DONE) by the block that finishes last, if the step did not fail;
+// * merge blocks read this step's rebuild flag and displacement bound while force blocks write
+//   the next step's, so no block clears the next step's words: the last block clears this
+//   step's (those of the step after next; a failed step keeps them for its replay), and a run of
+//   these steps starts with k_lean_free_prologue.
+// The grid is nmb merge blocks, then force blocks (both counts multiples of 8, so force block f
+// runs on XCD f % 8 as xcd_block assumes).  The merge blocks are few (at most 256): wave w of them
+// takes the groups g = w (mod its count), reads 64 of their gskip values at once and merges the
+// groups that stay one after another, so the merge holds few of the slots the force blocks stream
+// the state with.
+template <bool MIXED>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3)))
+k_lean_free(const Params P, const MergeArgs M, const ForceArgs A, unsigned long long* ctr, int skin_cur,
+            uint32_t nmb) {
+    __shared__ uint32_t lt[MERGE_LDS * TPB];
+    __shared__ uint32_t lw[MERGE_LDS * TPB];
+    __shared__ int s_last;
+    if (blockIdx.x < nmb) {
+        const MergeStep st = merge_step_begins(ctr, skin_cur, true, true, TPB, false);
+        if (st.go) {
+            const int lane = threadIdx.x & 63;
+            const uint32_t nw = nmb * (TPB / 64), w = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+            const uint32_t ng = (uint32_t)((P.n + 63) / 64);
+            for (uint32_t g0 = w; g0 < ng; g0 += 64u * nw) {
+                const uint64_t gl = (uint64_t)g0 + (uint64_t)lane * nw;
+                const float gs = gl < ng ? __uint_as_float(M.gskip[gl]) : __builtin_inff();
+                for (uint64_t stay = __ballot(gl < ng && gs <= st.twoD); stay; stay &= stay - 1ull) {
+                    const uint32_t g = g0 + (uint32_t)(__ffsll((unsigned long long)stay) - 1) * nw;
+                    const int64_t p64 = ((int64_t)g << 6) + lane;
+                    const bool in = p64 < P.n;
+                    const uint32_t p = in ? (uint32_t)p64 : 0u;
+                    const MergeIn mi = load_merge_in(M, p, true, true);
+                    MergeTally t;
+                    bool idle = true;
+                    if (in)
+                        idle = merge_particle<false, false, TPB, false>(P, M, p, mi, lt + threadIdx.x,
+                                                                        lw + threadIdx.x, true, st.twoD, t, true);
+                    merge_counters(ctr, false, true, t, true);
+                    float v = !in ? __builtin_inff() : (idle && mi.gm > 0.0f) ? mi.gm : 0.0f;
+                    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+                    if (lane == 0) M.gskip[g] = __float_as_uint(v);
+                }
+            }
+        }
+        __builtin_amdgcn_s_waitcnt(0);  // this wave's flag atomics are done
+        __syncthreads();
+        if (threadIdx.x == 0) count_done(ctr, CT_MDONE, blockIdx.x, nmb);
+    } else {
+        const uint32_t f = blockIdx.x - nmb;
+        const int64_t p64 = (int64_t)xcd_block(f, gridDim.x - nmb) * TPB + threadIdx.x;
+        // a step that has to rebuild fails in its merge blocks
+        const bool failed = step_failed(ctr) || skin_rebuilds(ctr, skin_cur);
+        ForceTally t;
+        if (p64 < P.n) {
+            const uint32_t p = (uint32_t)p64;
+            const Own own = load_own<false>(scene_of<false>(P, p), A, p, skin_cur, false);
+            force_particle<MIXED, false, false, false, true, true>(P, A, p, own, 0, skin_cur, failed, false, ctr,
+                                                                   t, nmb);
+        }
+        if (!failed) force_counters(ctr, skin_cur, false, 0, false, t);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) s_last = count_done(ctr, CT_TICKET, blockIdx.x, gridDim.x);
+    __syncthreads();
+    if (s_last) {
+        if (__hip_atomic_load(&ctr[CT_FAIL], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0ull) {
+            for (int c = threadIdx.x; c < CTR_COPIES; c += TPB) ctr[c * CTR_STRIDE + CT_DISP + skin_cur] = 0ull;
+            if (threadIdx.x == 0) {
+                ctr[CT_SKIN + skin_cur] = 0ull;
+                atomicAdd(&ctr[CT_DONE], 1ull);
+            }
+        }
+        if (threadIdx.x == 0) {
+            ctr[CT_MDONE + 2] = 0ull;
+            ctr[CT_TICKET + 2] = 0ull;
+        }
+    }
+}
+
+// Before a run of one-kernel lean steps: the next step's words (a lean step of two launches
+// leaves them for its successor to clear) and the block counts.
+__global__ void __launch_bounds__(64) k_lean_free_prologue(unsigned long long* ctr, int skin_cur) {
+    for (int c = threadIdx.x; c < CTR_COPIES; c += 64) {
+        ctr[c * CTR_STRIDE + CT_DISP + 1 - skin_cur] = 0ull;
+        ctr[c * CTR_STRIDE + CT_TICKET] = 0ull;
+        ctr[c * CTR_STRIDE + CT_MDONE] = 0ull;
+    }
+    if (threadIdx.x == 0) {
+        ctr[CT_SKIN + 1 - skin_cur] = 0ull;
+        ctr[CT_TICKET + 2] = 0ull;
+        ctr[CT_MDONE + 2] = 0ull;
+    }
 }
 
 // After a bond pass (rollback = 0): commit the rows it rebuilt.  After a failed step
@@ -2438,6 +2584,31 @@ __global__ void __launch_bounds__(TPB) k_clear_step_counters(unsigned long long*
         ctr[(i / CT_NSTEP) * CTR_STRIDE + i % CT_NSTEP] = 0ull;
 }
 
+// One lean step of a single scene in the contact-free mode (k_lean_free): a merge block per four
+// 64-grain groups up to 256 blocks, then the force blocks, each count rounded up to a multiple of 8.
+static hipError_t launch_lean_free(dem_scene* s) {
+    const Params& P = s->P;
+    const uint32_t nb = (uint32_t)((P.n + TPB - 1) / TPB);
+    const uint32_t nfb = (nb + 7u) / 8u * 8u;
+    const uint32_t nmb = ((nb < 256u ? nb : 256u) + 7u) & ~7u;
+    const MergeArgs M = merge_args(s);
+    const ForceArgs A = force_args(s);
+    if (s->precision == DEM_PRECISION_MIXED) {
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_lean_free<true>), dim3(nmb + nfb), dim3(TPB), 0, s->stream, P, M, A,
+                                              s->d.ctr, s->cur, nmb));
+    } else {
+        LAUNCH(s, K_FORCE, hipLaunchKernelGGL((k_lean_free<false>), dim3(nmb + nfb), dim3(TPB), 0, s->stream, P, M,
+                                              A, s->d.ctr, s->cur, nmb));
+    }
+    s->cur = 1 - s->cur;
+    return hipSuccess;
+}
+
+hipError_t run_lean_free_prologue(dem_scene* s) {
+    LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_lean_free_prologue, dim3(1), dim3(64), 0, s->stream, s->d.ctr, s->cur));
+    return hipSuccess;
+}
+
 template <bool SEG>
 static hipError_t lean_impl(dem_scene* s, bool last) {
     DevState& d = s->d;
@@ -2447,6 +2618,8 @@ static hipError_t lean_impl(dem_scene* s, bool last) {
     const int diag = last ? 1 : 0;
     if (last)
         LAUNCH(s, K_OTHER, hipLaunchKernelGGL(k_clear_step_counters, dim3(1), dim3(TPB), 0, s->stream, d.ctr));
+    if constexpr (!SEG)
+        if (s->force_free && !last) return launch_lean_free(s);
     // a scene whose bond pass ran leaves the near-list entries of intact bonds untested
     const bool bonds = s->initialized;
     const int fr = s->force_free ? 1 : 0;

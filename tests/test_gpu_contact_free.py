@@ -8,7 +8,9 @@ mode, the steps where contacts appear, and after; and the scene matches the orac
 
 The sample is the pour sample at its own radii (no contacts) with speeds up to 0.5 m/s: no pair
 touches in the first call, so the second call starts contact-free, and the first contacts form
-some tens of steps later.
+some tens of steps later.  The floor sample moves the same grains down until the lowest press into
+the floor, so the contact-free steps also carry wall contacts (whose history a one-kernel lean step
+writes only once its merge blocks are done and the step has not failed).
 """
 import numpy as np
 import pytest
@@ -25,18 +27,29 @@ def _sample():
     return packed_sample(2000, scale=1.0, vmax=0.5, seed=5)
 
 
+def _floor_sample():
+    """The grains moved down until the lowest one's centre is 0.7 radii above the floor (z = 0)."""
+    cfg, g, w = _sample()
+    g = g.copy()
+    r = np.cbrt(3.0 * g["V"] / (4.0 * np.pi))
+    low = np.argmin(g["k"][:, 2] - r)
+    g["k"][:, 2] -= g["k"][low, 2] - 0.7 * r[low]
+    return cfg, g, w
+
+
 def _bits(x):
     x = np.ascontiguousarray(x)
     return x.view(np.uint64) if x.dtype == np.float64 else x
 
 
-@pytest.mark.parametrize("graph", [True, False])
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_contact_free_mode_equals_batched_scene(precision, graph):
+@pytest.mark.parametrize("precision,graph,floor", [
+    ("fp64", True, False), ("fp64", False, False), ("mixed", True, False), ("mixed", False, False),
+    ("fp64", True, True), ("mixed", False, True)])
+def test_contact_free_mode_equals_batched_scene(precision, graph, floor):
     from taichi_dem_julia_port_amd.executor import Batch, Scene
-    cfg, g, w = _sample()
-    print(f"[case] contact-free {precision} graph={graph} n={len(g)}", flush=True)
-    seen_free = seen_contacts = False
+    cfg, g, w = _floor_sample() if floor else _sample()
+    print(f"[case] contact-free {precision} graph={graph} floor={floor} n={len(g)}", flush=True)
+    seen_free = seen_contacts = seen_free_walls = False
     with Scene(cfg, g, w, precision=precision, graph=graph) as s, \
             Batch([(cfg, g, w)], precision=precision, graph=graph) as b:
         for k in (1, 4, 16, 40, 100, 200):
@@ -58,10 +71,13 @@ def test_contact_free_mode_equals_batched_scene(precision, graph):
             c = s.counters()
             assert c["contacts"] == b.counters()["contacts"]
             seen_free |= before == 0
+            seen_free_walls |= before == 0 and c["wall_contacts"] > 0
             seen_contacts |= c["contacts"] > 0
-            print(f"[case] step {c['step']}: {c['contacts']} contacts", flush=True)
+            print(f"[case] step {c['step']}: {c['contacts']} contacts, {c['wall_contacts']} at walls",
+                  flush=True)
     # a call started with no contact (in the mode) and contacts formed later (it was left)
     assert seen_free and seen_contacts
+    assert seen_free_walls or not floor
 
 
 def test_contact_free_mode_matches_the_oracle():
