@@ -12,6 +12,8 @@
 //              contact row -- the BitTable state machine of update_contacts! (kernels.jl:87-120)
 //   k_force    per-particle loop over its row: Hertz-Mindlin / EBPM (kernels.jl:122-315), walls
 //              (kernels.jl:317-426), body force (kernels.jl:428-443) and update! (445-485)
+// Between near-list rebuilds a step is k_merge (testing the near lists) and k_force; while a
+// single scene has no contact, one k_lean_free launch does both.
 //
 // Contact state lives in per-particle rows (column-major [slot][particle] arrays, sorted by
 // partner index).  Every contact appears in the rows of both its particles and both copies are
